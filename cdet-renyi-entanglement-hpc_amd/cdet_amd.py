@@ -1,0 +1,272 @@
+"""Host-side mirror of the reference's integrand API over the gfx950 library.
+
+The reference evaluates the CDet integrand in-process from C++ templates
+(include/cdet/cdet_integrand_evaluator.h:542-590).  This module exposes the
+same calls, with the same argument meaning and error behaviour, over the
+C-ABI of libcdet_hip.so (include/cdet_hip.h):
+
+  Context.evaluate_integrand(vertices)               -> float     (:542-560)
+  Context.evaluate_all_subset_integrands(vertices)   -> ndarray   (:562-590)
+  BatchPlan.run(d_vertices, d_out, d_status)                        (batched)
+
+Errors the reference raises as std::runtime_error are raised here as
+RuntimeError with the reference's message.  There is no CPU fallback: if the
+HIP library is missing, loading fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcdet_hip.so")
+
+MODE_EXACT = 0
+MODE_MFMA64 = 1
+MODE_SPLIT32 = 2
+MODES = {"exact": MODE_EXACT, "mfma64": MODE_MFMA64, "split32": MODE_SPLIT32}
+
+TWISTED_UP, TWISTED_DOWN, ORDINARY_UP, ORDINARY_DOWN = 0, 1, 2, 3
+MASK_KINDS = {"empty": 0, "checkerboard": 1, "first_row": 2, "first_column": 3,
+              "origin_square": 4, "left_columns": 5}
+
+_lib = None
+
+
+class CdetParams(ctypes.Structure):
+    """cdet_params (include/cdet_hip.h)."""
+
+    _fields_ = [
+        ("N1", ctypes.c_int32), ("N2", ctypes.c_int32),
+        ("cheb_nu", ctypes.c_int32), ("cheb_nv", ctypes.c_int32),
+        ("beta", ctypes.c_double), ("U", ctypes.c_double), ("mu", ctypes.c_double),
+        ("alpha_up", ctypes.c_double), ("alpha_down", ctypes.c_double),
+        ("branch_epsilon", ctypes.c_double), ("tol_fpm", ctypes.c_double),
+        ("fpm_global_shift", ctypes.c_double),
+        ("ztransform_epsilon0", ctypes.c_double), ("ztransform_safety", ctypes.c_double),
+        ("scale", ctypes.c_double), ("diag_up_shift", ctypes.c_double),
+        ("diag_down_shift", ctypes.c_double),
+    ]
+
+
+class CdetCounters(ctypes.Structure):
+    _fields_ = [("integrand_calls", ctypes.c_uint64), ("fpm_fallbacks", ctypes.c_uint64),
+                ("green_pair_cache_hits", ctypes.c_uint64), ("green_pair_cache_misses", ctypes.c_uint64)]
+
+
+class CdetLattice(ctypes.Structure):
+    _fields_ = [("pbc", ctypes.c_int32), ("mask_kind", ctypes.c_int32),
+                ("mask_param", ctypes.c_int32), ("node_chunk", ctypes.c_int32)]
+
+
+# Exported symbols and their signatures, as declared in include/cdet_hip.h.
+def _signatures():
+    P = ctypes.POINTER
+    i, i64, d, vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+    dp = P(ctypes.c_double)
+    return {
+        "cdet_last_error": (ctypes.c_char_p, []),
+        "cdet_params_derive": (i, [P(CdetParams)]),
+        "cdet_ctx_create": (i, [P(CdetParams), P(vp)]),
+        "cdet_ctx_destroy": (i, [vp]),
+        "cdet_ctx_stream": (vp, [vp]),
+        "cdet_tables_upload": (i, [vp, i, dp, dp, dp, dp]),
+        "cdet_tables_alias": (i, [vp, i, i]),
+        "cdet_tables_build": (i, [vp, P(CdetLattice)]),
+        "cdet_tables_download": (i, [vp, i, dp, dp, dp, dp]),
+        "cdet_eval_single": (i, [vp, dp, i, dp]),
+        "cdet_eval_all_subsets": (i, [vp, dp, i, dp]),
+        "cdet_batch_plan_create": (i, [vp, P(ctypes.c_int32), i64, i, i, P(vp)]),
+        "cdet_batch_plan_destroy": (i, [vp]),
+        "cdet_batch_out_size": (i64, [vp]),
+        "cdet_batch_vertex_count": (i64, [vp]),
+        "cdet_batch_query_count": (i64, [vp]),
+        "cdet_batch_run": (i, [vp, vp, vp, vp, i, vp]),
+        "cdet_batch_profile": (i, [vp, vp, vp, vp, vp, P(ctypes.c_float)]),
+        "cdet_eval_batch": (i, [vp, vp, P(ctypes.c_int32), i64, vp, vp, i, vp]),
+        "cdet_green_pair_eval": (i, [vp, vp, vp, i64, vp, i, vp]),
+        "cdet_get_counters": (i, [vp, P(CdetCounters)]),
+        "cdet_reset_counters": (i, [vp]),
+    }
+
+
+def lib():
+    """Load libcdet_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libcdet_hip.so is not built; run `python cdet-renyi-entanglement-hpc_amd/build.py` "
+                "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _signatures().items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _err():
+    return lib().cdet_last_error().decode()
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(_err())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_params(N1, N2, cheb, beta, U, mu, alpha_up, alpha_down, branch_epsilon=1e-13, tol_fpm=1e-7,
+                fpm_global_shift=-1.0, ztransform_epsilon0=-1.0, ztransform_safety=1e4, derived=None):
+    """cdet_params with the derived scalars filled (or taken from `derived`)."""
+    p = CdetParams(N1=N1, N2=N2, cheb_nu=cheb, cheb_nv=cheb, beta=beta, U=U, mu=mu,
+                   alpha_up=alpha_up, alpha_down=alpha_down, branch_epsilon=branch_epsilon,
+                   tol_fpm=tol_fpm, fpm_global_shift=fpm_global_shift,
+                   ztransform_epsilon0=ztransform_epsilon0, ztransform_safety=ztransform_safety)
+    _check(lib().cdet_params_derive(ctypes.byref(p)))
+    if derived is not None:
+        p.scale, p.diag_up_shift, p.diag_down_shift = derived
+    return p
+
+
+class Context:
+    """One device context: parameters, resident tables, scratch and stream."""
+
+    def __init__(self, params):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib().cdet_ctx_create(ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cdet_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return lib().cdet_ctx_stream(self._h)
+
+    # -- tables ------------------------------------------------------------
+    def upload_table(self, kind, upper, lower, ucorner, lcorner):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (upper, lower, ucorner, lcorner)]
+        _check(lib().cdet_tables_upload(self._h, kind, *[_dptr(a) for a in arrs]))
+
+    def alias_table(self, dst, src):
+        _check(lib().cdet_tables_alias(self._h, dst, src))
+
+    def upload_tables(self, tables):
+        """tables: four objects with .upper/.lower/.ucorner/.lcorner; identical
+        objects are uploaded once and aliased (load_data.h:70-91)."""
+        seen = {}
+        for kind, t in enumerate(tables):
+            if id(t) in seen:
+                self.alias_table(kind, seen[id(t)])
+            else:
+                self.upload_table(kind, t.upper, t.lower, t.ucorner, t.lcorner)
+                seen[id(t)] = kind
+
+    def build_tables(self, pbc=True, mask="first_row", mask_param=0, node_chunk=0):
+        kind = MASK_KINDS[mask] if isinstance(mask, str) else int(mask)
+        lat = CdetLattice(pbc=int(pbc), mask_kind=kind, mask_param=mask_param, node_chunk=node_chunk)
+        _check(lib().cdet_tables_build(self._h, ctypes.byref(lat)))
+
+    def download_table(self, kind):
+        p = self.params
+        S = p.N1 * p.N2
+        SP = 4 * S * S
+        nc = SP * p.cheb_nu * p.cheb_nv
+        up, lo, uc, lc = np.zeros(nc), np.zeros(nc), np.zeros(SP), np.zeros(SP)
+        _check(lib().cdet_tables_download(self._h, kind, _dptr(up), _dptr(lo), _dptr(uc), _dptr(lc)))
+        return up, lo, uc, lc
+
+    # -- integrand (reference API) ----------------------------------------
+    def evaluate_integrand(self, vertices):
+        """evaluate_cdet_density_integrand_double (cdet_integrand_evaluator.h:542-560)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros(1)
+        _check(lib().cdet_eval_single(self._h, _dptr(v), len(v), _dptr(out)))
+        return float(out[0])
+
+    def evaluate_all_subset_integrands(self, vertices):
+        """evaluate_cdet_density_all_subset_integrands_double (:562-590)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros(1 << len(v))
+        _check(lib().cdet_eval_all_subsets(self._h, _dptr(v), len(v), _dptr(out)))
+        return out
+
+    def counters(self):
+        c = CdetCounters()
+        _check(lib().cdet_get_counters(self._h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in CdetCounters._fields_}
+
+    def reset_counters(self):
+        _check(lib().cdet_reset_counters(self._h))
+
+    def plan(self, orders, mode="exact", single=False):
+        return BatchPlan(self, orders, mode, single)
+
+
+class BatchPlan:
+    """Order-bucketed batched evaluation over device buffers (torch tensors or
+    raw device pointers)."""
+
+    def __init__(self, ctx, orders, mode="exact", single=False):
+        self.ctx = ctx
+        self.orders = np.ascontiguousarray(orders, dtype=np.int32)
+        self.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+        h = ctypes.c_void_p()
+        _check(lib().cdet_batch_plan_create(
+            ctx.handle, self.orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            len(self.orders), self.mode, 1 if single else 0, ctypes.byref(h)))
+        self._h = h
+        self.out_size = lib().cdet_batch_out_size(h)
+        self.vertex_count = lib().cdet_batch_vertex_count(h)
+        self.query_count = lib().cdet_batch_query_count(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cdet_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _ptr(x):
+        if x is None:
+            return None
+        if hasattr(x, "data_ptr"):
+            return ctypes.c_void_p(x.data_ptr())
+        return ctypes.c_void_p(int(x))
+
+    def profile(self, d_vertices, d_out, d_status=None, stream=None):
+        """Stage times in ms: (lookup grouping, contraction, minors+recursion)."""
+        ms = (ctypes.c_float * 3)()
+        _check(lib().cdet_batch_profile(self._h, self._ptr(d_vertices), self._ptr(d_out), self._ptr(d_status),
+                                        self._ptr(stream), ms))
+        return tuple(ms)
+
+    def run(self, d_vertices, d_out, d_status=None, use_graph=False, stream=None):
+        _check(lib().cdet_batch_run(self._h, self._ptr(d_vertices), self._ptr(d_out), self._ptr(d_status),
+                                    1 if use_graph else 0, self._ptr(stream)))
+
+
+def packed_offsets(orders, single=False):
+    """Vertex and output offsets of a packed batch (cdet_batch_plan_create)."""
+    orders = np.asarray(orders, dtype=np.int64)
+    vb = np.concatenate([[0], np.cumsum(orders)])
+    sizes = np.ones_like(orders) if single else (np.int64(1) << orders)
+    ob = np.concatenate([[0], np.cumsum(sizes)])
+    return vb, ob

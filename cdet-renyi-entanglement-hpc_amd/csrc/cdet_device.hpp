@@ -1,0 +1,361 @@
+// cdet_device.hpp -- gfx950 device building blocks of the CDet integrand path.
+//
+// Everything here is FP64 and follows the reference's arithmetic operation by
+// operation (compiled with -ffp-contract=off; fma only where the reference
+// calls std::fma), so a value computed here is bit-identical to the
+// reference's CPU value for the same inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cdet {
+
+constexpr int kMaxOrder = 14;
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Double-double (include/cdet/utility/types.h:30-150): Dekker/Knuth
+// error-free transforms in the reference's sequence.
+// ---------------------------------------------------------------------------
+struct DD {
+    double hi;
+    double lo;
+};
+__device__ __host__ inline double dd_quick_two_sum(double a, double b, double& err) {
+    const double s = a + b;
+    err = b - (s - a);
+    return s;
+}
+__device__ __host__ inline double dd_two_sum(double a, double b, double& err) {
+    const double s = a + b;
+    const double bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+    return s;
+}
+__device__ inline DD dd_sub(DD x, DD y) {
+    double s2, t2;
+    double s1 = dd_two_sum(x.hi, -y.hi, s2);
+    const double t1 = dd_two_sum(x.lo, -y.lo, t2);
+    s2 += t1;
+    s1 = dd_quick_two_sum(s1, s2, s2);
+    s2 += t2;
+    DD r;
+    r.hi = dd_quick_two_sum(s1, s2, r.lo);
+    return r;
+}
+__device__ inline DD dd_mul(DD x, DD y) {
+    const double p = x.hi * y.hi;
+    double e = __builtin_fma(x.hi, y.hi, -p);
+    e += x.hi * y.lo + x.lo * y.hi + x.lo * y.lo;
+    DD r;
+    r.hi = dd_quick_two_sum(p, e, r.lo);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Vertex decode and lookup descriptor (include/cdet/g_funcs.h:300-342,
+// :609-653; include/cdet/integrand_replica_randomly_sampled.h:142-153, :652-662).
+// ---------------------------------------------------------------------------
+struct Vtx {
+    double tau;
+    double tau_unit;
+    int x;
+    int y;
+    int replica;
+};
+
+__device__ inline int coord_to_index(double coordinate, int extent) {
+    // floor of a double is exact, as is the reference's long-double floor.
+    int index = static_cast<int>(floor(coordinate));
+    if (index < 0) index = 0;
+    if (index >= extent) index = extent - 1;
+    return index;
+}
+
+__device__ inline Vtx decode_vertex(const double* v4, double beta, int N1, int N2, int rc) {
+    Vtx s;
+    s.tau_unit = v4[0];
+    s.tau = beta * s.tau_unit;
+    s.x = coord_to_index(static_cast<double>(N1) * v4[1], N1);
+    s.y = coord_to_index(static_cast<double>(N2) * v4[2], N2);
+    if (rc <= 1) {
+        s.replica = 0;
+    } else {
+        int r = coord_to_index(static_cast<double>(rc) * v4[3], rc);
+        s.replica = r < 0 ? 0 : (r > rc - 1 ? rc - 1 : r);
+    }
+    return s;
+}
+
+__device__ inline double clamp_unit(double v) {
+    if (v < -1.0) return -1.0;
+    if (v > 1.0) return 1.0;
+    return v;
+}
+
+__device__ __host__ inline int64_t sitepair_index(int ra, int rb, int x1, int y1, int x2, int y2, int N1, int N2) {
+    const int64_t r = static_cast<int64_t>(ra) * 2 + rb;
+    return ((((r * N1 + x1) * N2 + y1) * N1 + x2) * N2) + y2;
+}
+
+struct Desc {
+    int upper;
+    int corner;
+    int64_t sitepair;
+    double u;
+    double v;
+};
+
+__device__ inline Desc make_desc(const Vtx& vi, const Vtx& vj, bool upper, double beta, double branch_eps, int N1,
+                                 int N2) {
+    Desc d;
+    d.upper = upper ? 1 : 0;
+    d.corner = 0;
+    d.u = 0.0;
+    d.v = 0.0;
+    d.sitepair = sitepair_index(vi.replica, vj.replica, vi.x, vi.y, vj.x, vj.y, N1, N2);
+    const double t1 = vi.tau;
+    const double t2 = vj.tau;
+    if (upper) {
+        if (t1 < branch_eps) {
+            d.corner = 1;
+            return d;
+        }
+        d.u = clamp_unit(2.0 * t1 / beta - 1.0);
+        d.v = clamp_unit(2.0 * t2 / t1 - 1.0);
+    } else {
+        if (t2 < branch_eps) {
+            d.corner = 1;
+            return d;
+        }
+        d.u = clamp_unit(2.0 * t2 / beta - 1.0);
+        d.v = clamp_unit(2.0 * t1 / t2 - 1.0);
+    }
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Clenshaw over one contiguous coefficient row (include/cdet/g_funcs.h:426-463).
+// ---------------------------------------------------------------------------
+__device__ inline double clenshaw_row(const double* c, int degree, double x) {
+    if (degree <= 0) return degree < 0 ? 0.0 : c[0];
+    if (degree == 1) return c[0] + c[1] * x;
+    const double two_x = 2.0 * x;
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = degree; k >= 1; --k) {
+        const double bk = c[k] + two_x * b1 - b2;
+        b2 = b1;
+        b1 = bk;
+    }
+    return c[0] + b1 * x - b2;
+}
+
+// Incremental Clenshaw over u that consumes row values r[nu-1], ..., r[0] in
+// that order (the order in which the reference's final Clenshaw reads them,
+// include/cdet/g_funcs.h:748 -> :450-461).
+struct UClenshaw {
+    double b1, b2, r1, two_u, u;
+    int degree;
+    __device__ inline void init(double uu, int nu) {
+        b1 = 0.0;
+        b2 = 0.0;
+        r1 = 0.0;
+        u = uu;
+        two_u = 2.0 * uu;
+        degree = nu - 1;
+    }
+    // Feed row value of index a (a descending); returns the final value at a == 0.
+    __device__ inline double feed(int a, double r) {
+        if (degree >= 2) {
+            if (a >= 1) {
+                const double bk = r + two_u * b1 - b2;
+                b2 = b1;
+                b1 = bk;
+                return 0.0;
+            }
+            return r + b1 * u - b2;
+        }
+        if (degree == 1) {
+            if (a == 1) {
+                r1 = r;
+                return 0.0;
+            }
+            return r + r1 * u;
+        }
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Workgroup-cooperative fast principal minors
+// (include/cdet/fast_principle_minors.h:120-246).  On return sm[mask] holds
+// det(A[mask,mask]) for every mask (sm[0] = 1).  All intermediates are formed
+// with the reference's operations; only the iteration order across
+// independent elements differs, which cannot change any value.
+// Returns 1 when a pivot triggered the brute-force fallback (then sm holds the
+// pivoted-determinant minors, include/cdet/fast_principle_minors.h:39-115).
+// Scratch: bufA/bufB hold max_l 2^(l+1)(n-l-1)^2 doubles (and n*n), inv holds
+// 2^(n-2) doubles, red holds blockDim doubles.
+// ---------------------------------------------------------------------------
+__device__ inline double det_minor_brute(const double* A, int n, uint32_t mask) {
+    int idx[kMaxOrder];
+    int k = 0;
+    for (int i = 0; i < n; ++i)
+        if (mask & (1u << i)) idx[k++] = i;
+    if (k == 0) return 1.0;
+    double m[kMaxOrder * kMaxOrder];
+    for (int r = 0; r < k; ++r)
+        for (int c = 0; c < k; ++c) m[r * k + c] = A[idx[r] * n + idx[c]];
+    double det = 1.0;
+    int sign = 1;
+    for (int col = 0; col < k; ++col) {
+        int prow = col;
+        double pabs = fabs(m[col * k + col]);
+        for (int r = col + 1; r < k; ++r) {
+            const double cand = fabs(m[r * k + col]);
+            if (cand > pabs) {
+                pabs = cand;
+                prow = r;
+            }
+        }
+        if (pabs == 0.0) return 0.0;
+        if (prow != col) {
+            for (int c = col; c < k; ++c) {
+                const double t = m[col * k + c];
+                m[col * k + c] = m[prow * k + c];
+                m[prow * k + c] = t;
+            }
+            sign = -sign;
+        }
+        const double piv = m[col * k + col];
+        det *= piv;
+        for (int r = col + 1; r < k; ++r) {
+            const double f = m[r * k + col] / piv;
+            if (f == 0.0) continue;
+            m[r * k + col] = 0.0;
+            for (int c = col + 1; c < k; ++c) m[r * k + c] = __builtin_fma(-f, m[col * k + c], m[r * k + c]);
+        }
+    }
+    return sign < 0 ? -det : det;
+}
+
+__device__ inline double block_max(double v, double* red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+        if (tid < s) {
+            const double o = red[tid + s];
+            if (o > red[tid]) red[tid] = o;
+        }
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline int fpm_block(const double* A, int n, double tol, double thresh, double* sm, double* bufA,
+                                double* bufB, double* inv, double* red, int* flag) {
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const uint32_t sc = 1u << n;
+    const int msize = n * n;
+
+    double lmax = 0.0;
+    for (int i = tid; i < msize; i += nt) {
+        const double e = fabs(A[i]);
+        if (e > lmax) lmax = e;
+    }
+    const double max_abs = block_max(lmax, red);
+    const bool explicit_shift = !(thresh < 0.0);
+    const double p = explicit_shift ? thresh : max_abs / static_cast<double>(msize);
+    if (max_abs == 0.0 && p == 0.0) {
+        for (uint32_t m = tid; m < sc; m += nt) sm[m] = (m == 0) ? 1.0 : 0.0;
+        __syncthreads();
+        return 0;
+    }
+    const double fb_eps = (tol > 0.0 && max_abs > 0.0) ? tol * max_abs : 0.0;
+
+    for (int i = tid; i < msize; i += nt) bufA[i] = A[i];
+    for (uint32_t m = tid; m < sc; m += nt) sm[m] = (m == 0) ? 1.0 : p;
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    if (tid == 0) sm[1] += A[0];
+
+    double* oldl = bufA;
+    double* newl = bufB;
+    for (int l = 0; l < n - 1; ++l) {
+        const int os = n - l;
+        const int oms = os * os;
+        const int ns = os - 1;
+        const int nms = ns * ns;
+        const int sets = 1 << l;
+        for (int s = tid; s < sets; s += nt) {
+            const double pd = oldl[s * oms] + p;
+            const double pda = fabs(pd);
+            if (pda == 0.0 || (fb_eps > 0.0 && pda <= fb_eps)) *flag = 1;
+            inv[s] = 1.0 / pd;
+        }
+        __syncthreads();
+        if (*flag) break;
+        const int items = sets * nms;
+        for (int it = tid; it < items; it += nt) {
+            const int s = it / nms;
+            const int rem = it - s * nms;
+            const int r = rem / ns;
+            const int c = rem - r * ns;
+            const int src = s * oms;
+            const double f = oldl[src + (r + 1) * os] * inv[s];
+            const double base = oldl[src + (r + 1) * os + (c + 1)];
+            newl[s * nms + rem] = base;
+            newl[(s + sets) * nms + rem] = __builtin_fma(-oldl[src + 1 + c], f, base);
+        }
+        __syncthreads();
+        const int gen = 2 * sets;
+        for (int m = tid; m < gen; m += nt) sm[(2u << l) + m] += newl[m * nms];
+        __syncthreads();
+        double* t = oldl;
+        oldl = newl;
+        newl = t;
+    }
+    if (*flag) {
+        for (uint32_t m = tid; m < sc; m += nt) sm[m] = (m == 0) ? 1.0 : det_minor_brute(A, n, m);
+        __syncthreads();
+        return 1;
+    }
+    // Prefix products (:217-223).
+    for (int l = 0; l < n; ++l) {
+        const uint32_t step = 1u << l;
+        for (uint32_t s = step + 1 + tid; s < 2 * step; s += nt) sm[s] *= sm[s - step];
+        __syncthreads();
+    }
+    // Removal of the uniform shift, highest bit first (:225-239).
+    for (int l = n; l > 0; --l) {
+        const uint32_t half = 1u << (l - 1);
+        // masks with bit (l-1) set: m = hi_part * 2^l + half + lo_part
+        const uint32_t cnt = sc >> 1;
+        for (uint32_t it = tid; it < cnt; it += nt) {
+            const uint32_t lo = it & (half - 1);
+            const uint32_t hi = it >> (l - 1);
+            const uint32_t m = (hi << l) | half | lo;
+            sm[m] = __builtin_fma(-p, sm[m - half], sm[m]);
+        }
+        __syncthreads();
+    }
+    return 0;
+}
+
+// Size (doubles) of each FPM list buffer for order n.
+__host__ __device__ inline int fpm_list_size(int n) {
+    int best = n * n;
+    for (int l = 0; l < n - 1; ++l) {
+        const int ns = n - l - 1;
+        const int c = (1 << (l + 1)) * ns * ns;
+        if (c > best) best = c;
+    }
+    return best;
+}
+
+}  // namespace cdet

@@ -1,0 +1,239 @@
+// cdet_mfma.hpp -- matrix-core (MFMA) coefficient x basis contraction for the
+// batched table evaluation.
+//
+// For the Q queries (u_q, v_q) that hit one coefficient block C (nu x nv):
+//     R[a][q] = sum_b C[a][b] T_b(v_q)            (MFMA, M = a, N = q, K = b)
+//     G_q     = sum_a T_a(u_q) R[a][q]            (per-lane FMA + cross-lane sum)
+// One wave owns a tile of up to 32 queries (two 16-wide N tiles) and sweeps
+// the block in 16-row slabs, so only two 16x16 accumulators are live; the
+// A fragment of each K step is loaded once and used for both N tiles.
+//
+// Operand layouts of v_mfma_f64_16x16x4_f64 and v_mfma_f32_16x16x4_f32
+// (lane l, g = l / 16, j = l % 16):  A[j][g],  B[g][j],  D[4g + r][j] in
+// accumulator register r.
+//
+// Chebyshev bases are generated in registers:  T_{k+4} = 2 T_4 T_k - T_{k-4}
+// along K (each lane owns k = 4s + g) and T_{a+16} = 2 T_16 T_a - T_{a-16}
+// across row slabs, seeded from T_0..T_16 by the ordinary three-term
+// recurrence (T_{-m} = T_m).
+//
+// Modes:
+//   MFMA64   FP64 operands, FP64 accumulation (v_mfma_f64_16x16x4_f64).
+//   SPLIT32  C = C_hi + C_lo, B = B_hi + B_lo in FP32; the three products
+//            hi*hi + hi*lo + lo*hi are accumulated in FP32 over one K step
+//            (4 terms) and added into FP64 accumulators.
+#pragma once
+
+#include "cdet_scalars.hpp"
+
+namespace cdet {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaTile = 32;  // queries per wave
+
+// T_m(x) for m = 0..16.
+__device__ inline void cheb_0_16(double x, double* t) {
+    t[0] = 1.0;
+    t[1] = x;
+    const double tx = 2.0 * x;
+#pragma unroll
+    for (int m = 2; m <= 16; ++m) t[m] = tx * t[m - 1] - t[m - 2];
+}
+
+// Select t[4g + r] and t[|4g + r - 16|] (g in 0..3) without dynamic indexing.
+__device__ inline double sel4(int g, double a0, double a1, double a2, double a3) {
+    return g == 0 ? a0 : (g == 1 ? a1 : (g == 2 ? a2 : a3));
+}
+
+struct BasisK {  // lane's T_k(v) along K with k = 4s + g
+    double cur, prev, w;
+    __device__ inline void init(double v, int g) {
+        double t[17];
+        cheb_0_16(v, t);
+        cur = sel4(g, t[0], t[1], t[2], t[3]);
+        prev = sel4(g, t[4], t[3], t[2], t[1]);  // T_{4-g} = T_{g-4}
+        w = 2.0 * t[4];
+    }
+    __device__ inline void step() {
+        const double nx = w * cur - prev;
+        prev = cur;
+        cur = nx;
+    }
+};
+
+struct BasisA {  // lane's T_a(u) for its four rows a = 16 m + 4 g + r
+    double cur[4], prev[4], w;
+    __device__ inline void init(double u, int g) {
+        double t[17];
+        cheb_0_16(u, t);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cur[r] = sel4(g, t[r], t[4 + r], t[8 + r], t[12 + r]);
+            prev[r] = sel4(g, t[16 - r], t[12 - r], t[8 - r], t[4 - r]);
+        }
+        w = 2.0 * t[16];
+    }
+    __device__ inline void step() {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double nx = w * cur[r] - prev[r];
+            prev[r] = cur[r];
+            cur[r] = nx;
+        }
+    }
+};
+
+__device__ inline int xcd_remap(int b, int grid) {
+    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous range of logical blocks so tiles of one coefficient block
+    // share that XCD's L2.
+    const int per = grid / 8;
+    if (per == 0 || grid % 8 != 0) return b;
+    return (b % 8) * per + b / 8;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_contract_mfma(KScalars ks, const int32_t* __restrict__ n_tiles,
+                                                       const int32_t* __restrict__ tile_key,
+                                                       const int32_t* __restrict__ tile_start,
+                                                       const int32_t* __restrict__ tile_len,
+                                                       const double* __restrict__ s_u, const double* __restrict__ s_v,
+                                                       const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
+                                                       double* __restrict__ g_dn) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int total = *n_tiles;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int nu = ks.nu, nv = ks.nv;
+    const int mslabs = (nu + 15) >> 4;
+    const int ksteps = (nv + 3) >> 2;
+    for (int t = lb * wpb + wave; t < total; t += gridDim.x * wpb) {
+        const int tile = __builtin_amdgcn_readfirstlane(t);
+        const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
+        const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);
+        const int32_t len = __builtin_amdgcn_readfirstlane(tile_len[tile]);
+        const double* cu;
+        const double* cd;
+        bool same;
+        ks.key_tables(key, cu, cd, same);
+        const float* hu = nullptr;
+        const float* lu = nullptr;
+        const float* hd = nullptr;
+        const float* ld = nullptr;
+        if (MODE == CDET_MODE_SPLIT32) ks.key_split(key, hu, lu, hd, ld);
+        // queries of the two N tiles owned by this lane
+        const int q0 = j, q1 = j + 16;
+        const int p0 = start + (q0 < len ? q0 : 0);
+        const int p1 = start + (q1 < len ? q1 : 0);
+        const double u0 = s_u[p0], v0 = s_v[p0];
+        const double u1 = s_u[p1], v1 = s_v[p1];
+        BasisA A0, A1;
+        A0.init(u0, g);
+        A1.init(u1, g);
+        double G0u = 0.0, G1u = 0.0, G0d = 0.0, G1d = 0.0;
+        for (int ms = 0; ms < mslabs; ++ms) {
+            const int row = 16 * ms + j;
+            const bool row_ok = row < nu;
+            d4 acc0u = {0.0, 0.0, 0.0, 0.0}, acc1u = acc0u, acc0d = acc0u, acc1d = acc0u;
+            BasisK B0, B1;
+            B0.init(v0, g);
+            B1.init(v1, g);
+            const int64_t roff = static_cast<int64_t>(row_ok ? row : 0) * nv;
+            for (int s = 0; s < ksteps; ++s) {
+                const int k = 4 * s + g;
+                const bool ok = row_ok && k < nv;
+                if (MODE == CDET_MODE_MFMA64) {
+                    const double au = ok ? cu[roff + k] : 0.0;
+                    acc0u = __builtin_amdgcn_mfma_f64_16x16x4f64(au, B0.cur, acc0u, 0, 0, 0);
+                    acc1u = __builtin_amdgcn_mfma_f64_16x16x4f64(au, B1.cur, acc1u, 0, 0, 0);
+                    if (!same) {
+                        const double ad = ok ? cd[roff + k] : 0.0;
+                        acc0d = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, B0.cur, acc0d, 0, 0, 0);
+                        acc1d = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, B1.cur, acc1d, 0, 0, 0);
+                    }
+                } else {
+                    const float ah = ok ? hu[roff + k] : 0.0f;
+                    const float al = ok ? lu[roff + k] : 0.0f;
+                    const float b0h = static_cast<float>(B0.cur);
+                    const float b0l = static_cast<float>(B0.cur - static_cast<double>(b0h));
+                    const float b1h = static_cast<float>(B1.cur);
+                    const float b1l = static_cast<float>(B1.cur - static_cast<double>(b1h));
+                    const f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+                    f4 t0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, b0h, z, 0, 0, 0);
+                    t0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, b0l, t0, 0, 0, 0);
+                    t0 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, b0h, t0, 0, 0, 0);
+                    f4 t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, b1h, z, 0, 0, 0);
+                    t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, b1l, t1, 0, 0, 0);
+                    t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, b1h, t1, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        acc0u[r] += static_cast<double>(t0[r]);
+                        acc1u[r] += static_cast<double>(t1[r]);
+                    }
+                    if (!same) {
+                        const float dh = ok ? hd[roff + k] : 0.0f;
+                        const float dl = ok ? ld[roff + k] : 0.0f;
+                        f4 e0 = __builtin_amdgcn_mfma_f32_16x16x4f32(dh, b0h, z, 0, 0, 0);
+                        e0 = __builtin_amdgcn_mfma_f32_16x16x4f32(dh, b0l, e0, 0, 0, 0);
+                        e0 = __builtin_amdgcn_mfma_f32_16x16x4f32(dl, b0h, e0, 0, 0, 0);
+                        f4 e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(dh, b1h, z, 0, 0, 0);
+                        e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(dh, b1l, e1, 0, 0, 0);
+                        e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(dl, b1h, e1, 0, 0, 0);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            acc0d[r] += static_cast<double>(e0[r]);
+                            acc1d[r] += static_cast<double>(e1[r]);
+                        }
+                    }
+                }
+                B0.step();
+                B1.step();
+            }
+            // Row slab contribution: rows 16 ms + 4 g + r of this lane.
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const bool ok = (16 * ms + 4 * g + r) < nu;
+                const double w0 = ok ? A0.cur[r] : 0.0;
+                const double w1 = ok ? A1.cur[r] : 0.0;
+                G0u = __builtin_fma(w0, acc0u[r], G0u);
+                G1u = __builtin_fma(w1, acc1u[r], G1u);
+                if (!same) {
+                    G0d = __builtin_fma(w0, acc0d[r], G0d);
+                    G1d = __builtin_fma(w1, acc1d[r], G1d);
+                }
+            }
+            A0.step();
+            A1.step();
+        }
+        // Sum the four row groups (lanes j, j+16, j+32, j+48).
+        G0u += __shfl_xor(G0u, 16);
+        G0u += __shfl_xor(G0u, 32);
+        G1u += __shfl_xor(G1u, 16);
+        G1u += __shfl_xor(G1u, 32);
+        if (!same) {
+            G0d += __shfl_xor(G0d, 16);
+            G0d += __shfl_xor(G0d, 32);
+            G1d += __shfl_xor(G1d, 16);
+            G1d += __shfl_xor(G1d, 32);
+        }
+        if (g == 0) {
+            if (q0 < len) {
+                const int32_t d0 = s_dest[start + q0];
+                g_up[d0] = G0u;
+                g_dn[d0] = same ? G0u : G0d;
+            }
+            if (q1 < len) {
+                const int32_t d1 = s_dest[start + q1];
+                g_up[d1] = G1u;
+                g_dn[d1] = same ? G1u : G1d;
+            }
+        }
+    }
+}
+
+}  // namespace cdet

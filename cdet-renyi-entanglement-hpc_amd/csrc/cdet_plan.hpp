@@ -1,0 +1,411 @@
+// cdet_plan.hpp -- batched evaluation plans (included by cdet_kernels.hip).
+//
+// A plan fixes the order of every configuration of a batch.  It owns the
+// per-configuration offsets, the order buckets and all scratch, so that a run
+// is a fixed launch sequence with no allocation and no host synchronisation;
+// the sequence is captured once into a hipGraph and replayed.
+
+struct cdet_batch_plan {
+    cdet_ctx* ctx = nullptr;
+    int contract_mode = CDET_MODE_EXACT;
+    int eval_kind = 0;  // 0 all-subset, 1 single-order
+    int64_t count = 0;
+    int64_t nq = 0;         // 2 * sum n^2 query slots
+    int64_t out_total = 0;  // doubles written
+    int64_t vert_total = 0;
+    int32_t nkeys = 0;
+    int64_t max_tiles = 0;
+    int tile = 64;
+    std::vector<int> bucket_off, bucket_cnt;  // by order
+    int32_t* d_order = nullptr;
+    int64_t* d_vbase = nullptr;
+    int64_t* d_pbase = nullptr;
+    int64_t* d_obase = nullptr;
+    int32_t* d_lists = nullptr;
+    double* d_gup = nullptr;
+    double* d_gdn = nullptr;
+    int32_t* d_qkey = nullptr;
+    int32_t* d_qrank = nullptr;
+    double* d_qu = nullptr;
+    double* d_qv = nullptr;
+    int32_t* d_hist = nullptr;
+    int32_t* d_keyoff = nullptr;
+    int32_t* d_tkey = nullptr;
+    int32_t* d_tstart = nullptr;
+    int32_t* d_tlen = nullptr;
+    int32_t* d_ntiles = nullptr;
+    double* d_su = nullptr;
+    double* d_sv = nullptr;
+    int32_t* d_sdest = nullptr;
+    int contract_grid = 0;
+    // graph cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* g_verts = nullptr;
+    void* g_out = nullptr;
+    void* g_status = nullptr;
+    hipStream_t g_stream = nullptr;
+};
+
+static void plan_free(cdet_batch_plan* p) {
+    if (!p) return;
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    void* bufs[] = {p->d_order, p->d_vbase, p->d_pbase, p->d_obase, p->d_lists, p->d_gup, p->d_gdn, p->d_qkey,
+                    p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
+                    p->d_ntiles, p->d_su, p->d_sv, p->d_sdest};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+template <typename T>
+static int dev_alloc(T** p, size_t n) {
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(1, n) * sizeof(T)));
+    return 0;
+}
+
+static int contract_launch(cdet_batch_plan* p, hipStream_t s);
+
+static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                                hipStream_t s) {
+    cdet_ctx* ctx = p->ctx;
+    const int64_t nk = p->nkeys;
+    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, s, p->d_hist, nk, 0);
+    hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
+                       reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
+                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist);
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, s, p->d_hist, p->nkeys, p->tile, p->d_keyoff, p->d_tkey,
+                       p->d_tstart, p->d_tlen, p->d_ntiles);
+    hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>((p->nq + 255) / 256)), dim3(256), 0, s, p->nq, p->d_qkey,
+                       p->d_qrank, p->d_qu, p->d_qv, p->d_keyoff, p->d_su, p->d_sv, p->d_sdest);
+    int rc = contract_launch(p, s);
+    if (rc) return rc;
+    for (int n = kMaxOrder; n >= 1; --n) {
+        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
+        if (!cnt) continue;
+        rc = launch_fpm(ctx, n, p->eval_kind, p->d_lists + p->bucket_off[static_cast<size_t>(n)], cnt, p->d_pbase,
+                        p->d_obase, p->d_gup, p->d_gdn, d_out, d_status, s);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int contract_launch(cdet_batch_plan* p, hipStream_t s) {
+    cdet_ctx* ctx = p->ctx;
+    switch (p->contract_mode) {
+        case CDET_MODE_EXACT:
+            hipLaunchKernelGGL(k_contract_exact, dim3(p->contract_grid), dim3(256), 0, s, ctx->ks, p->d_ntiles,
+                               p->d_tkey, p->d_tstart, p->d_tlen, p->d_su, p->d_sv, p->d_sdest, p->d_gup, p->d_gdn);
+            break;
+        case CDET_MODE_MFMA64:
+        case CDET_MODE_SPLIT32:
+            return contract_launch_mfma(ctx->ks, p->contract_mode, p->contract_grid, p->d_ntiles, p->d_tkey,
+                                        p->d_tstart, p->d_tlen, p->d_su, p->d_sv, p->d_sdest, p->d_gup, p->d_gdn, s);
+        default:
+            return fail(CDET_ERR_ARGUMENT, "unknown evaluation mode");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" {
+
+int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int mode, int eval_kind,
+                           cdet_batch_plan** out) {
+    if (!ctx || !orders || !out || count <= 0) return fail(CDET_ERR_ARGUMENT, "cdet_batch_plan_create: bad argument");
+    if (mode < 0 || mode > 2) return fail(CDET_ERR_ARGUMENT, "unknown evaluation mode");
+    if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
+    auto* p = new cdet_batch_plan();
+    p->ctx = ctx;
+    p->contract_mode = mode;
+    p->eval_kind = eval_kind ? 1 : 0;
+    p->count = count;
+    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks);
+    std::vector<int64_t> vb(static_cast<size_t>(count)), pb(static_cast<size_t>(count)), ob(static_cast<size_t>(count));
+    p->bucket_cnt.assign(kMaxOrder + 1, 0);
+    p->bucket_off.assign(kMaxOrder + 2, 0);
+    int64_t v = 0, pp = 0, o = 0;
+    for (int64_t c = 0; c < count; ++c) {
+        const int n = orders[c];
+        const int nmin = p->eval_kind ? 1 : 2;
+        if (n < nmin || n > kMaxOrder) {
+            const bool order_error = n <= 1 && !p->eval_kind;
+            plan_free(p);
+            if (order_error)
+                return fail(CDET_ERR_ORDER, "The all-subset replica integrand is defined only for envelope order >= 2");
+            return fail(CDET_ERR_CAPACITY, "configuration order outside the supported range");
+        }
+        vb[static_cast<size_t>(c)] = v;
+        pb[static_cast<size_t>(c)] = pp;
+        ob[static_cast<size_t>(c)] = o;
+        v += n;
+        pp += static_cast<int64_t>(n) * n;
+        o += p->eval_kind ? 1 : (int64_t(1) << n);
+        p->bucket_cnt[static_cast<size_t>(n)] += 1;
+    }
+    p->vert_total = v;
+    p->out_total = o;
+    p->nq = 2 * pp;
+    if (p->nq >= (int64_t(1) << 31)) {
+        plan_free(p);
+        return fail(CDET_ERR_CAPACITY, "batch too large for one plan");
+    }
+    for (int n = 1; n <= kMaxOrder; ++n) p->bucket_off[static_cast<size_t>(n + 1)] = p->bucket_off[static_cast<size_t>(n)] + p->bucket_cnt[static_cast<size_t>(n)];
+    std::vector<int32_t> lists(static_cast<size_t>(count));
+    {
+        std::vector<int> fillpos(p->bucket_off.begin(), p->bucket_off.end());
+        for (int64_t c = 0; c < count; ++c) lists[static_cast<size_t>(fillpos[static_cast<size_t>(orders[c])]++)] = static_cast<int32_t>(c);
+    }
+    p->nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
+    const int64_t nonempty = std::min<int64_t>(p->nkeys, p->nq);
+    p->max_tiles = p->nq / p->tile + nonempty + 1;
+    int rc = 0;
+    rc |= dev_alloc(&p->d_order, count);
+    rc |= dev_alloc(&p->d_vbase, count);
+    rc |= dev_alloc(&p->d_pbase, count);
+    rc |= dev_alloc(&p->d_obase, count);
+    rc |= dev_alloc(&p->d_lists, count);
+    rc |= dev_alloc(&p->d_gup, p->nq);
+    rc |= dev_alloc(&p->d_gdn, p->nq);
+    rc |= dev_alloc(&p->d_qkey, p->nq);
+    rc |= dev_alloc(&p->d_qrank, p->nq);
+    rc |= dev_alloc(&p->d_qu, p->nq);
+    rc |= dev_alloc(&p->d_qv, p->nq);
+    rc |= dev_alloc(&p->d_hist, p->nkeys);
+    rc |= dev_alloc(&p->d_keyoff, p->nkeys);
+    rc |= dev_alloc(&p->d_tkey, p->max_tiles);
+    rc |= dev_alloc(&p->d_tstart, p->max_tiles);
+    rc |= dev_alloc(&p->d_tlen, p->max_tiles);
+    rc |= dev_alloc(&p->d_ntiles, 1);
+    rc |= dev_alloc(&p->d_su, p->nq);
+    rc |= dev_alloc(&p->d_sv, p->nq);
+    rc |= dev_alloc(&p->d_sdest, p->nq);
+    if (rc) {
+        plan_free(p);
+        return CDET_ERR_DEVICE;
+    }
+    auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
+    if (up(p->d_order, orders, sizeof(int32_t) * count) != hipSuccess ||
+        up(p->d_vbase, vb.data(), sizeof(int64_t) * count) != hipSuccess ||
+        up(p->d_pbase, pb.data(), sizeof(int64_t) * count) != hipSuccess ||
+        up(p->d_obase, ob.data(), sizeof(int64_t) * count) != hipSuccess ||
+        up(p->d_lists, lists.data(), sizeof(int32_t) * count) != hipSuccess) {
+        plan_free(p);
+        return fail(CDET_ERR_DEVICE, "plan upload failed");
+    }
+    // Persistent contraction grid: enough workgroups to fill 256 CUs several
+    // times, a multiple of the 8 XCDs.
+    const int64_t wgs = (p->max_tiles + 3) / 4;
+    int64_t grid = std::min<int64_t>(std::max<int64_t>(wgs, 1), 256 * 8);
+    grid = ((grid + 7) / 8) * 8;
+    p->contract_grid = static_cast<int>(grid);
+    // Global scratch of the orders whose minors do not fit in LDS, sized now
+    // so that runs never allocate (and can be graph-captured).
+    for (int n = 1; n <= kMaxOrder; ++n) {
+        if (!p->bucket_cnt[static_cast<size_t>(n)] || fpm_lds_bytes(n) <= kMaxLds) continue;
+        const size_t need = fpm_lds_bytes(n) * static_cast<size_t>(p->bucket_cnt[static_cast<size_t>(n)]);
+        void* g = ctx->d_gscratch;
+        rc = grow(&g, ctx->gscratch_bytes, need);
+        ctx->d_gscratch = static_cast<double*>(g);
+        if (rc) {
+            plan_free(p);
+            return rc;
+        }
+    }
+    if (mode == CDET_MODE_SPLIT32) {
+        rc = ensure_split_tables(ctx);
+        if (rc) {
+            plan_free(p);
+            return rc;
+        }
+    }
+    *out = p;
+    return 0;
+}
+
+int cdet_batch_plan_destroy(cdet_batch_plan* plan) {
+    if (plan && plan->ctx) (void)hipStreamSynchronize(plan->ctx->stream);
+    plan_free(plan);
+    return 0;
+}
+
+int64_t cdet_batch_out_size(const cdet_batch_plan* plan) { return plan ? plan->out_total : -1; }
+int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan) { return plan ? plan->vert_total : -1; }
+int64_t cdet_batch_query_count(const cdet_batch_plan* plan) { return plan ? plan->nq : -1; }
+
+int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                   int use_graph, void* stream) {
+    if (!p || !d_vertices || !d_out) return fail(CDET_ERR_ARGUMENT, "cdet_batch_run: bad argument");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
+    if (!use_graph) {
+        int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
+        if (!rc) p->ctx->calls += static_cast<uint64_t>(p->count);
+        return rc;
+    }
+    if (!p->graph_exec || p->g_verts != d_vertices || p->g_out != d_out || p->g_status != d_status || p->g_stream != s) {
+        if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->graph_exec = nullptr;
+        p->graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(CDET_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+        p->graph = g;
+        HIP_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+        p->g_verts = d_vertices;
+        p->g_out = d_out;
+        p->g_status = d_status;
+        p->g_stream = s;
+    }
+    HIP_TRY(hipGraphLaunch(p->graph_exec, s));
+    p->ctx->calls += static_cast<uint64_t>(p->count);
+    return 0;
+}
+
+int cdet_batch_profile(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                       void* stream, float* stage_ms) {
+    if (!p || !d_vertices || !d_out || !stage_ms) return fail(CDET_ERR_ARGUMENT, "cdet_batch_profile: bad argument");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
+    cdet_ctx* ctx = p->ctx;
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    const int64_t nk = p->nkeys;
+    HIP_TRY(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, s, p->d_hist, nk, 0);
+    hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
+                       reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
+                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist);
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, s, p->d_hist, p->nkeys, p->tile, p->d_keyoff, p->d_tkey,
+                       p->d_tstart, p->d_tlen, p->d_ntiles);
+    hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>((p->nq + 255) / 256)), dim3(256), 0, s, p->nq, p->d_qkey,
+                       p->d_qrank, p->d_qu, p->d_qv, p->d_keyoff, p->d_su, p->d_sv, p->d_sdest);
+    HIP_TRY(hipEventRecord(ev[1], s));
+    int rc = contract_launch(p, s);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ev[2], s));
+    for (int n = kMaxOrder; n >= 1; --n) {
+        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
+        if (!cnt) continue;
+        rc = launch_fpm(ctx, n, p->eval_kind, p->d_lists + p->bucket_off[static_cast<size_t>(n)], cnt, p->d_pbase,
+                        p->d_obase, p->d_gup, p->d_gdn, d_out, d_status, s);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(ev[3], s));
+    HIP_TRY(hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; ++k) HIP_TRY(hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return 0;
+}
+
+int cdet_eval_batch(cdet_ctx* ctx, const cdet_vertex* d_vertices, const int32_t* h_orders, int64_t count,
+                    double* d_out, int32_t* d_status, int mode, void* stream) {
+    cdet_batch_plan* p = nullptr;
+    int rc = cdet_batch_plan_create(ctx, h_orders, count, mode, 0, &p);
+    if (rc) return rc;
+    rc = cdet_batch_run(p, d_vertices, d_out, d_status, 0, stream);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    (void)hipStreamSynchronize(s);
+    plan_free(p);
+    return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Table evaluation only (cdet_green_pair_eval): lookups grouped by coefficient
+// block and contracted by the same kernels as the integrand batches.
+// ---------------------------------------------------------------------------
+struct QueryScratch {
+    int64_t cap = 0;
+    int64_t tiles_cap = 0;
+    double *gup = nullptr, *gdn = nullptr, *qu = nullptr, *qv = nullptr, *su = nullptr, *sv = nullptr;
+    int32_t *qkey = nullptr, *qrank = nullptr, *sdest = nullptr, *hist = nullptr, *keyoff = nullptr;
+    int32_t *tkey = nullptr, *tstart = nullptr, *tlen = nullptr, *ntiles = nullptr;
+};
+
+static QueryScratch& query_scratch(cdet_ctx* ctx) {
+    static std::mutex mu;
+    static std::vector<std::pair<cdet_ctx*, QueryScratch*>> all;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& e : all)
+        if (e.first == ctx) return *e.second;
+    all.emplace_back(ctx, new QueryScratch());
+    return *all.back().second;
+}
+
+extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus, int64_t count,
+                                    double* out, int mode, void* stream) {
+    if (!ctx || !query_ints || !query_taus || !out || count <= 0)
+        return fail(CDET_ERR_ARGUMENT, "cdet_green_pair_eval: bad argument");
+    if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
+    if (count >= (int64_t(1) << 31)) return fail(CDET_ERR_CAPACITY, "too many lookups for one call");
+    if (mode == CDET_MODE_SPLIT32) {
+        int rc = ensure_split_tables(ctx);
+        if (rc) return rc;
+    }
+    QueryScratch& q = query_scratch(ctx);
+    const int32_t nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
+    const int tile = mode == CDET_MODE_EXACT ? 64 : kMfmaTile;
+    const int64_t max_tiles = count / tile + std::min<int64_t>(nkeys, count) + 1;
+    if (q.cap < count || q.tiles_cap < max_tiles || !q.hist) {
+        void* bufs[] = {q.gup, q.gdn, q.qu, q.qv, q.su, q.sv, q.qkey, q.qrank, q.sdest, q.hist, q.keyoff,
+                        q.tkey, q.tstart, q.tlen, q.ntiles};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        q = QueryScratch();
+        int rc = 0;
+        rc |= dev_alloc(&q.gup, count);
+        rc |= dev_alloc(&q.gdn, count);
+        rc |= dev_alloc(&q.qu, count);
+        rc |= dev_alloc(&q.qv, count);
+        rc |= dev_alloc(&q.su, count);
+        rc |= dev_alloc(&q.sv, count);
+        rc |= dev_alloc(&q.qkey, count);
+        rc |= dev_alloc(&q.qrank, count);
+        rc |= dev_alloc(&q.sdest, count);
+        rc |= dev_alloc(&q.hist, nkeys);
+        rc |= dev_alloc(&q.keyoff, nkeys);
+        rc |= dev_alloc(&q.tkey, max_tiles);
+        rc |= dev_alloc(&q.tstart, max_tiles);
+        rc |= dev_alloc(&q.tlen, max_tiles);
+        rc |= dev_alloc(&q.ntiles, 1);
+        if (rc) return CDET_ERR_DEVICE;
+        q.cap = count;
+        q.tiles_cap = max_tiles;
+    }
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    const unsigned gq = static_cast<unsigned>((count + 255) / 256);
+    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nkeys + 255) / 256)), dim3(256), 0, s, q.hist,
+                       static_cast<int64_t>(nkeys), 0);
+    hipLaunchKernelGGL(k_plan_queries, dim3(gq), dim3(256), 0, s, ctx->ks, count, query_ints, query_taus, q.gup, q.gdn,
+                       q.qkey, q.qrank, q.qu, q.qv, q.hist);
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, s, q.hist, nkeys, tile, q.keyoff, q.tkey, q.tstart, q.tlen,
+                       q.ntiles);
+    hipLaunchKernelGGL(k_scatter, dim3(gq), dim3(256), 0, s, count, q.qkey, q.qrank, q.qu, q.qv, q.keyoff, q.su, q.sv,
+                       q.sdest);
+    int64_t grid = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), 256 * 8);
+    grid = ((grid + 7) / 8) * 8;
+    if (mode == CDET_MODE_EXACT) {
+        hipLaunchKernelGGL(k_contract_exact, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, ctx->ks, q.ntiles,
+                           q.tkey, q.tstart, q.tlen, q.su, q.sv, q.sdest, q.gup, q.gdn);
+    } else {
+        int rc = contract_launch_mfma(ctx->ks, mode, static_cast<int>(grid), q.ntiles, q.tkey, q.tstart, q.tlen, q.su,
+                                      q.sv, q.sdest, q.gup, q.gdn, s);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_interleave, dim3(gq), dim3(256), 0, s, count, q.gup, q.gdn, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int cdet_tables_build(cdet_ctx* ctx, const cdet_lattice* lattice) {
+    return build_tables_gpu(ctx, lattice);
+}

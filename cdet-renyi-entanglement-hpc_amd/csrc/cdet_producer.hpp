@@ -1,0 +1,533 @@
+// cdet_producer.hpp -- FP64 replica-Green table construction on the GPU
+// (included by cdet_kernels.hip).
+//
+// Reference: load_data_into_resident_tables (include/cdet/load_data.h:49-99),
+// build_green_function_table / build_ordinary_replica00_green_function_table
+// (include/cdet/replica_chebyshev_tables.h:592-793) and the stable graded
+// kernel (include/cdet/replica_free_greens.h:423-534).
+//
+// Host: lattice Hamiltonian, cyclic Jacobi eigendecomposition (S x S, once)
+// and the sector-mixing matrix S_0 = V^T Omega_0^{-1} V.
+// Device, per Chebyshev node (one workgroup, matrices in LDS):
+//   middle = diag(e^{-theta c}) S_0 diag(e^{-(1-theta) c}) + diag(e^{x-c})
+//   (conjugate transpose of S_0 on the lower branch), solved against the
+//   diagonal right-hand side by pivoted complex Gauss-Jordan, scaled by the
+//   left factor, and rotated to sites G = V K V^T.
+// The second replica sector is the complex conjugate of the first
+// (S_1 = conj(S_0) with everything else real), so one solve per node gives
+// both: (g0+g1)/2 = Re g0, i(g0-g1)/2 = -Im g0, -i(g0-g1)/2 = Im g0.
+// Samples land at [sitepair][node] in the table itself and are transformed in
+// place, per sitepair, by the separable cosine transform
+// (include/cdet/replica_chebyshev_tables.h:552-590).
+
+namespace {
+
+constexpr int kProdMaxSites = 64;  // LDS-resident solve: 2 * 64^2 complex
+
+struct ProdArgs {
+    int S, N1, N2, nu, nv, ordinary, upper;
+    double beta;
+    const double* evals;
+    const double* V;       // V[site * S + p]
+    const double* mix_re;  // S_0, row-major
+    const double* mix_im;
+    const double* node_u;
+    const double* node_v;
+    double* dst;        // samples [sitepair][node]
+    double* corner;     // corner values [sitepair] (corner launch)
+    int32_t* err;       // 1 = singular pivot
+};
+
+__device__ inline void cmul_(double ar, double ai, double br, double bi, double& r, double& i) {
+    r = ar * br - ai * bi;
+    i = ar * bi + ai * br;
+}
+
+__device__ inline void cdiv_(double ar, double ai, double br, double bi, double& r, double& i) {
+    const double den = br * br + bi * bi;
+    r = (ar * br + ai * bi) / den;
+    i = (ai * br - ar * bi) / den;
+}
+
+__global__ __launch_bounds__(256) void k_node_green(ProdArgs a, int node_begin, int corner_mode) {
+    extern __shared__ double sh[];
+    const int S = a.S;
+    const int S2 = S * S;
+    double* Mr = sh;
+    double* Mi = Mr + S2;
+    double* Rr = Mi + S2;
+    double* Ri = Rr + S2;
+    double* left = Ri + S2;
+    double* colr = left + S;
+    double* coli = colr + S;
+    __shared__ int s_piv;
+    __shared__ double s_pabs[256];
+    __shared__ int s_prow[256];
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const bool upper = a.upper != 0;
+    const int node = node_begin + blockIdx.x;
+    double tau = 0.0, taup = 0.0;
+    if (!corner_mode) {
+        const int i = node / a.nv;
+        const int j = node - i * a.nv;
+        const double cu = a.node_u[i];
+        const double cv = a.node_v[j];
+        if (upper) {
+            tau = a.beta * (1.0 + cu) / 2.0;
+            taup = tau * (1.0 + cv) / 2.0;
+        } else {
+            taup = a.beta * (1.0 + cu) / 2.0;
+            tau = taup * (1.0 + cv) / 2.0;
+        }
+    }
+    const double tf = tau / a.beta;
+    const double tpf = taup / a.beta;
+    const double theta = upper ? (tf + tpf) / 2.0 : 1.0 - (tf + tpf) / 2.0;
+    // middle and diagonal right-hand side (replica_free_greens.h:445-476)
+    for (int e = tid; e < S2; e += nt) {
+        const int r = e / S;
+        const int c = e - r * S;
+        const double xr = a.beta * a.evals[r];
+        const double xc = a.beta * a.evals[c];
+        const double sxr = upper ? xr : -xr;
+        const double sxc = upper ? xc : -xc;
+        const double cr = fmax(0.0, sxr);
+        const double cc = fmax(0.0, sxc);
+        const double rs = exp(-theta * cr);
+        const double cs = exp(-(1.0 - theta) * cc);
+        double mr, mi;
+        if (upper) {
+            mr = a.mix_re[r * S + c];
+            mi = a.mix_im[r * S + c];
+        } else {
+            mr = a.mix_re[c * S + r];
+            mi = -a.mix_im[c * S + r];
+        }
+        Mr[e] = (rs * mr) * cs;
+        Mi[e] = (rs * mi) * cs;
+        if (r == c) Mr[e] += exp(sxr - cr);
+        Rr[e] = 0.0;
+        Ri[e] = 0.0;
+    }
+    for (int r = tid; r < S; r += nt) {
+        const double x = a.beta * a.evals[r];
+        const double sx = upper ? x : -x;
+        const double c = fmax(0.0, sx);
+        double le, re;
+        if (upper) {
+            le = (1.0 - tf) * x - (1.0 - theta) * c;
+            re = tpf * x - theta * c;
+        } else {
+            le = tf * sx - (1.0 - theta) * c;
+            re = (1.0 - tpf) * sx - theta * c;
+        }
+        left[r] = upper ? exp(le) : -exp(le);
+        Rr[r * S + r] = exp(re);
+    }
+    __syncthreads();
+    // Pivoted complex Gauss-Jordan (fermionic_graded_complex.h:304-361).
+    for (int col = 0; col < S; ++col) {
+        double best = -1.0;
+        int brow = S;
+        for (int r = col + tid; r < S; r += nt) {
+            const double v = sqrt(Mr[r * S + col] * Mr[r * S + col] + Mi[r * S + col] * Mi[r * S + col]);
+            if (v > best) {
+                best = v;
+                brow = r;
+            }
+        }
+        s_pabs[tid] = best;
+        s_prow[tid] = brow;
+        __syncthreads();
+        if (tid == 0) {
+            double pb = -1.0;
+            int pr = S;
+            for (int t = 0; t < nt; ++t) {
+                if (s_pabs[t] > pb || (s_pabs[t] == pb && s_prow[t] < pr)) {
+                    pb = s_pabs[t];
+                    pr = s_prow[t];
+                }
+            }
+            if (pb <= 1e-12) atomicOr(a.err, 1);
+            s_piv = pr;
+        }
+        __syncthreads();
+        const int prow = s_piv;
+        if (prow != col) {
+            for (int j = tid; j < S; j += nt) {
+                double t;
+                t = Mr[col * S + j]; Mr[col * S + j] = Mr[prow * S + j]; Mr[prow * S + j] = t;
+                t = Mi[col * S + j]; Mi[col * S + j] = Mi[prow * S + j]; Mi[prow * S + j] = t;
+                t = Rr[col * S + j]; Rr[col * S + j] = Rr[prow * S + j]; Rr[prow * S + j] = t;
+                t = Ri[col * S + j]; Ri[col * S + j] = Ri[prow * S + j]; Ri[prow * S + j] = t;
+            }
+            __syncthreads();
+        }
+        const double pr_ = Mr[col * S + col];
+        const double pi_ = Mi[col * S + col];
+        __syncthreads();
+        for (int j = tid; j < 2 * S; j += nt) {
+            double* xr = j < S ? Mr : Rr;
+            double* xi = j < S ? Mi : Ri;
+            const int jj = j < S ? j : j - S;
+            if (j < S && jj < col) continue;
+            double r, i;
+            cdiv_(xr[col * S + jj], xi[col * S + jj], pr_, pi_, r, i);
+            xr[col * S + jj] = r;
+            xi[col * S + jj] = i;
+        }
+        for (int r = tid; r < S; r += nt) {
+            colr[r] = Mr[r * S + col];
+            coli[r] = Mi[r * S + col];
+        }
+        __syncthreads();
+        for (int e = tid; e < S * 2 * S; e += nt) {
+            const int r = e / (2 * S);
+            const int j = e - r * 2 * S;
+            if (r == col) continue;
+            const double fr = colr[r], fi = coli[r];
+            if (fr == 0.0 && fi == 0.0) continue;
+            double* xr = j < S ? Mr : Rr;
+            double* xi = j < S ? Mi : Ri;
+            const int jj = j < S ? j : j - S;
+            if (j < S && jj < col) continue;
+            double pr2, pi2;
+            cmul_(fr, fi, xr[col * S + jj], xi[col * S + jj], pr2, pi2);
+            xr[r * S + jj] -= pr2;
+            xi[r * S + jj] -= pi2;
+        }
+        __syncthreads();
+    }
+    // K = diag(left) X, then T = V K (into M), G = T V^T (into R).
+    for (int e = tid; e < S2; e += nt) {
+        const int r = e / S;
+        Rr[e] *= left[r];
+        Ri[e] *= left[r];
+    }
+    __syncthreads();
+    for (int e = tid; e < S2; e += nt) {
+        const int i = e / S;
+        const int q = e - i * S;
+        double tr = 0.0, ti = 0.0;
+        for (int p = 0; p < S; ++p) {
+            const double v = a.V[i * S + p];
+            tr = __builtin_fma(v, Rr[p * S + q], tr);
+            ti = __builtin_fma(v, Ri[p * S + q], ti);
+        }
+        Mr[e] = tr;
+        Mi[e] = ti;
+    }
+    __syncthreads();
+    for (int e = tid; e < S2; e += nt) {
+        const int i = e / S;
+        const int jx = e - i * S;
+        double gr = 0.0, gi = 0.0;
+        for (int q = 0; q < S; ++q) {
+            const double v = a.V[jx * S + q];
+            gr = __builtin_fma(Mr[i * S + q], v, gr);
+            gi = __builtin_fma(Mi[i * S + q], v, gi);
+        }
+        Rr[e] = gr;
+        Ri[e] = gi;
+    }
+    __syncthreads();
+    const int64_t stride = corner_mode ? 1 : static_cast<int64_t>(a.nu) * a.nv;
+    double* out = corner_mode ? a.corner : a.dst + node;
+    if (a.ordinary) {
+        for (int e = tid; e < S2; e += nt) out[static_cast<int64_t>(e) * stride] = Rr[e];
+        return;
+    }
+    const int SP = 4 * S2;
+    for (int f = tid; f < SP; f += nt) {
+        int g = f;
+        const int y2 = g % a.N2;
+        g /= a.N2;
+        const int x2 = g % a.N1;
+        g /= a.N1;
+        const int y1 = g % a.N2;
+        g /= a.N2;
+        const int x1 = g % a.N1;
+        g /= a.N1;
+        const int rb = g & 1;
+        const int ra = g >> 1;
+        const int s1 = y1 * a.N1 + x1;
+        const int s2 = y2 * a.N1 + x2;
+        const double re = Rr[s1 * S + s2];
+        const double im = Ri[s1 * S + s2];
+        const double v = (ra == rb) ? re : (ra == 0 ? -im : im);
+        out[static_cast<int64_t>(f) * stride] = v;
+    }
+}
+
+// In-place separable cosine transform of one sitepair's nu x nv sample grid.
+__global__ __launch_bounds__(256) void k_dct(double* __restrict__ tab, int nu, int nv, const double* __restrict__ cos_u,
+                                             const double* __restrict__ cos_v, double* __restrict__ gtmp) {
+    extern __shared__ double tmp_lds[];
+    const int64_t sp = blockIdx.x;
+    double* blk = tab + sp * static_cast<int64_t>(nu) * nv;
+    double* tmp = gtmp ? gtmp + sp * static_cast<int64_t>(nu) * nv : tmp_lds;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nu * nv; e += blockDim.x) {
+        const int i = e / nv;
+        const int b = e - i * nv;
+        const double pv = (b == 0 ? 1.0 : 2.0) / static_cast<double>(nv);
+        double t = 0.0;
+        for (int j = 0; j < nv; ++j) t += blk[i * nv + j] * cos_v[b * nv + j];
+        tmp[e] = pv * t;
+    }
+    __syncthreads();
+    for (int e = tid; e < nu * nv; e += blockDim.x) {
+        const int aa = e / nv;
+        const int b = e - aa * nv;
+        const double pu = (aa == 0 ? 1.0 : 2.0) / static_cast<double>(nu);
+        double t = 0.0;
+        for (int i = 0; i < nu; ++i) t += tmp[i * nv + b] * cos_u[aa * nu + i];
+        blk[e] = pu * t;
+    }
+}
+
+// Cyclic Jacobi eigendecomposition of a real symmetric matrix (row-major),
+// eigenvalues ascending, eigenvectors in columns of V.
+void host_jacobi(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V) {
+    V.assign(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) V[static_cast<size_t>(i) * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double x = A[static_cast<size_t>(i) * n + j];
+                tot += x * x;
+                if (i != j) off += x * x;
+            }
+        if (off <= 1e-32 * std::max(tot, 1e-300)) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[static_cast<size_t>(p) * n + q];
+                if (apq == 0.0) continue;
+                const double app = A[static_cast<size_t>(p) * n + p];
+                const double aqq = A[static_cast<size_t>(q) * n + q];
+                const double th = (aqq - app) / (2.0 * apq);
+                const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0);
+                const double s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[static_cast<size_t>(k) * n + p];
+                    const double akq = A[static_cast<size_t>(k) * n + q];
+                    A[static_cast<size_t>(k) * n + p] = c * akp - s * akq;
+                    A[static_cast<size_t>(k) * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[static_cast<size_t>(p) * n + k];
+                    const double aqk = A[static_cast<size_t>(q) * n + k];
+                    A[static_cast<size_t>(p) * n + k] = c * apk - s * aqk;
+                    A[static_cast<size_t>(q) * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[static_cast<size_t>(k) * n + p];
+                    const double vkq = V[static_cast<size_t>(k) * n + q];
+                    V[static_cast<size_t>(k) * n + p] = c * vkp - s * vkq;
+                    V[static_cast<size_t>(k) * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> ord(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) ord[static_cast<size_t>(i)] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int l, int r) {
+        return A[static_cast<size_t>(l) * n + l] < A[static_cast<size_t>(r) * n + r];
+    });
+    ev.assign(static_cast<size_t>(n), 0.0);
+    std::vector<double> W(static_cast<size_t>(n) * n);
+    for (int c = 0; c < n; ++c) {
+        const int o = ord[static_cast<size_t>(c)];
+        ev[static_cast<size_t>(c)] = A[static_cast<size_t>(o) * n + o];
+        for (int r = 0; r < n; ++r) W[static_cast<size_t>(r) * n + c] = V[static_cast<size_t>(r) * n + o];
+    }
+    V.swap(W);
+}
+
+bool host_site_in_mask(int x, int y, int N1, int N2, int kind, int param) {
+    switch (kind) {
+        case 0: return false;
+        case 1: return ((x + y) % 2) == 0;
+        case 2: return y == 0;
+        case 3: return x == 0;
+        case 4: {
+            auto in_iv = [&](int c, int extent) {
+                if (param <= 0) return false;
+                if (2 * param >= extent) return true;
+                return c < param || c >= extent - param;
+            };
+            return in_iv(x, N1) && in_iv(y, N2);
+        }
+        case 5: return x < param;
+        default: return false;
+    }
+}
+
+// One table kind: twisted (mask as given) or ordinary (empty mask, replica
+// (0,0) block in site-major order).
+int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const cdet_lattice* lat) {
+    const cdet_params& p = ctx->p;
+    const int N1 = p.N1, N2 = p.N2, S = N1 * N2;
+    if (S > kProdMaxSites)
+        return fail(CDET_ERR_CAPACITY, "cdet_tables_build: lattices above 64 sites are not supported by this build");
+    // Square-lattice Hamiltonian (replica_free_greens.h:133-169).
+    std::vector<double> H(static_cast<size_t>(S) * S, 0.0);
+    for (int y = 0; y < N2; ++y)
+        for (int x = 0; x < N1; ++x) {
+            const int i = y * N1 + x;
+            H[static_cast<size_t>(i) * S + i] = -mu_eff;
+            int nb[2][2], nn = 0;
+            if (x + 1 < N1) { nb[nn][0] = x + 1; nb[nn][1] = y; ++nn; }
+            else if (lat->pbc) { nb[nn][0] = 0; nb[nn][1] = y; ++nn; }
+            if (y + 1 < N2) { nb[nn][0] = x; nb[nn][1] = y + 1; ++nn; }
+            else if (lat->pbc) { nb[nn][0] = x; nb[nn][1] = 0; ++nn; }
+            for (int k = 0; k < nn; ++k) {
+                const int j = nb[k][1] * N1 + nb[k][0];
+                H[static_cast<size_t>(i) * S + j] -= 1.0;
+                H[static_cast<size_t>(j) * S + i] -= 1.0;
+            }
+        }
+    std::vector<double> ev, V;
+    host_jacobi(H, S, ev, V);
+    // S_0 = V^T Omega_0^{-1} V (Omega_0^{-1} = -i on the subsystem, 1 elsewhere).
+    std::vector<double> mre(static_cast<size_t>(S) * S, 0.0), mim(static_cast<size_t>(S) * S, 0.0);
+    for (int site = 0; site < S; ++site) {
+        const bool in = !ordinary && host_site_in_mask(site % N1, site / N1, N1, N2, lat->mask_kind, lat->mask_param);
+        for (int a = 0; a < S; ++a)
+            for (int b = 0; b < S; ++b) {
+                const double w = V[static_cast<size_t>(site) * S + a] * V[static_cast<size_t>(site) * S + b];
+                if (in)
+                    mim[static_cast<size_t>(a) * S + b] -= w;
+                else
+                    mre[static_cast<size_t>(a) * S + b] += w;
+            }
+    }
+    std::vector<double> nu_nodes(static_cast<size_t>(p.cheb_nu)), nv_nodes(static_cast<size_t>(p.cheb_nv));
+    std::vector<double> cu(static_cast<size_t>(p.cheb_nu) * p.cheb_nu), cv(static_cast<size_t>(p.cheb_nv) * p.cheb_nv);
+    const double pi = M_PI;
+    for (int i = 0; i < p.cheb_nu; ++i) {
+        const double th = (static_cast<double>(2 * i + 1) * pi) / static_cast<double>(2 * p.cheb_nu);
+        nu_nodes[static_cast<size_t>(i)] = std::cos(th);
+        for (int aa = 0; aa < p.cheb_nu; ++aa) cu[static_cast<size_t>(aa) * p.cheb_nu + i] = std::cos(static_cast<double>(aa) * th);
+    }
+    for (int j = 0; j < p.cheb_nv; ++j) {
+        const double th = (static_cast<double>(2 * j + 1) * pi) / static_cast<double>(2 * p.cheb_nv);
+        nv_nodes[static_cast<size_t>(j)] = std::cos(th);
+        for (int b = 0; b < p.cheb_nv; ++b) cv[static_cast<size_t>(b) * p.cheb_nv + j] = std::cos(static_cast<double>(b) * th);
+    }
+    double *d_ev, *d_V, *d_mre, *d_mim, *d_nu, *d_nv, *d_cu, *d_cv;
+    int32_t* d_err;
+    HIP_TRY(hipMalloc(&d_ev, sizeof(double) * S));
+    HIP_TRY(hipMalloc(&d_V, sizeof(double) * S * S));
+    HIP_TRY(hipMalloc(&d_mre, sizeof(double) * S * S));
+    HIP_TRY(hipMalloc(&d_mim, sizeof(double) * S * S));
+    HIP_TRY(hipMalloc(&d_nu, sizeof(double) * nu_nodes.size()));
+    HIP_TRY(hipMalloc(&d_nv, sizeof(double) * nv_nodes.size()));
+    HIP_TRY(hipMalloc(&d_cu, sizeof(double) * cu.size()));
+    HIP_TRY(hipMalloc(&d_cv, sizeof(double) * cv.size()));
+    HIP_TRY(hipMalloc(&d_err, sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(d_ev, ev.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_V, V.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_mre, mre.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_mim, mim.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_nu, nu_nodes.data(), sizeof(double) * nu_nodes.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_nv, nv_nodes.data(), sizeof(double) * nv_nodes.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_cu, cu.data(), sizeof(double) * cu.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_cv, cv.data(), sizeof(double) * cv.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d_err, 0, sizeof(int32_t)));
+
+    int rc = alloc_table(ctx, kind);
+    if (rc) return rc;
+    DevTable& t = ctx->tab[kind];
+    size_t ncoef, ncorner;
+    table_sizes(ctx, ncoef, ncorner);
+    HIP_TRY(hipMemset(t.upper, 0, ncoef * sizeof(double)));
+    HIP_TRY(hipMemset(t.lower, 0, ncoef * sizeof(double)));
+    HIP_TRY(hipMemset(t.ucorner, 0, ncorner * sizeof(double)));
+    HIP_TRY(hipMemset(t.lcorner, 0, ncorner * sizeof(double)));
+
+    const size_t lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kMaxLds)));
+    const int nodes = p.cheb_nu * p.cheb_nv;
+    const int chunk = lat->node_chunk > 0 ? lat->node_chunk : nodes;
+    const int64_t active = ordinary ? static_cast<int64_t>(S) * S : static_cast<int64_t>(4) * S * S;
+    const size_t blk = static_cast<size_t>(p.cheb_nu) * p.cheb_nv;
+    double* gtmp = nullptr;
+    size_t dct_lds = blk * sizeof(double);
+    if (dct_lds > kMaxLds) {
+        HIP_TRY(hipMalloc(&gtmp, blk * sizeof(double) * static_cast<size_t>(active)));
+        dct_lds = 0;
+    } else {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(kMaxLds)));
+    }
+    hipStream_t s = ctx->stream;
+    for (int br = 0; br < 2; ++br) {
+        ProdArgs a{};
+        a.S = S;
+        a.N1 = N1;
+        a.N2 = N2;
+        a.nu = p.cheb_nu;
+        a.nv = p.cheb_nv;
+        a.ordinary = ordinary;
+        a.upper = br == 0 ? 1 : 0;
+        a.beta = p.beta;
+        a.evals = d_ev;
+        a.V = d_V;
+        a.mix_re = d_mre;
+        a.mix_im = d_mim;
+        a.node_u = d_nu;
+        a.node_v = d_nv;
+        a.dst = br == 0 ? t.upper : t.lower;
+        a.corner = br == 0 ? t.ucorner : t.lcorner;
+        a.err = d_err;
+        hipLaunchKernelGGL(k_node_green, dim3(1), dim3(256), lds, s, a, 0, 1);
+        for (int b0 = 0; b0 < nodes; b0 += chunk) {
+            const int cnt = std::min(chunk, nodes - b0);
+            hipLaunchKernelGGL(k_node_green, dim3(cnt), dim3(256), lds, s, a, b0, 0);
+        }
+        hipLaunchKernelGGL(k_dct, dim3(static_cast<unsigned>(active)), dim3(256), dct_lds, s, a.dst, p.cheb_nu,
+                           p.cheb_nv, d_cu, d_cv, gtmp);
+        HIP_TRY(hipGetLastError());
+    }
+    int32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    void* tmp[] = {d_ev, d_V, d_mre, d_mim, d_nu, d_nv, d_cu, d_cv, d_err, gtmp};
+    for (void* b : tmp)
+        if (b) (void)hipFree(b);
+    if (err) return fail(CDET_ERR_SOLVE, "Complex matrix solve is singular within the requested precision");
+    return 0;
+}
+
+}  // namespace
+
+static int build_tables_gpu(cdet_ctx* ctx, const cdet_lattice* lattice) {
+    if (!ctx || !lattice) return fail(CDET_ERR_ARGUMENT, "cdet_tables_build: bad argument");
+    const cdet_params& p = ctx->p;
+    // Twisted tables use mu - alpha_sigma with the subsystem mask, ordinary
+    // tables the same mu with the empty mask
+    // (include/cdet/chunked_load_data.h:582-604).
+    const bool equal = p.alpha_up == p.alpha_down;
+    int rc = build_one_table(ctx, CDET_TABLE_TWISTED_UP, p.mu - p.alpha_up, 0, lattice);
+    if (rc) return rc;
+    if (equal) {
+        rc = cdet_tables_alias(ctx, CDET_TABLE_TWISTED_DOWN, CDET_TABLE_TWISTED_UP);
+    } else {
+        rc = build_one_table(ctx, CDET_TABLE_TWISTED_DOWN, p.mu - p.alpha_down, 0, lattice);
+    }
+    if (rc) return rc;
+    rc = build_one_table(ctx, CDET_TABLE_ORDINARY_UP, p.mu - p.alpha_up, 1, lattice);
+    if (rc) return rc;
+    if (equal) {
+        rc = cdet_tables_alias(ctx, CDET_TABLE_ORDINARY_DOWN, CDET_TABLE_ORDINARY_UP);
+    } else {
+        rc = build_one_table(ctx, CDET_TABLE_ORDINARY_DOWN, p.mu - p.alpha_down, 1, lattice);
+    }
+    if (rc) return rc;
+    refresh_scalars(ctx);
+    return 0;
+}

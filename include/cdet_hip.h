@@ -1,0 +1,240 @@
+/*
+ * cdet_hip.h -- C-ABI of the MI355X (gfx950) connected-determinant (CDet)
+ * integrand path.
+ *
+ * The reference exposes this path only as in-process C++ templates driven by
+ * process globals.  Every entry point below replaces one reference call and
+ * takes its implicit inputs (globals in include/cdet/parameters.h:11-88 of the
+ * reference) explicitly through cdet_params / cdet_ctx.  All entry points are
+ * plain C: pointers, sizes and status codes, no C++ or torch types.
+ *
+ * Status codes: 0 = success, negative = error; the message of the most recent
+ * error on the calling host thread is returned by cdet_last_error().  The
+ * messages of errors that the reference raises as std::runtime_error carry the
+ * reference's text so that a host shim can rethrow them unchanged.
+ */
+#ifndef CDET_HIP_H
+#define CDET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define CDET_OK 0
+#define CDET_ERR_ARGUMENT (-1)      /* bad argument / shape */
+#define CDET_ERR_ORDER (-2)         /* order <= 1 where the reference throws */
+#define CDET_ERR_NONFINITE (-3)     /* "Value is not finite" throw sites */
+#define CDET_ERR_TABLES (-4)        /* table missing / mismatched */
+#define CDET_ERR_DEVICE (-5)        /* HIP runtime error */
+#define CDET_ERR_CAPACITY (-6)      /* order or size beyond the compiled maximum */
+#define CDET_ERR_SOLVE (-7)         /* producer: singular / not-real entry */
+
+/* Largest envelope order accepted by the integrand entry points. */
+#define CDET_MAX_ORDER 14
+
+/*
+ * One interaction vertex as four unit coordinates in [0,1): imaginary time,
+ * x, y, replica.  Layout-identical to the reference's UnitVertex
+ * (include/cdet/cdet_integrand_evaluator.h:16-21).
+ */
+typedef struct cdet_vertex {
+    double tau;
+    double x;
+    double y;
+    double replica;
+} cdet_vertex;
+
+/*
+ * Scalar inputs of the integrand.  Replaces the process globals read by the
+ * reference hot path (include/cdet/parameters.h:11-24) and the runtime scalar
+ * cache (include/cdet/integrand_replica_randomly_sampled.h:98-119).
+ *
+ * The three derived scalars (scale, diag_up_shift, diag_down_shift) are the
+ * double roundings of sqrt(U*N1*N2*beta), alpha_down/U and alpha_up/U.  The
+ * reference forms them in double-double from the decimal parameter text;
+ * cdet_params_derive() forms them from the double fields with 113-bit
+ * arithmetic, and a caller that holds the decimal text may overwrite them.
+ */
+typedef struct cdet_params {
+    int32_t N1;
+    int32_t N2;
+    int32_t cheb_nu;              /* coefficient count along u (degree+1) */
+    int32_t cheb_nv;              /* coefficient count along v (degree+1) */
+    double beta;
+    double U;
+    double mu;
+    double alpha_up;
+    double alpha_down;
+    double branch_epsilon;
+    double tol_fpm;
+    double fpm_global_shift;      /* < 0: automatic shift max|A|/n^2 */
+    double ztransform_epsilon0;   /* <= 0: automatic (2e-16 for double) */
+    double ztransform_safety;     /* 0 disables the round-off filter */
+    double scale;                 /* derived: sqrt(U*N1*N2*beta) */
+    double diag_up_shift;         /* derived: alpha_down/U */
+    double diag_down_shift;       /* derived: alpha_up/U */
+} cdet_params;
+
+/* Table kinds (include/cdet/chunked_load_data.h:29-34 of the reference). */
+#define CDET_TABLE_TWISTED_UP 0
+#define CDET_TABLE_TWISTED_DOWN 1
+#define CDET_TABLE_ORDINARY_UP 2
+#define CDET_TABLE_ORDINARY_DOWN 3
+
+/* Evaluation modes of cdet_eval_batch. */
+#define CDET_MODE_EXACT 0     /* FP64 Clenshaw in the reference's order: bit-identical */
+#define CDET_MODE_MFMA64 1    /* FP64 MFMA coefficient x basis contraction */
+#define CDET_MODE_SPLIT32 2   /* split-FP32 MFMA (hi/lo, three products), FP64 combine */
+
+typedef struct cdet_counters {
+    uint64_t integrand_calls;     /* include/cdet/runtime_counters.h:14 */
+    uint64_t fpm_fallbacks;       /* include/cdet/runtime_counters.h:15 */
+    uint64_t green_pair_cache_hits;
+    uint64_t green_pair_cache_misses;
+} cdet_counters;
+
+typedef struct cdet_ctx cdet_ctx;
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* cdet_last_error(void);
+
+/* Fill scale / diag_up_shift / diag_down_shift from the double fields. */
+int cdet_params_derive(cdet_params* params);
+
+/*
+ * Create a context on the current HIP device.  The context owns device copies
+ * of the tables and all scratch; nothing is allocated on the per-call path
+ * once the scratch has grown to the largest order seen.
+ */
+int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx);
+int cdet_ctx_destroy(cdet_ctx* ctx);
+
+/*
+ * Upload one resident table kind from host memory.  Layout as the reference's
+ * PiecewiseChebyshevTable (include/cdet/replica_chebyshev_tables.h:25-33):
+ * coeffs [sitepair][a][b] with b fastest, sitepair count 4*(N1*N2)^2
+ * (include/cdet/replica_chebyshev_tables.h:360-400); corners [sitepair].
+ * Replaces the resident globals g_table_* (include/cdet/parameters.h:81-84).
+ */
+int cdet_tables_upload(cdet_ctx* ctx, int kind,
+                       const double* upper_coeffs, const double* lower_coeffs,
+                       const double* upper_corners, const double* lower_corners);
+
+/* Make table kind dst share the device storage of kind src
+ * (include/cdet/load_data.h:70-91: one table reused when ALPHA_UP == ALPHA_DOWN). */
+int cdet_tables_alias(cdet_ctx* ctx, int dst_kind, int src_kind);
+
+/*
+ * Lattice description for the GPU table producer.  mask_kind follows
+ * include/cdet/replica_free_greens.h:20-26 (0 empty, 1 checkerboard,
+ * 2 first_row, 3 first_column, 4 origin_square) plus 5 = left_columns
+ * (sites with x < mask_param), used for half-strip and half-lattice regions.
+ */
+typedef struct cdet_lattice {
+    int32_t pbc;
+    int32_t mask_kind;
+    int32_t mask_param;           /* origin_square radius / left_columns width */
+    int32_t node_chunk;           /* Chebyshev nodes per producer launch; <= 0: all */
+} cdet_lattice;
+
+/*
+ * Build the twisted and ordinary tables on the GPU in FP64 (the reference's
+ * load_data(), include/cdet/load_data.h:49-99 with build_green_function_table,
+ * include/cdet/replica_chebyshev_tables.h:592-793).  When alpha_up ==
+ * alpha_down one table per class is built and aliased.
+ */
+int cdet_tables_build(cdet_ctx* ctx, const cdet_lattice* lattice);
+
+/* Copy a device table kind back to host buffers (same layout as upload). */
+int cdet_tables_download(cdet_ctx* ctx, int kind,
+                         double* upper_coeffs, double* lower_coeffs,
+                         double* upper_corners, double* lower_corners);
+
+/*
+ * Single-order integrand C(n) of all n vertices, twisted - 2*ordinary combined
+ * in double-double and rounded.  Replaces
+ * evaluate_cdet_density_integrand_double
+ * (include/cdet/cdet_integrand_evaluator.h:542-560).  n >= 1.
+ */
+int cdet_eval_single(cdet_ctx* ctx, const cdet_vertex* vertices, int n, double* out_value);
+
+/*
+ * All-subset integrand: out[mask] for every mask of the E-vertex envelope
+ * (out has 2^E doubles, out[0] = 0).  Replaces
+ * evaluate_cdet_density_all_subset_integrands_double_into
+ * (include/cdet/cdet_integrand_evaluator.h:562-583).  E >= 2.
+ * Plain form: one configuration per launch sequence, FP64, bit-identical to
+ * the reference arithmetic.
+ */
+int cdet_eval_all_subsets(cdet_ctx* ctx, const cdet_vertex* vertices, int envelope_order,
+                          double* out_values);
+
+/*
+ * Batched evaluation.  A plan fixes the order of every configuration
+ * (h_orders: host int32[count]); configurations are packed in the given order:
+ * configuration c owns vertices [sum_{c'<c} n_c', ... + n_c) and, for the
+ * all-subset kind, outputs [sum_{c'<c} 2^n_c', ... + 2^n_c) (out[mask] as
+ * cdet_eval_all_subsets); for the single-order kind one output per
+ * configuration.  mode is one of CDET_MODE_*; eval_kind 0 = all subsets
+ * (orders 2..CDET_MAX_ORDER), 1 = single order (orders 1..CDET_MAX_ORDER).
+ * Batched form of the reference's per-chain calls
+ * (include/cdet/mcmcmc_sampler.h:538, :624-625; cdet_integrand_evaluator.h:542-583).
+ */
+typedef struct cdet_batch_plan cdet_batch_plan;
+int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* h_orders, int64_t count, int mode, int eval_kind,
+                           cdet_batch_plan** out_plan);
+int cdet_batch_plan_destroy(cdet_batch_plan* plan);
+int64_t cdet_batch_out_size(const cdet_batch_plan* plan);     /* doubles written by one run */
+int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan); /* vertices read by one run */
+int64_t cdet_batch_query_count(const cdet_batch_plan* plan);  /* Green-pair lookups per run */
+
+/*
+ * Run a plan on DEVICE buffers: d_vertices (cdet_vertex[vertex_count]),
+ * d_out (double[out_size]), d_status (int32[count] or NULL: bit 0 = a
+ * non-finite value, bit 1 = FPM fallback taken).  Asynchronous on `stream`
+ * (hipStream_t; NULL = the context stream).  With use_graph != 0 the launch
+ * sequence is captured once per (buffers, stream) into a hipGraph and
+ * replayed.  Status reports launch errors only.
+ */
+int cdet_batch_run(cdet_batch_plan* plan, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                   int use_graph, void* stream);
+
+/*
+ * Eager run with HIP events between the three stages, on `stream`: stage_ms[0]
+ * = lookup planning and grouping, [1] = coefficient x basis contraction,
+ * [2] = minors, subset recursion and assembly.  Synchronises.  For
+ * measurement; results are identical to cdet_batch_run.
+ */
+int cdet_batch_profile(cdet_batch_plan* plan, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                       void* stream, float* stage_ms);
+
+/* One-shot batched all-subset evaluation: plan, run, wait, free. */
+int cdet_eval_batch(cdet_ctx* ctx, const cdet_vertex* d_vertices, const int32_t* h_orders, int64_t count,
+                    double* d_out, int32_t* d_status, int mode, void* stream);
+
+/*
+ * Table evaluation only: (G_up, G_down) for `count` lookups, device pointers.
+ * Each query is (kind, replica_alpha, replica_beta, x1, y1, x2, y2) packed in
+ * int32[7] and (tau1, tau2) in double[2]; upper branch iff tau1 > tau2.
+ * Replaces prepare_green_lookup_descriptor + evaluate_green_pair_descriptor
+ * (include/cdet/g_funcs.h:609-653, :793-818).  out holds 2 doubles per query.
+ */
+int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus,
+                         int64_t count, double* out, int mode, void* stream);
+
+/* Runtime counters (include/cdet/runtime_counters.h:7-51). */
+int cdet_get_counters(cdet_ctx* ctx, cdet_counters* out);
+int cdet_reset_counters(cdet_ctx* ctx);
+
+/* Context stream (hipStream_t) on which the library launches by default. */
+void* cdet_ctx_stream(cdet_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDET_HIP_H */

@@ -1,0 +1,265 @@
+"""GPU parity: the HIP path against the oracle on the same seeded inputs.
+
+FP64 exact paths (plain one-configuration form, batched EXACT mode, table
+evaluation) must be bit-identical to the oracle.  The MFMA modes are checked
+against the tolerance stated per test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cdet_amd
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+_ctx_cache = {}
+
+
+def gpu_ctx(name, **over):
+    key = (name, tuple(sorted(over.items())))
+    if key not in _ctx_cache:
+        s = dict(helpers.SETS[name])
+        op = helpers.params_of(name)
+        p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"],
+                                 s["alpha_down"], derived=(op.scale, op.diag_up_shift, op.diag_down_shift),
+                                 **over)
+        ctx = cdet_amd.Context(p)
+        ctx.upload_tables(helpers.tables_of(name))
+        _ctx_cache[key] = ctx
+    return _ctx_cache[key]
+
+
+def oracle_for(name, **over):
+    from oracle import oracle as O
+    s = helpers.SETS[name]
+    p = O.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"],
+                      **{{"tol_fpm": "tol_fpm"}.get(k, k): v for k, v in over.items()})
+    return O.Oracle(p, helpers.tables_of(name))
+
+
+def bits_equal(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("name", ["ref2x2", "ref2x2_eq", "c1"])
+def test_plain_all_subsets_bit_identical(gpu_available, name):
+    ctx = gpu_ctx(name)
+    orc = helpers.oracle_of(name)
+    orders = [2, 3, 4, 5, 6, 7, 7]
+    for k, v in enumerate(helpers.seeded_configs(1000 + len(name), orders)):
+        say(f"plain all-subset {name} config {k} order {len(v)}")
+        got = ctx.evaluate_all_subset_integrands(v)
+        want = orc.all_subsets(v)
+        assert bits_equal(got, want), (name, k, np.max(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("name", ["ref2x2", "c1"])
+def test_plain_single_bit_identical(gpu_available, name):
+    ctx = gpu_ctx(name)
+    orc = helpers.oracle_of(name)
+    orders = [1, 1, 2, 3, 4, 5, 6, 7]
+    for k, v in enumerate(helpers.seeded_configs(2000 + len(name), orders)):
+        say(f"plain single {name} config {k} order {len(v)}")
+        got = ctx.evaluate_integrand(v)
+        want = orc.single(v)
+        assert bits_equal(got, want), (name, k, got, want)
+
+
+def test_reference_envelopes_on_gpu(gpu_available):
+    # the reference's all-subset vs single-order test, run through the GPU path
+    from oracle import oracle as O
+    ctx = gpu_ctx("ref2x2")
+    u = O.uniform_stream(0x4DCD173B, 0.01, 0.99, 5 * 6 * 4).reshape(5, 6, 4)
+    for env in u:
+        say("reference envelope")
+        allv = ctx.evaluate_all_subset_integrands(env)
+        for m in range(1, 64):
+            sel = env[[i for i in range(6) if m >> i & 1]]
+            d = ctx.evaluate_integrand(sel)
+            assert abs(allv[m] - d) <= 2e-9 + 2e-9 * max(1.0, abs(allv[m]), abs(d))
+
+
+def run_batch(ctx, configs, mode, single=False, graph=False, repeats=1):
+    import torch
+    orders = [len(v) for v in configs]
+    plan = ctx.plan(orders, mode=mode, single=single)
+    verts = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    status = torch.zeros(len(orders), dtype=torch.int32, device="cuda")
+    results = []
+    for _ in range(repeats):
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run(verts, out, status, use_graph=graph, stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        results.append(out.cpu().numpy().copy())
+    plan.close()
+    _, ob = cdet_amd.packed_offsets(orders, single)
+    return results, ob, status.cpu().numpy()
+
+
+@pytest.mark.parametrize("name", ["ref2x2", "ref2x2_eq", "c1"])
+def test_batched_exact_bit_identical(gpu_available, name):
+    ctx = gpu_ctx(name)
+    orc = helpers.oracle_of(name)
+    rng = np.random.default_rng(5)
+    orders = list(rng.integers(2, 9, size=48))
+    configs = helpers.seeded_configs(3000 + len(name), orders)
+    say(f"batched exact {name}: {len(configs)} configs")
+    (out,), ob, status = run_batch(ctx, configs, "exact")
+    assert not np.any(status & 1)
+    for k, v in enumerate(configs):
+        want = orc.all_subsets(v)
+        assert bits_equal(out[ob[k]:ob[k + 1]], want), (name, k)
+    say(f"batched exact single {name}")
+    orders1 = list(rng.integers(1, 9, size=32))
+    configs1 = helpers.seeded_configs(4000 + len(name), orders1)
+    (out1,), _, _ = run_batch(ctx, configs1, "exact", single=True)
+    for k, v in enumerate(configs1):
+        assert bits_equal(out1[k], orc.single(v)), (name, k)
+
+
+def test_batched_graph_replays_match_eager(gpu_available):
+    ctx = gpu_ctx("c1")
+    orders = [2, 5, 7, 3, 8, 6, 4, 7]
+    configs = helpers.seeded_configs(77, orders)
+    say("graph replay c1")
+    eager, ob, _ = run_batch(ctx, configs, "exact")
+    replays, _, _ = run_batch(ctx, configs, "exact", graph=True, repeats=3)
+    for r in replays:
+        assert bits_equal(r, eager[0])
+
+
+def rel_err(got, want):
+    scale = np.maximum(np.abs(want), 1e-300)
+    return np.abs(got - want) / scale
+
+
+@pytest.mark.parametrize("name", ["ref2x2", "c1"])
+def test_batched_mfma64_within_tolerance(gpu_available, name):
+    # FP64 matrix-core contraction: table values agree to ~1e-15 relative; the
+    # integrand is checked at 1e-9 relative per mask, measured against the
+    # largest |value| of the configuration for masks that cancel below it.
+    ctx = gpu_ctx(name)
+    orc = helpers.oracle_of(name)
+    orders = [2, 3, 4, 5, 6, 7, 8, 8]
+    configs = helpers.seeded_configs(5000 + len(name), orders)
+    say(f"batched mfma64 {name}")
+    (out,), ob, _ = run_batch(ctx, configs, "mfma64")
+    worst = 0.0
+    for k, v in enumerate(configs):
+        want = orc.all_subsets(v)
+        got = out[ob[k]:ob[k + 1]]
+        scale = np.maximum(np.abs(want), 1e-6 * np.max(np.abs(want)))
+        err = np.max(np.abs(got - want) / scale)
+        worst = max(worst, err)
+    say(f"mfma64 {name} worst scaled error {worst:.3e}")
+    assert worst < 1e-9
+
+
+def test_split32_runs_and_reports_error(gpu_available):
+    ctx = gpu_ctx("c1")
+    orc = helpers.oracle_of("c1")
+    configs = helpers.seeded_configs(6000, [4, 6, 8])
+    say("batched split32 c1")
+    (out,), ob, _ = run_batch(ctx, configs, "split32")
+    assert np.all(np.isfinite(out))
+    errs = []
+    for k, v in enumerate(configs):
+        want = orc.all_subsets(v)
+        got = out[ob[k]:ob[k + 1]]
+        scale = np.maximum(np.abs(want), 1e-6 * np.max(np.abs(want)))
+        errs.append(np.max(np.abs(got - want) / scale))
+    say(f"split32 c1 scaled errors {errs}")
+    assert max(errs) < 1e-3  # split-FP32 accumulates at 2^-24 per K step
+
+
+def test_fpm_fallback_path_bit_identical(gpu_available):
+    # A large TOL_FPM forces the pivoted-determinant fallback
+    # (include/cdet/fast_principle_minors.h:180-186) on most matrices.
+    ctx = gpu_ctx("c1", tol_fpm=0.9)
+    orc = oracle_for("c1", tol_fpm=0.9)
+    ctx.reset_counters()
+    configs = helpers.seeded_configs(7000, [3, 5, 6])
+    for v in configs:
+        say(f"fallback order {len(v)}")
+        got = ctx.evaluate_all_subset_integrands(v)
+        assert bits_equal(got, orc.all_subsets(v))
+    assert ctx.counters()["fpm_fallbacks"] > 0
+    assert orc.fpm_fallbacks > 0
+
+
+def test_high_orders_global_scratch_bit_identical(gpu_available):
+    ctx = gpu_ctx("c1")
+    orc = helpers.oracle_of("c1")
+    for n in (12, 13, 14):
+        v = helpers.seeded_configs(8000 + n, [n])[0]
+        say(f"order {n} plain")
+        got = ctx.evaluate_all_subset_integrands(v)
+        assert bits_equal(got, orc.all_subsets(v))
+    say("orders 12-14 batched")
+    configs = helpers.seeded_configs(8100, [12, 13, 14, 13])
+    (out,), ob, _ = run_batch(ctx, configs, "exact")
+    for k, v in enumerate(configs):
+        assert bits_equal(out[ob[k]:ob[k + 1]], orc.all_subsets(v))
+
+
+def test_nonfinite_tables_raise_reference_message(gpu_available):
+    from oracle import oracle as O
+    s = helpers.SETS["ref2x2"]
+    p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"])
+    ctx = cdet_amd.Context(p)
+    good = helpers.tables_of("ref2x2")
+    bad = O.Table(np.full_like(good[0].upper, np.inf), np.full_like(good[0].lower, np.inf),
+                  np.full_like(good[0].ucorner, np.inf), np.full_like(good[0].lcorner, np.inf))
+    ctx.upload_tables([bad, bad, good[2], good[3]])
+    say("nonfinite")
+    with pytest.raises(RuntimeError, match="not finite"):
+        ctx.evaluate_all_subset_integrands(helpers.seeded_configs(9, [4])[0])
+    with pytest.raises(RuntimeError, match="envelope order >= 2"):
+        ctx.evaluate_all_subset_integrands(np.full((1, 4), 0.5))
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["exact", "mfma64"])
+def test_green_pair_eval(gpu_available, mode):
+    import torch
+    name = "c1"
+    ctx = gpu_ctx(name)
+    orc = helpers.oracle_of(name)
+    rng = np.random.default_rng(3)
+    n = 4096
+    qi = np.zeros((n, 7), dtype=np.int32)
+    qi[:, 0] = rng.integers(0, 2, n)
+    qi[:, 1:3] = rng.integers(0, 2, (n, 2))
+    qi[qi[:, 0] == 1, 1:3] = 0
+    qi[:, 3:7] = rng.integers(0, 3, (n, 4))
+    qt = rng.uniform(0, 4.0, (n, 2))
+    qt[:5, 0] = 0.0
+    out = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    d_qi = torch.tensor(qi, device="cuda")
+    d_qt = torch.tensor(qt, device="cuda")
+    say(f"green pair eval {mode}")
+    import ctypes
+    rc = cdet_amd.lib().cdet_green_pair_eval(ctx.handle, ctypes.c_void_p(d_qi.data_ptr()),
+                                             ctypes.c_void_p(d_qt.data_ptr()), n, ctypes.c_void_p(out.data_ptr()),
+                                             cdet_amd.MODES[mode], None)
+    assert rc == 0, cdet_amd._err()
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(n, 2)
+    want = np.array([orc.green_pair(2 * qi[k, 0], 2 * qi[k, 0] + 1, *qi[k, 1:7], qt[k, 0], qt[k, 1])
+                     for k in range(n)])
+    if mode == "exact":
+        assert bits_equal(got, want)
+    else:
+        assert np.max(np.abs(got - want)) < 1e-13

@@ -25,6 +25,26 @@ FLAGS = [
 ]
 
 
+DRIVER_SRC = os.path.join(HERE, "driver", "cdet_mcmcmc.cpp")
+DRIVER_OUT = os.path.join(HERE, "libcdet_mcmcmc.so")
+# Host driver: g++/libstdc++ (the reference's toolchain), generic x86-64.
+DRIVER_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off", "-Wall"]
+
+
+def build_driver(verbose=False):
+    cxx = os.environ.get("CXX_HOST", "g++")
+    tmp = DRIVER_OUT + ".tmp"
+    cmd = [cxx] + DRIVER_FLAGS + ["-o", tmp, DRIVER_SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed")
+    os.replace(tmp, DRIVER_OUT)
+    return DRIVER_OUT
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
@@ -36,6 +56,7 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed")
     os.replace(tmp, OUT)
+    build_driver(verbose)
     return OUT
 
 

@@ -85,6 +85,8 @@ def _signatures():
         "cdet_batch_profile": (i, [vp, vp, vp, vp, vp, P(ctypes.c_float)]),
         "cdet_eval_batch": (i, [vp, vp, P(ctypes.c_int32), i64, vp, vp, i, vp]),
         "cdet_green_pair_eval": (i, [vp, vp, vp, i64, vp, i, vp]),
+        "cdet_eval_batch_host": (i, [vp, vp, P(ctypes.c_int32), i64, dp]),
+        "cdet_ctx_set_batch_mode": (i, [vp, i]),
         "cdet_get_counters": (i, [vp, P(CdetCounters)]),
         "cdet_reset_counters": (i, [vp]),
     }

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/cdet_hip.h"
+#include "../../include/cdet_mcmcmc.h"
 #include "cdet_cdet.hpp"
 #include "cdet_device.hpp"
 #include "cdet_scalars.hpp"
@@ -527,6 +528,16 @@ struct cdet_ctx {
     int64_t* d_zero64 = nullptr;  // pbase/obase {0}
     double* d_gscratch = nullptr;
     size_t gscratch_bytes = 0;
+    // host-buffer batch path (driver callback)
+    int batch_mode = CDET_MODE_EXACT;
+    cdet_batch_plan* host_plan = nullptr;
+    std::vector<int32_t> host_plan_orders;
+    void* d_hv = nullptr;
+    size_t d_hv_cap = 0;
+    void* d_hout = nullptr;
+    size_t d_hout_cap = 0;
+    void* d_hstatus = nullptr;
+    size_t d_hstatus_cap = 0;
 };
 
 namespace {
@@ -834,6 +845,10 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_one);
     (void)hipFree(ctx->d_zero64);
     (void)hipFree(ctx->d_gscratch);
+    if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
+    (void)hipFree(ctx->d_hv);
+    (void)hipFree(ctx->d_hout);
+    (void)hipFree(ctx->d_hstatus);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;

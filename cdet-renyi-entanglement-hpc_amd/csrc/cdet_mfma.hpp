@@ -8,9 +8,10 @@
 // the block in 16-row slabs, so only two 16x16 accumulators are live; the
 // A fragment of each K step is loaded once and used for both N tiles.
 //
-// Operand layouts of v_mfma_f64_16x16x4_f64 and v_mfma_f32_16x16x4_f32
-// (lane l, g = l / 16, j = l % 16):  A[j][g],  B[g][j],  D[4g + r][j] in
-// accumulator register r.
+// Operand layouts (lane l, g = l / 16, j = l % 16):  A[j][g] and B[g][j] for
+// both instructions; the accumulator register r of lane l holds
+//   v_mfma_f32_16x16x4_f32:  D[4 g + r][j]   (four consecutive rows)
+//   v_mfma_f64_16x16x4_f64:  D[4 r + g][j]   (rows strided by four)
 //
 // Chebyshev bases are generated in registers:  T_{k+4} = 2 T_4 T_k - T_{k-4}
 // along K (each lane owns k = 4s + g) and T_{a+16} = 2 T_16 T_a - T_{a-16}
@@ -63,15 +64,27 @@ struct BasisK {  // lane's T_k(v) along K with k = 4s + g
     }
 };
 
-struct BasisA {  // lane's T_a(u) for its four rows a = 16 m + 4 g + r
+// Row of accumulator register r in slab m: 16 m + row_in_slab(g, r).
+template <int MODE>
+__device__ inline int row_in_slab(int g, int r) {
+    return MODE == CDET_MODE_MFMA64 ? 4 * r + g : 4 * g + r;
+}
+
+template <int MODE>
+struct BasisA {  // lane's T_a(u) for its four rows a = 16 m + row_in_slab(g, r)
     double cur[4], prev[4], w;
     __device__ inline void init(double u, int g) {
         double t[17];
         cheb_0_16(u, t);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            cur[r] = sel4(g, t[r], t[4 + r], t[8 + r], t[12 + r]);
-            prev[r] = sel4(g, t[16 - r], t[12 - r], t[8 - r], t[4 - r]);
+            if (MODE == CDET_MODE_MFMA64) {  // rows g + 4 r
+                cur[r] = sel4(g, t[4 * r], t[4 * r + 1], t[4 * r + 2], t[4 * r + 3]);
+                prev[r] = sel4(g, t[16 - 4 * r], t[15 - 4 * r], t[14 - 4 * r], t[13 - 4 * r]);
+            } else {  // rows 4 g + r
+                cur[r] = sel4(g, t[r], t[4 + r], t[8 + r], t[12 + r]);
+                prev[r] = sel4(g, t[16 - r], t[12 - r], t[8 - r], t[4 - r]);
+            }
         }
         w = 2.0 * t[16];
     }
@@ -132,7 +145,7 @@ __global__ __launch_bounds__(256) void k_contract_mfma(KScalars ks, const int32_
         const int p1 = start + (q1 < len ? q1 : 0);
         const double u0 = s_u[p0], v0 = s_v[p0];
         const double u1 = s_u[p1], v1 = s_v[p1];
-        BasisA A0, A1;
+        BasisA<MODE> A0, A1;
         A0.init(u0, g);
         A1.init(u1, g);
         double G0u = 0.0, G1u = 0.0, G0d = 0.0, G1d = 0.0;
@@ -194,10 +207,10 @@ __global__ __launch_bounds__(256) void k_contract_mfma(KScalars ks, const int32_
                 B0.step();
                 B1.step();
             }
-            // Row slab contribution: rows 16 ms + 4 g + r of this lane.
+            // Row slab contribution: rows 16 ms + row_in_slab(g, r) of this lane.
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const bool ok = (16 * ms + 4 * g + r) < nu;
+                const bool ok = (16 * ms + row_in_slab<MODE>(g, r)) < nu;
                 const double w0 = ok ? A0.cur[r] : 0.0;
                 const double w1 = ok ? A1.cur[r] : 0.0;
                 G0u = __builtin_fma(w0, acc0u[r], G0u);

@@ -409,3 +409,53 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
 extern "C" int cdet_tables_build(cdet_ctx* ctx, const cdet_lattice* lattice) {
     return build_tables_gpu(ctx, lattice);
 }
+
+// ---------------------------------------------------------------------------
+// Host-buffer batch evaluation for the driver (cdet_batch_eval_fn).
+// ---------------------------------------------------------------------------
+extern "C" int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 2) return fail(CDET_ERR_ARGUMENT, "cdet_ctx_set_batch_mode: bad argument");
+    if (ctx->batch_mode != mode && ctx->host_plan) {
+        cdet_batch_plan_destroy(ctx->host_plan);
+        ctx->host_plan = nullptr;
+        ctx->host_plan_orders.clear();
+    }
+    ctx->batch_mode = mode;
+    return 0;
+}
+
+extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, const int32_t* h_orders, int64_t count,
+                                    double* h_out) {
+    cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
+    if (!ctx || !h_vertices || !h_orders || !h_out || count <= 0)
+        return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_host: bad argument");
+    const bool same = ctx->host_plan && static_cast<int64_t>(ctx->host_plan_orders.size()) == count &&
+                      std::equal(ctx->host_plan_orders.begin(), ctx->host_plan_orders.end(), h_orders);
+    if (!same) {
+        if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
+        ctx->host_plan = nullptr;
+        int rc = cdet_batch_plan_create(ctx, h_orders, count, ctx->batch_mode, 0, &ctx->host_plan);
+        if (rc) return rc;
+        ctx->host_plan_orders.assign(h_orders, h_orders + count);
+    }
+    cdet_batch_plan* p = ctx->host_plan;
+    const size_t vbytes = sizeof(cdet_vertex) * static_cast<size_t>(p->vert_total);
+    const size_t obytes = sizeof(double) * static_cast<size_t>(p->out_total);
+    const size_t sbytes = sizeof(int32_t) * static_cast<size_t>(count);
+    int rc = grow(&ctx->d_hv, ctx->d_hv_cap, vbytes);
+    if (!rc) rc = grow(&ctx->d_hout, ctx->d_hout_cap, obytes);
+    if (!rc) rc = grow(&ctx->d_hstatus, ctx->d_hstatus_cap, sbytes);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(ctx->d_hv, h_vertices, vbytes, hipMemcpyHostToDevice, s));
+    rc = cdet_batch_run(p, static_cast<const cdet_vertex*>(ctx->d_hv), static_cast<double*>(ctx->d_hout),
+                        static_cast<int32_t*>(ctx->d_hstatus), 1, s);
+    if (rc) return rc;
+    std::vector<int32_t> st(static_cast<size_t>(count));
+    HIP_TRY(hipMemcpyAsync(h_out, ctx->d_hout, obytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(st.data(), ctx->d_hstatus, sbytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int32_t x : st)
+        if (x & 1) return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
+    return 0;
+}

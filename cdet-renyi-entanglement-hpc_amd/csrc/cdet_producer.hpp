@@ -449,8 +449,9 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
     HIP_TRY(hipMemset(t.lcorner, 0, ncorner * sizeof(double)));
 
     const size_t lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(kMaxLds)));
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const int nodes = p.cheb_nu * p.cheb_nv;
     const int chunk = lat->node_chunk > 0 ? lat->node_chunk : nodes;
     const int64_t active = ordinary ? static_cast<int64_t>(S) * S : static_cast<int64_t>(4) * S * S;
@@ -460,9 +461,9 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
     if (dct_lds > kMaxLds) {
         HIP_TRY(hipMalloc(&gtmp, blk * sizeof(double) * static_cast<size_t>(active)));
         dct_lds = 0;
-    } else {
+    } else if (dct_lds > 64 * 1024) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    static_cast<int>(kMaxLds)));
+                                    static_cast<int>(dct_lds)));
     }
     hipStream_t s = ctx->stream;
     for (int br = 0; br < 2; ++br) {

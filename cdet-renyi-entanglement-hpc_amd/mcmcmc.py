@@ -1,0 +1,150 @@
+"""Host mirror of the reference's MCMCMC driver configuration and entry point.
+
+Reads the reference's params.txt keys (src/main.cpp:218-576) for the sampler
+and the integrand, and runs the lockstep driver of libcdet_mcmcmc.so
+(include/cdet_mcmcmc.h) over any batch-evaluation callback: the GPU library
+(cdet_eval_batch_host) in production, the oracle in tests.
+"""
+import ctypes
+import os
+from fractions import Fraction
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcdet_mcmcmc.so")
+MAXE = 48
+_lib = None
+
+EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                           ctypes.c_int64, ctypes.POINTER(ctypes.c_double))
+
+
+class MCMCMCConfig(ctypes.Structure):
+    """cdet_mcmcmc_config (include/cdet_mcmcmc.h)."""
+
+    _fields_ = [
+        ("order_min", ctypes.c_int32), ("order_max", ctypes.c_int32),
+        ("envelope_order", ctypes.c_int32), ("normalization_order", ctypes.c_int32),
+        ("lambdas", ctypes.c_double * (MAXE + 1)),
+        ("lambda_warmup_steps", ctypes.c_int32), ("lambda_adapt_interval", ctypes.c_int32),
+        ("lambda_adapt_rate", ctypes.c_double),
+        ("production_steps", ctypes.c_int32), ("chain_count", ctypes.c_int32),
+        ("base_seed", ctypes.c_uint64), ("seed_mode", ctypes.c_int32),
+        ("checkpoint_enabled", ctypes.c_int32), ("checkpoint_interval_steps", ctypes.c_uint64),
+        ("resume_mode", ctypes.c_int32), ("checkpoint_dir", ctypes.c_char * 1024),
+        ("stop_after_steps", ctypes.c_uint64),
+    ]
+
+
+class ChainResult(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64), ("warmup_proposed_steps", ctypes.c_uint64),
+        ("production_proposed_steps", ctypes.c_uint64),
+        ("warmup_chosen_counts", ctypes.c_uint64 * (MAXE + 1)),
+        ("production_chosen_counts", ctypes.c_uint64 * (MAXE + 1)),
+        ("final_lambdas", ctypes.c_double * (MAXE + 1)),
+        ("order_signal_mean", ctypes.c_double * (MAXE + 1)),
+        ("order_signal_stderr", ctypes.c_double * (MAXE + 1)),
+        ("order_signal_count", ctypes.c_uint64 * (MAXE + 1)),
+        ("retained_fraction_mean", ctypes.c_double),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcdet_mcmcmc.so is not built; run cdet-renyi-entanglement-hpc_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cdet_mcmcmc_run.restype = ctypes.c_int
+        L.cdet_mcmcmc_run.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.POINTER(ChainResult), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
+        L.cdet_mcmcmc_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+# ---- params.txt (src/main.cpp:218-261: KEY=value[,value...], '#' comments) ----
+def read_params(path):
+    params = {}
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line or "=" not in line:
+                continue
+            k, v = line.split("=", 1)
+            params[k.strip()] = [x.strip() for x in v.split(",")]
+    return params
+
+
+def _one(params, key, default=None):
+    if key in params:
+        return params[key][0]
+    if default is None:
+        raise KeyError(f"missing required parameter {key}")
+    return default
+
+
+def config_from_params(params, **over):
+    """MCMCMCConfig from params.txt keys and defaults (src/main.cpp:461-535)."""
+    omin = int(_one(params, "MCMCMC_ORDER_MIN", "2"))
+    omax = int(_one(params, "MCMCMC_ORDER_MAX", "4"))
+    env = int(_one(params, "MCMCMC_ENVELOPE_ORDER", str(omax + 1)))
+    norm = int(_one(params, "MCMCMC_NORMALIZATION_ORDER", str(omin)))
+    lam = [float(x) for x in params.get("MCMCMC_LAMBDAS", ["1"] * (omax - omin + 1))]
+    if len(lam) == 1:
+        lam = lam * (omax - omin + 1)
+    c = MCMCMCConfig()
+    c.order_min, c.order_max, c.envelope_order, c.normalization_order = omin, omax, env, norm
+    for o in range(omin, omax + 1):
+        c.lambdas[o] = lam[o - omin]
+    c.lambda_warmup_steps = int(_one(params, "MCMCMC_LAMBDA_WARMUP_STEPS", "10000"))
+    c.lambda_adapt_interval = int(_one(params, "MCMCMC_LAMBDA_ADAPT_INTERVAL", "1000"))
+    c.lambda_adapt_rate = float(_one(params, "MCMCMC_LAMBDA_ADAPT_RATE", "0.25"))
+    c.production_steps = int(_one(params, "MCMCMC_PRODUCTION_STEPS", "100000"))
+    c.chain_count = int(_one(params, "MCMCMC_CHAIN_COUNT", "8"))
+    c.base_seed = int(_one(params, "MCMCMC_BASE_SEED", "998875432210"))
+    c.seed_mode = {"uniform": 0, "histogram": 1}[_one(params, "MCMCMC_SEED_MODE", "uniform").lower()]
+    c.checkpoint_enabled = int(_one(params, "MCMCMC_CHECKPOINT_ENABLED", "false").lower() in ("1", "true", "yes", "on"))
+    c.checkpoint_interval_steps = int(_one(params, "MCMCMC_CHECKPOINT_INTERVAL_STEPS", "10000000"))
+    rm = _one(params, "MCMCMC_RESUME", "off").lower()
+    c.resume_mode = {"off": 0, "auto": 1, "require": 2, "required": 2}[rm]
+    c.checkpoint_dir = _one(params, "MCMCMC_CHECKPOINT_DIR", "mcmcmc_checkpoints").encode()
+    for k, v in over.items():
+        setattr(c, k, v.encode() if k == "checkpoint_dir" else v)
+    return c
+
+
+def integrand_scalars_from_params(params):
+    """Derived integrand scalars from the decimal parameter text, correctly
+    rounded (the reference forms them in double-double from 100-digit values,
+    include/cdet/integrand_replica_randomly_sampled.h:98-119)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    N1 = int(_one(params, "N1"))
+    N2 = int(_one(params, "N2"))
+    U = Fraction(_one(params, "U"))
+    beta = Fraction(_one(params, "BETA"))
+    a_up = Fraction(_one(params, "ALPHA_UP"))
+    a_dn = Fraction(_one(params, "ALPHA_DOWN"))
+    prod = U * N1 * N2 * beta
+    scale = float(Fraction((Decimal(prod.numerator) / Decimal(prod.denominator)).sqrt()))
+    return scale, float(a_dn / U), float(a_up / U)
+
+
+def run(config, eval_fn, eval_user, trace_steps=0):
+    """Run the lockstep driver; eval_fn is a ctypes function pointer (or an
+    address) of cdet_batch_eval_fn type."""
+    n = config.chain_count
+    results = (ChainResult * n)()
+    ratios = (ctypes.c_double * (MAXE + 1))()
+    trace = np.zeros((n, max(trace_steps, 1)), dtype=np.uint64)
+    fn = ctypes.cast(eval_fn, ctypes.c_void_p)
+    rc = lib().cdet_mcmcmc_run(ctypes.byref(config), fn, eval_user, results, ratios,
+                               trace.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if trace_steps else None,
+                               trace_steps)
+    if rc != 0:
+        raise RuntimeError(lib().cdet_mcmcmc_last_error().decode())
+    return list(results), np.array(ratios[:config.envelope_order + 1]), (trace if trace_steps else None)

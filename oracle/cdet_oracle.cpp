@@ -1184,6 +1184,39 @@ void orc_uniform_stream(uint64_t seed, double lo, double hi, int64_t count, doub
     for (int64_t i = 0; i < count; ++i) out[i] = dist(rng);
 }
 
+// Batch callback of the MCMCMC driver over the oracle (packed envelopes).
+int orc_eval_batch_host(void* ctx, const cdet_vertex* v, const int32_t* orders, int64_t count, double* out) {
+    int64_t vo = 0, oo = 0;
+    for (int64_t c = 0; c < count; ++c) {
+        const int rc = orc_eval_all_subsets(ctx, v + vo, orders[c], out + oo);
+        if (rc) return rc;
+        vo += orders[c];
+        oo += int64_t(1) << orders[c];
+    }
+    return 0;
+}
+
+// Synthetic product integrand of the reference's driver test
+// (tests/test_mcmcmc_synthetic.cpp:18-28): values[mask] = prod (0.5 + tau_i),
+// whose integral is 1 at every order.
+int orc_synthetic_batch_eval(void*, const cdet_vertex* v, const int32_t* orders, int64_t count, double* out) {
+    int64_t vo = 0, oo = 0;
+    for (int64_t c = 0; c < count; ++c) {
+        const int n = orders[c];
+        const uint64_t sc = uint64_t(1) << n;
+        double* vals = out + oo;
+        vals[0] = 1.0;
+        for (uint64_t m = 1; m < sc; ++m) {
+            const uint64_t bit = m & (~m + 1U);
+            const int k = __builtin_ctzll(bit);
+            vals[m] = vals[m ^ bit] * (0.5 + v[vo + k].tau);
+        }
+        vo += n;
+        oo += static_cast<int64_t>(sc);
+    }
+    return 0;
+}
+
 // Probe set of the reference's Chebyshev accuracy test
 // (tests/test_chebyshev_accuracy.cpp:132-152): out[8*k..] = tau1/beta,
 // tau2/beta, replica1, replica2, x1, y1, x2, y2.

@@ -13,6 +13,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "cdet_hip.h")).read()
+    text += "\n".join(l for l in open(os.path.join(ROOT, "include", "cdet_mcmcmc.h")).read().splitlines()
+                      if "cdet_eval_batch_host" in l or "cdet_ctx_set_batch_mode" in l)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(cdet_[a-z0-9_]+)\s*\(", text)))
 
@@ -38,3 +40,10 @@ def test_params_derive_is_host_only():
     p = cdet_amd.make_params(4, 4, 96, 6.0, 2.0, 1.0, 1.0, 1.0)
     assert p.scale == 192.0 ** 0.5
     assert p.diag_up_shift == 0.5 and p.diag_down_shift == 0.5
+
+
+def test_driver_library_exports():
+    import mcmcmc
+    lib = ctypes.CDLL(mcmcmc.LIB_PATH)
+    for name in ("cdet_mcmcmc_run", "cdet_mcmcmc_last_error"):
+        assert hasattr(lib, name)

@@ -1,0 +1,80 @@
+"""MCMCMC driver on the GPU path vs the oracle: identical chosen subsets over
+the 3x3 workload (1000 warmup + 1500 production steps, 2 chains) and
+byte-identical checkpoints, including a 137 + 1363 step resume."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import cdet_amd
+import helpers
+import mcmcmc
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def c1_config(d, **over):
+    c = mcmcmc.MCMCMCConfig()
+    c.order_min, c.order_max, c.envelope_order, c.normalization_order = 1, 6, 7, 1
+    for o in range(1, 7):
+        c.lambdas[o] = 1.0
+    c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = 1000, 200, 0.2
+    c.production_steps, c.chain_count, c.base_seed = 1500, 2, 2026101401
+    c.seed_mode = 0
+    c.checkpoint_enabled, c.checkpoint_interval_steps, c.resume_mode = 1, 1000, 0
+    c.checkpoint_dir = str(d).encode()
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c
+
+
+def chk(d):
+    return {n: open(os.path.join(d, n), "rb").read() for n in sorted(os.listdir(d)) if n.endswith(".chk")}
+
+
+def gpu_ctx():
+    s = helpers.SETS["c1"]
+    op = helpers.params_of("c1")
+    p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"],
+                             derived=(op.scale, op.diag_up_shift, op.diag_down_shift))
+    ctx = cdet_amd.Context(p)
+    ctx.upload_tables(helpers.tables_of("c1"))
+    return ctx
+
+
+def test_driver_gpu_matches_oracle_and_resumes(gpu_available, tmp_path):
+    ctx = gpu_ctx()
+    gfn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    ofn = ctypes.cast(O.lib().orc_eval_batch_host, ctypes.c_void_p)
+    orc = helpers.oracle_of("c1")
+    print("driver gpu full", flush=True)
+    rg, ratg, tg = mcmcmc.run(c1_config(tmp_path / "gpu"), gfn, ctx.handle, trace_steps=2500)
+    print("driver oracle full", flush=True)
+    ro, rato, to = mcmcmc.run(c1_config(tmp_path / "orc"), ofn, orc._h, trace_steps=2500)
+    assert np.array_equal(tg, to)
+    assert chk(tmp_path / "gpu") == chk(tmp_path / "orc")
+    assert np.array_equal(ratg[1:], rato[1:], equal_nan=True)
+    print("driver gpu resume", flush=True)
+    mcmcmc.run(c1_config(tmp_path / "res", production_steps=137), gfn, ctx.handle)
+    mcmcmc.run(c1_config(tmp_path / "res", resume_mode=2), gfn, ctx.handle)
+    assert chk(tmp_path / "res") == chk(tmp_path / "gpu")
+    for b in chk(tmp_path / "gpu").values():
+        assert b"production_proposed_steps 1500\n" in b
+    ctx.close()
+
+
+def test_driver_mfma64_decisions(gpu_available, tmp_path):
+    # FP64 matrix-core mode: report how many chosen subsets differ from the
+    # bit-exact form (values differ at round-off only).
+    ctx = gpu_ctx()
+    gfn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    _, _, t_exact = mcmcmc.run(c1_config(tmp_path / "e", checkpoint_enabled=0), gfn, ctx.handle, trace_steps=2500)
+    cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODE_MFMA64)
+    _, _, t_mfma = mcmcmc.run(c1_config(tmp_path / "m", checkpoint_enabled=0), gfn, ctx.handle, trace_steps=2500)
+    first = np.argmax(t_exact != t_mfma, axis=1) if np.any(t_exact != t_mfma) else None
+    print(f"mfma64 vs exact: differing steps {int(np.count_nonzero(t_exact != t_mfma))}, first {first}", flush=True)
+    # chains agree at least until the first round-off flip
+    assert np.array_equal(t_exact[:, :100], t_mfma[:, :100])
+    ctx.close()

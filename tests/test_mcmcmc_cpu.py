@@ -1,0 +1,111 @@
+"""Driver checks on the CPU: the lockstep MCMCMC driver over the oracle and
+over the reference's synthetic product integrand."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import helpers
+import mcmcmc
+from oracle import oracle as O
+
+
+def base_config(tmp, **over):
+    c = mcmcmc.MCMCMCConfig()
+    c.order_min, c.order_max, c.envelope_order, c.normalization_order = 1, 3, 4, 1
+    for o in (1, 2, 3):
+        c.lambdas[o] = 1.0
+    c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = 200, 60, 0.2
+    c.production_steps = 300
+    c.chain_count = 2
+    c.base_seed = 2026101401
+    c.seed_mode = 0
+    c.checkpoint_enabled = 1
+    c.checkpoint_interval_steps = 100
+    c.resume_mode = 0
+    c.checkpoint_dir = str(tmp).encode()
+    for k, v in over.items():
+        setattr(c, k, v.encode() if k == "checkpoint_dir" else v)
+    return c
+
+
+def oracle_eval():
+    orc = helpers.oracle_of("ref2x2")
+    return ctypes.cast(O.lib().orc_eval_batch_host, ctypes.c_void_p), orc
+
+
+def chk_bytes(d):
+    return {n: open(os.path.join(d, n), "rb").read() for n in sorted(os.listdir(d)) if n.endswith(".chk")}
+
+
+def test_synthetic_integral_ratios_recovered():
+    # tests/test_mcmcmc_synthetic.cpp:135-159: orders 1..4, envelope 5,
+    # product integrand with integral 1 at every order.
+    c = mcmcmc.MCMCMCConfig()
+    c.order_min, c.order_max, c.envelope_order, c.normalization_order = 1, 4, 5, 1
+    for o, l in zip((1, 2, 3, 4), (0.7, 1.1, 1.6, 2.0)):
+        c.lambdas[o] = l
+    c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = 20000, 1000, 0.25
+    c.production_steps, c.chain_count, c.base_seed = 400000, 1, 0x7812BC91
+    fn = ctypes.cast(O.lib().orc_synthetic_batch_eval, ctypes.c_void_p)
+    res, ratios, _ = mcmcmc.run(c, fn, None)
+    for o in (2, 3, 4):
+        assert abs(ratios[o] - 1.0) < 0.04, ratios
+        assert res[0].production_chosen_counts[o] > 100
+
+
+def test_checkpoint_resume_byte_identical(tmp_path):
+    # tests/test_checkpoint_resume.py: full run vs shorter run + resume with
+    # MCMCMC_RESUME=require; final checkpoints byte-identical.
+    fn, orc = oracle_eval()
+    full = tmp_path / "full"
+    part = tmp_path / "part"
+    mcmcmc.run(base_config(full), fn, orc._h)
+    mcmcmc.run(base_config(part, production_steps=137), fn, orc._h)
+    res, _, _ = mcmcmc.run(base_config(part, resume_mode=2), fn, orc._h)
+    a, b = chk_bytes(full), chk_bytes(part)
+    assert len(a) == len(b) == 2
+    for k in a:
+        assert a[k] == b[k], k
+        assert b"production_proposed_steps 300\n" in a[k]
+    assert res[0].production_proposed_steps == 300
+
+
+def test_interrupted_run_resumes_identically(tmp_path):
+    fn, orc = oracle_eval()
+    full = tmp_path / "full"
+    cut = tmp_path / "cut"
+    mcmcmc.run(base_config(full), fn, orc._h)
+    # killed after 333 steps: last periodic checkpoint at 300 completed steps
+    mcmcmc.run(base_config(cut, stop_after_steps=333), fn, orc._h)
+    mcmcmc.run(base_config(cut, resume_mode=2), fn, orc._h)
+    assert chk_bytes(full) == chk_bytes(cut)
+
+
+def test_resume_require_without_checkpoint_fails(tmp_path):
+    fn, orc = oracle_eval()
+    with pytest.raises(RuntimeError, match="checkpoint is missing"):
+        mcmcmc.run(base_config(tmp_path / "none", resume_mode=2), fn, orc._h)
+
+
+def test_lockstep_chains_are_independent(tmp_path):
+    fn, orc = oracle_eval()
+    r3, _, t3 = mcmcmc.run(base_config(tmp_path / "a", chain_count=3, checkpoint_enabled=0), fn, orc._h,
+                           trace_steps=500)
+    r1, _, t1 = mcmcmc.run(base_config(tmp_path / "b", chain_count=1, checkpoint_enabled=0), fn, orc._h,
+                           trace_steps=500)
+    assert np.array_equal(t3[0], t1[0])
+    assert list(r3[0].order_signal_mean) == list(r1[0].order_signal_mean)
+
+
+def test_params_file_parsing(tmp_path):
+    p = tmp_path / "params.txt"
+    p.write_text("# comment\nN1=3\nN2=3\nU=2.0\nBETA=4\nALPHA_UP=1\nALPHA_DOWN=1\n"
+                 "MCMCMC_ORDER_MIN=1\nMCMCMC_ORDER_MAX=6\nMCMCMC_ENVELOPE_ORDER=7\nMCMCMC_LAMBDAS=1\n"
+                 "MCMCMC_CHAIN_COUNT=2\nMCMCMC_BASE_SEED=2026101401\nMCMCMC_RESUME=required\n")
+    params = mcmcmc.read_params(str(p))
+    c = mcmcmc.config_from_params(params)
+    assert (c.order_min, c.order_max, c.envelope_order, c.normalization_order) == (1, 6, 7, 1)
+    assert list(c.lambdas[1:7]) == [1.0] * 6 and c.resume_mode == 2 and c.chain_count == 2
+    assert mcmcmc.integrand_scalars_from_params(params) == (72.0 ** 0.5, 0.5, 0.5)

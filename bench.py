@@ -17,6 +17,10 @@ Printed JSON line fields beyond the driver contract:
   cpu_baseline  the oracle (CPU restatement of the reference) on the host
                 cores, bounded sample of the same order-10 configurations
   plain_fp64    one-configuration-per-launch FP64 GPU form, evals/s
+  errors        against the bit-identical FP64 form: max relative error on
+                masks >= 1e-6 of the configuration's largest value, on
+                well-conditioned masks (sum of |terms| <= 1e4 |value|), and
+                max |error| / sum of |terms| over all masks
 """
 import argparse
 import json
@@ -49,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=64)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -130,10 +136,15 @@ def main():
     d_out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
     d_status = torch.zeros(count, dtype=torch.int32, device=dev)
 
-    # exact reference values on the GPU (bit-identical FP64 form) for the error report
+    # exact reference values on the GPU (bit-identical FP64 form) and the
+    # conditioning scale of every value (sum of |terms| the value cancels
+    # from) for the error report
     plan_exact = ctx.plan([w["order"]] * count, mode="exact")
     d_ref = torch.empty(plan_exact.out_size, dtype=torch.float64, device=dev)
     plan_exact.run(d_verts, d_ref, None, use_graph=False, stream=sptr)
+    plan_cond = ctx.plan([w["order"]] * count, mode="exact", kind=2)
+    d_cond = torch.empty(plan_cond.out_size, dtype=torch.float64, device=dev)
+    plan_cond.run(d_verts, d_cond, None, use_graph=False, stream=sptr)
 
     for _ in range(args.warmup):
         plan.run(d_verts, d_out, d_status, use_graph=True, stream=sptr)
@@ -153,7 +164,16 @@ def main():
     rel = np.abs(got - ref) / scale
     big = np.abs(ref) >= 1e-6 * np.abs(ref).max(axis=1, keepdims=True)
     max_rel = float(np.max(rel[big])) if np.any(big) else 0.0
-    status_bad = int(np.count_nonzero(d_status.cpu().numpy() & 1))
+    cond = d_cond.cpu().numpy().reshape(count, -1)
+    diff = np.abs(got - ref)
+    pos = cond > 0
+    err_vs_cond = float(np.max(diff[pos] / cond[pos])) if np.any(pos) else 0.0
+    well = pos & (np.abs(ref) > 0) & (cond <= 1e4 * np.abs(ref))
+    max_rel_well = float(np.max(rel[well])) if np.any(well) else 0.0
+    well_frac = float(np.count_nonzero(well) / max(1, np.count_nonzero(pos)))
+    st = d_status.cpu().numpy()
+    status_bad = int(np.count_nonzero(st & 1))
+    fallbacks = int(np.count_nonzero(st & 2))
 
     # stage timing with HIP events on the launch stream (dominant kernel: contraction)
     stage = np.zeros(3)
@@ -166,7 +186,7 @@ def main():
     flops = queries * (2.0 * nc * nc + 2.0 * nc)
     contract_s = stage[1] * 1e-3
     achieved = flops / contract_s / 1e12
-    kname = {"exact": "k_contract_exact", "mfma64": "k_contract_mfma<1>", "split32": "k_contract_mfma<2>"}[args.mode]
+    kname = {"exact": "k_contract_exact", "mfma64": "k_contract_f64<true>", "split32": "k_contract_mfma<2>"}[args.mode]
     roofline = {"bound": "mfma" if args.mode != "exact" else "fp64-valu", "achieved": round(achieved, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                 "traffic": traffic_from_profiles(kname), "kernel": kname,
@@ -207,11 +227,21 @@ def main():
         "plain_fp64_evals_per_s": round(plain, 1),
         "speedup_vs_plain_fp64": round(value / plain, 2),
         "max_rel_err_vs_fp64_exact": max_rel,
+        "max_rel_err_well_conditioned": max_rel_well,
+        "well_conditioned_fraction": round(well_frac, 4),
+        "max_err_over_conditioning": err_vs_cond,
         "nonfinite_configs": status_bad,
+        "fpm_fallback_configs": fallbacks,
     }
+    if args.copy_probe:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     plan.close()
     plan_exact.close()
+    plan_cond.close()
     ctx.close()
 
 

@@ -216,22 +216,24 @@ class Context:
     def reset_counters(self):
         _check(lib().cdet_reset_counters(self._h))
 
-    def plan(self, orders, mode="exact", single=False):
-        return BatchPlan(self, orders, mode, single)
+    def plan(self, orders, mode="exact", single=False, kind=None):
+        return BatchPlan(self, orders, mode, single, kind)
 
 
 class BatchPlan:
     """Order-bucketed batched evaluation over device buffers (torch tensors or
     raw device pointers)."""
 
-    def __init__(self, ctx, orders, mode="exact", single=False):
+    def __init__(self, ctx, orders, mode="exact", single=False, kind=None):
+        """kind: 0 all-subset values, 1 single-order values, 2 conditioning
+        scales of the all-subset values (defaults from `single`)."""
         self.ctx = ctx
         self.orders = np.ascontiguousarray(orders, dtype=np.int32)
         self.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         h = ctypes.c_void_p()
         _check(lib().cdet_batch_plan_create(
             ctx.handle, self.orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-            len(self.orders), self.mode, 1 if single else 0, ctypes.byref(h)))
+            len(self.orders), self.mode, (1 if single else 0) if kind is None else kind, ctypes.byref(h)))
         self._h = h
         self.out_size = lib().cdet_batch_out_size(h)
         self.vertex_count = lib().cdet_batch_vertex_count(h)

@@ -1,20 +1,63 @@
-// cdet_cdet.hpp -- connected-determinant subset recursions and output
-// assembly, workgroup-cooperative, FP64 in the reference's operation order.
+// cdet_cdet.hpp -- connected-determinant subset recursions,
+// workgroup-cooperative, FP64 in the reference's operation order.
 #pragma once
 
 #include "cdet_device.hpp"
 
 namespace cdet {
 
+// One mask of the subset recursion: acc = a[mask] - sum over the
+// anchor-containing proper subsets `anchor | s` (s over the proper subsets of
+// rest, descending) of dp[sub] * a[mask ^ sub], with the running |term| sum,
+// exactly the reference's sequence
+// (include/cdet/integrand_replica_randomly_sampled.h:326-343).  The operand
+// loads of eight consecutive terms are issued before their dependent
+// subtractions so that LDS latency overlaps; the arithmetic order is
+// unchanged.
+__device__ inline double connected_chain(const double* a, const double* dp, uint32_t mask, uint32_t anchor,
+                                         uint32_t rest, int terms, double thr, bool filter, double* rhs_out = nullptr) {
+    double acc = a[mask];
+    double rhs = fabs(a[mask]);
+    uint32_t s = (rest - 1u) & rest;
+    int done = 0;
+    for (; done + 8 <= terms; done += 8) {
+        double dv[8], av[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t sub = anchor | s;
+            dv[u] = dp[sub];
+            av[u] = a[mask ^ sub];
+            s = (s - 1u) & rest;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double term = dv[u] * av[u];
+            acc = acc - term;
+            rhs += fabs(term);
+        }
+    }
+    for (; done < terms; ++done) {
+        const uint32_t sub = anchor | s;
+        const double term = dp[sub] * a[mask ^ sub];
+        acc = acc - term;
+        rhs += fabs(term);
+        s = (s - 1u) & rest;
+    }
+    if (filter && fabs(acc) < thr * rhs) acc = 0.0;
+    if (rhs_out) *rhs_out = rhs;
+    return acc;
+}
+
 // All-subset recursion (include/cdet/integrand_replica_randomly_sampled.h:292-347).
 // On entry a[mask] = pm_up * pm_down (a[0] = 0); on exit dp[mask] holds the
 // connected value.  masks_by_pop lists every mask ordered by popcount with
-// pop_off[k] the first index of popcount k.  Each mask is one sequential
-// chain of subtractions over its anchor-containing proper subsets in
-// descending order, exactly as the reference iterates them; masks of equal
-// popcount are independent and run in parallel.
+// pop_off[k] the first index of popcount k; masks of one popcount are
+// independent and run in parallel.
+// With rhs non-null, rhs[mask] receives the sum of |a[mask]| and |terms| of
+// the mask's chain (the scale against which the round-off filter judges the
+// connected value, :330-341).
 __device__ inline void ztransform_all_block(const double* a, double* dp, int n, double thr, bool filter,
-                                            const uint16_t* masks_by_pop, const int* pop_off) {
+                                            const uint16_t* masks_by_pop, const int* pop_off, double* rhs = nullptr) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
@@ -23,23 +66,13 @@ __device__ inline void ztransform_all_block(const double* a, double* dp, int n, 
     for (int bits = 2; bits <= n; ++bits) {
         const int beg = pop_off[bits];
         const int end = pop_off[bits + 1];
+        const int terms = (1 << (bits - 1)) - 1;
         for (int it = beg + tid; it < end; it += nt) {
             const uint32_t mask = masks_by_pop[it];
             const uint32_t anchor = mask & (~mask + 1u);
-            const uint32_t rest = mask ^ anchor;
-            double acc = a[mask];
-            double rhs = fabs(a[mask]);
-            // subsets containing the anchor, descending: anchor | s, s over
-            // proper subsets of rest in descending order.
-            for (uint32_t s = (rest - 1u) & rest;; s = (s - 1u) & rest) {
-                const uint32_t sub = anchor | s;
-                const double term = dp[sub] * a[mask ^ sub];
-                acc = acc - term;
-                rhs += fabs(term);
-                if (s == 0u) break;
-            }
-            if (filter && fabs(acc) < thr * rhs) acc = 0.0;
-            dp[mask] = acc;
+            double r = 0.0;
+            dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, terms, thr, filter, rhs ? &r : nullptr);
+            if (rhs) rhs[mask] = r;
         }
         __syncthreads();
     }
@@ -57,22 +90,10 @@ __device__ inline double ztransform_single_block(const double* a, double* dp, in
     for (int bits = 2; bits <= n; ++bits) {
         const int beg = pop_off[bits];
         const int end = pop_off[bits + 1];
+        const int terms = (1 << (bits - 1)) - 1;
         for (int it = beg + tid; it < end; it += nt) {
             const uint32_t mask = masks_by_pop[it];
-            double acc = a[mask];
-            if (mask & 1u) {
-                double rhs = fabs(a[mask]);
-                const uint32_t rest = mask ^ 1u;
-                for (uint32_t s = (rest - 1u) & rest;; s = (s - 1u) & rest) {
-                    const uint32_t sub = 1u | s;
-                    const double term = dp[sub] * a[mask ^ sub];
-                    acc = acc - term;
-                    rhs += fabs(term);
-                    if (s == 0u) break;
-                }
-                if (filter && fabs(acc) < thr * rhs) acc = 0.0;
-            }
-            dp[mask] = acc;
+            dp[mask] = (mask & 1u) ? connected_chain(a, dp, mask, 1u, mask ^ 1u, terms, thr, filter) : a[mask];
         }
         __syncthreads();
     }

@@ -256,7 +256,15 @@ __device__ inline double block_max(double v, double* red) {
     return r;
 }
 
-__device__ inline int fpm_block(const double* A, int n, double tol, double thresh, double* sm, double* bufA,
+// out[mask] = det(A[mask, mask]) for every mask (out[0] = 1).  With
+// brute = false this is the Schur-complement tree of A + pI and returns 1,
+// leaving out undefined, as soon as a pivot |A00 + p| falls to
+// tol * max|A| (the reference then switches to pivoted determinants,
+// include/cdet/fast_principle_minors.h:180-186); with brute = true the
+// pivoted determinants are formed directly.  Scratch: bufA/bufB hold
+// fpm_list_size(n) >= 2^n doubles, inv 2^(n-2), red blockDim.
+template <bool BRUTE>
+__device__ inline int fpm_block(const double* A, int n, double tol, double thresh, double* out, double* bufA,
                                 double* bufB, double* inv, double* red, int* flag) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
@@ -272,17 +280,22 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
     const bool explicit_shift = !(thresh < 0.0);
     const double p = explicit_shift ? thresh : max_abs / static_cast<double>(msize);
     if (max_abs == 0.0 && p == 0.0) {
-        for (uint32_t m = tid; m < sc; m += nt) sm[m] = (m == 0) ? 1.0 : 0.0;
+        for (uint32_t m = tid; m < sc; m += nt) out[m] = (m == 0) ? 1.0 : 0.0;
+        __syncthreads();
+        return 0;
+    }
+    if (BRUTE) {
+        for (uint32_t m = tid; m < sc; m += nt) out[m] = (m == 0) ? 1.0 : det_minor_brute(A, n, m);
         __syncthreads();
         return 0;
     }
     const double fb_eps = (tol > 0.0 && max_abs > 0.0) ? tol * max_abs : 0.0;
 
+    // shifted_minors: 1, p + A00, p, p, ... (:160-163)
     for (int i = tid; i < msize; i += nt) bufA[i] = A[i];
-    for (uint32_t m = tid; m < sc; m += nt) sm[m] = (m == 0) ? 1.0 : p;
+    for (uint32_t m = tid; m < sc; m += nt) out[m] = (m == 0) ? 1.0 : (m == 1 ? p + A[0] : p);
     if (tid == 0) *flag = 0;
     __syncthreads();
-    if (tid == 0) sm[1] += A[0];
 
     double* oldl = bufA;
     double* newl = bufB;
@@ -293,13 +306,16 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
         const int nms = ns * ns;
         const int sets = 1 << l;
         for (int s = tid; s < sets; s += nt) {
-            const double pd = oldl[s * oms] + p;
+            const double a00 = oldl[s * oms];
+            // minors of the matrices generated one level up (:210-213)
+            if (l >= 1) out[(1u << l) + s] += a00;
+            const double pd = a00 + p;
             const double pda = fabs(pd);
             if (pda == 0.0 || (fb_eps > 0.0 && pda <= fb_eps)) *flag = 1;
             inv[s] = 1.0 / pd;
         }
         __syncthreads();
-        if (*flag) break;
+        if (*flag) return 1;
         const int items = sets * nms;
         for (int it = tid; it < items; it += nt) {
             const int s = it / nms;
@@ -313,37 +329,48 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
             newl[(s + sets) * nms + rem] = __builtin_fma(-oldl[src + 1 + c], f, base);
         }
         __syncthreads();
-        const int gen = 2 * sets;
-        for (int m = tid; m < gen; m += nt) sm[(2u << l) + m] += newl[m * nms];
-        __syncthreads();
         double* t = oldl;
         oldl = newl;
         newl = t;
     }
-    if (*flag) {
-        for (uint32_t m = tid; m < sc; m += nt) sm[m] = (m == 0) ? 1.0 : det_minor_brute(A, n, m);
-        __syncthreads();
-        return 1;
+    if (n >= 2) {
+        const uint32_t top = 1u << (n - 1);
+        for (uint32_t t = tid; t < top; t += nt) out[top + t] += oldl[t];
     }
-    // Prefix products (:217-223).
-    for (int l = 0; l < n; ++l) {
-        const uint32_t step = 1u << l;
-        for (uint32_t s = step + 1 + tid; s < 2 * step; s += nt) sm[s] *= sm[s - step];
-        __syncthreads();
+    __syncthreads();
+    // Prefix products (:217-223): final[m] = r[m] * final[m - topbit(m)],
+    // formed per mask from the lowest bit upward into bufA.
+    for (uint32_t m = tid; m < sc; m += nt) {
+        if (m == 0) {
+            bufA[0] = 1.0;
+            continue;
+        }
+        uint32_t pre = m & (~m + 1u);
+        uint32_t rest = m ^ pre;
+        double val = out[pre];
+        while (rest) {
+            const uint32_t b = rest & (~rest + 1u);
+            rest ^= b;
+            pre |= b;
+            val = out[pre] * val;
+        }
+        bufA[m] = val;
     }
+    __syncthreads();
     // Removal of the uniform shift, highest bit first (:225-239).
     for (int l = n; l > 0; --l) {
         const uint32_t half = 1u << (l - 1);
-        // masks with bit (l-1) set: m = hi_part * 2^l + half + lo_part
         const uint32_t cnt = sc >> 1;
         for (uint32_t it = tid; it < cnt; it += nt) {
             const uint32_t lo = it & (half - 1);
             const uint32_t hi = it >> (l - 1);
             const uint32_t m = (hi << l) | half | lo;
-            sm[m] = __builtin_fma(-p, sm[m - half], sm[m]);
+            bufA[m] = __builtin_fma(-p, bufA[m - half], bufA[m]);
         }
         __syncthreads();
     }
+    for (uint32_t m = tid; m < sc; m += nt) out[m] = bufA[m];
+    __syncthreads();
     return 0;
 }
 

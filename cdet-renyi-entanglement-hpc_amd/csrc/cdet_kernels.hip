@@ -11,10 +11,12 @@
 //   k_scatter        queries grouped by coefficient block
 //   k_contract_*     persistent: each wave takes a tile of queries sharing one
 //                    coefficient block, so the block is read once per tile
-//   k_fpm_cdet<n>    order-bucketed: one workgroup per configuration, vertex
-//                    matrices, principal minors and subset recursion in LDS
+//   k_minors         order-bucketed: one workgroup per (configuration, table
+//                    class), vertex matrices, principal minors and subset
+//                    recursion in LDS; k_minors_repair finishes the rare
+//                    pivot fallbacks; k_combine_* forms twisted - 2*ordinary
 // The plain form (one configuration per launch) uses k_pairs_plain (one wave
-// per Green pair) and the same k_fpm_cdet.
+// per Green pair) and the same k_minors.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -266,12 +268,126 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t nq, const int32_t* __re
     s_dest[pos] = static_cast<int32_t>(q);
 }
 
+// Row Clenshaw of ROWS consecutive rows (a, a-1, ...) in lockstep: ROWS
+// independent dependency chains per lane, each the reference's recurrence
+// (include/cdet/g_funcs.h:450-461) for one row.
+template <int ROWS>
+__device__ inline void clenshaw_rows(const double* __restrict__ c, int a, int nv, double v, double two_v,
+                                     double* __restrict__ r) {
+    double b1[ROWS], b2[ROWS];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        b1[q] = 0.0;
+        b2[q] = 0.0;
+    }
+    for (int k = nv - 1; k >= 1; --k) {
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            const double bk = c[static_cast<int64_t>(a - q) * nv + k] + two_v * b1[q] - b2[q];
+            b2[q] = b1[q];
+            b1[q] = bk;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) r[q] = c[static_cast<int64_t>(a - q) * nv] + b1[q] * v - b2[q];
+}
+
+typedef double d8v __attribute__((ext_vector_type(8)));
+
+// clenshaw_rows for nv % 8 == 0: each row's coefficients are fetched eight at
+// a time (one 64-byte wave-uniform load, i.e. one scalar-cache line) and the
+// eight recurrence steps of the chunk run from registers.
+template <int ROWS>
+__device__ inline void clenshaw_rows8(const double* __restrict__ c, int a, int nv, double v, double two_v,
+                                      double* __restrict__ r) {
+    double b1[ROWS], b2[ROWS], c0[ROWS];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        b1[q] = 0.0;
+        b2[q] = 0.0;
+    }
+    for (int kc = nv - 8; kc >= 0; kc -= 8) {
+        d8v x[ROWS];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) x[q] = *reinterpret_cast<const d8v*>(c + static_cast<int64_t>(a - q) * nv + kc);
+        if (kc > 0) {
+#pragma unroll
+            for (int j = 7; j >= 0; --j) {
+#pragma unroll
+                for (int q = 0; q < ROWS; ++q) {
+                    const double bk = x[q][j] + two_v * b1[q] - b2[q];
+                    b2[q] = b1[q];
+                    b1[q] = bk;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 7; j >= 1; --j) {
+#pragma unroll
+                for (int q = 0; q < ROWS; ++q) {
+                    const double bk = x[q][j] + two_v * b1[q] - b2[q];
+                    b2[q] = b1[q];
+                    b1[q] = bk;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) c0[q] = x[q][0];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) r[q] = c0[q] + b1[q] * v - b2[q];
+}
+
+// G for one query from one coefficient block, rows produced four at a time
+// from a = nu-1 downwards and consumed in that order by the Clenshaw over u;
+// bit-identical to chebyshev_sum_2d_pair_from_ptrs (include/cdet/g_funcs.h:687-763).
+__device__ inline double cheb2d_exact(const double* __restrict__ c, int nu, int nv, double u, double v) {
+    UClenshaw fu;
+    fu.init(u, nu);
+    double g = 0.0;
+    const int dv = nv - 1;
+    if (dv < 2) {
+        for (int a = nu - 1; a >= 0; --a) g = fu.feed(a, clenshaw_row(c + static_cast<int64_t>(a) * nv, dv, v));
+        return g;
+    }
+    const double two_v = 2.0 * v;
+    int a = nu - 1;
+    if ((nv & 7) == 0) {
+        for (; a >= 3; a -= 4) {
+            double r[4];
+            clenshaw_rows8<4>(c, a, nv, v, two_v, r);
+            g = fu.feed(a, r[0]);
+            g = fu.feed(a - 1, r[1]);
+            g = fu.feed(a - 2, r[2]);
+            g = fu.feed(a - 3, r[3]);
+        }
+        for (; a >= 0; --a) {
+            double r[1];
+            clenshaw_rows8<1>(c, a, nv, v, two_v, r);
+            g = fu.feed(a, r[0]);
+        }
+        return g;
+    }
+    for (; a >= 3; a -= 4) {
+        double r[4];
+        clenshaw_rows<4>(c, a, nv, v, two_v, r);
+        g = fu.feed(a, r[0]);
+        g = fu.feed(a - 1, r[1]);
+        g = fu.feed(a - 2, r[2]);
+        g = fu.feed(a - 3, r[3]);
+    }
+    for (; a >= 0; --a) {
+        double r[1];
+        clenshaw_rows<1>(c, a, nv, v, two_v, r);
+        g = fu.feed(a, r[0]);
+    }
+    return g;
+}
+
 // Exact batched contraction: one wave per tile of <= 64 queries sharing a
 // coefficient block; lane = query.  The block address is wave-uniform, so the
 // coefficients stream through the scalar cache and are broadcast to all
-// lanes.  Rows are produced from a = nu-1 down to 0 and consumed at once by
-// the Clenshaw over u, so no row buffer is needed.  Bit-identical to the
-// reference's chebyshev_sum_2d_pair_from_ptrs (include/cdet/g_funcs.h:687-763).
+// lanes.
 __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32_t* __restrict__ n_tiles,
                                                         const int32_t* __restrict__ tile_key,
                                                         const int32_t* __restrict__ tile_start,
@@ -296,86 +412,53 @@ __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32
         const int pos = start + (active ? lane : 0);
         const double u = s_u[pos];
         const double v = s_v[pos];
-        const double two_v = 2.0 * v;
-        UClenshaw fu, fd;
-        fu.init(u, ks.nu);
-        fd.init(u, ks.nu);
-        double gu = 0.0, gd = 0.0;
-        const int dv = ks.nv - 1;
-        for (int a = ks.nu - 1; a >= 0; --a) {
-            const double* ru = cu + static_cast<int64_t>(a) * ks.nv;
-            double r_u;
-            if (dv >= 2) {
-                double b1 = 0.0, b2 = 0.0;
-                for (int k = dv; k >= 1; --k) {
-                    const double bk = ru[k] + two_v * b1 - b2;
-                    b2 = b1;
-                    b1 = bk;
-                }
-                r_u = ru[0] + b1 * v - b2;
-            } else {
-                r_u = clenshaw_row(ru, dv, v);
-            }
-            gu = fu.feed(a, r_u);
-            if (!same) {
-                const double* rd = cd + static_cast<int64_t>(a) * ks.nv;
-                double r_d;
-                if (dv >= 2) {
-                    double b1 = 0.0, b2 = 0.0;
-                    for (int k = dv; k >= 1; --k) {
-                        const double bk = rd[k] + two_v * b1 - b2;
-                        b2 = b1;
-                        b1 = bk;
-                    }
-                    r_d = rd[0] + b1 * v - b2;
-                } else {
-                    r_d = clenshaw_row(rd, dv, v);
-                }
-                gd = fd.feed(a, r_d);
-            }
-        }
+        const double gu = cheb2d_exact(cu, ks.nu, ks.nv, u, v);
+        const double gd = same ? gu : cheb2d_exact(cd, ks.nu, ks.nv, u, v);
         if (active) {
             const int32_t dest = s_dest[pos];
             g_up[dest] = gu;
-            g_dn[dest] = same ? gu : gd;
+            g_dn[dest] = gd;
         }
     }
 }
 
-// Order-bucketed vertex-matrix -> minors -> connected values -> assembly.
-// One workgroup per configuration of order NORD; LDS holds everything.
-//   mode 0: all-subset values into out[cfg_obase[c] .. + 2^n)
-//   mode 1: single-order value into out[cfg_obase[c]]
-// Configurations are taken from cfg_list (indices into the plan arrays).
-struct FpmLds {
-    static __host__ __device__ inline size_t bytes(int n) {
+// Order-bucketed vertex matrices -> principal minors -> connected values ->
+// assembly.  One workgroup per (configuration, table class): blockIdx.x =
+// 2 * list index + class, class 0 = twisted (rc = 2), 1 = ordinary (rc = 1).
+//   mode 0 (all subsets): class values into out (twisted) / out_ord
+//          (ordinary) at [obase, obase + 2^n); k_combine_all forms
+//          twisted - 2 * ordinary.
+//   mode 1 (single order): class value as a double-double pair into
+//          single_tmp[4 c + 2 class + {0, 1}]; k_combine_single forms the
+//          double-double difference.
+// A spin matrix whose Schur pivot needs the reference's brute-force fallback
+// is queued in fb_list (entry = (list index << 3) | (class << 2) | spin mask)
+// and finished by k_minors_repair, so this kernel carries no per-lane
+// scratch.
+struct MinorsLds {
+    static __host__ __device__ inline size_t doubles(int n) {
         const size_t sc = size_t(1) << n;
         const size_t list = static_cast<size_t>(fpm_list_size(n));
         const size_t inv = n >= 2 ? (size_t(1) << (n - 2)) : 1;
-        return sizeof(double) * (2 * n * n + 2 * n + 2 * sc + 2 * list + inv + 256) + 64;
+        return 2 * n * n + 2 * n + 2 * sc + 2 * list + inv + 256;
     }
+    static __host__ __device__ inline size_t bytes(int n) { return sizeof(double) * doubles(n) + 64; }
 };
 
-__global__ __launch_bounds__(256) void k_fpm_cdet(KScalars ks, int n, int mode, const int32_t* __restrict__ cfg_list,
-                                                  const int64_t* __restrict__ cfg_pbase,
-                                                  const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up,
-                                                  const double* __restrict__ g_dn, double* __restrict__ out,
-                                                  int32_t* __restrict__ status, uint64_t* __restrict__ fallback_count,
-                                                  const uint16_t* __restrict__ masks_by_pop,
-                                                  const int* __restrict__ pop_off_all, double* __restrict__ gscratch,
-                                                  size_t gscratch_stride) {
-    extern __shared__ double lds_dyn[];
-    __shared__ int s_flag;
-    __shared__ int s_status;
-    __shared__ double s_single[2][2];
-    const int c = cfg_list[blockIdx.x];
+template <bool REPAIR>
+__device__ inline int minors_connected(const KScalars& ks, int n, int mode, int c, int cls, int spin_brute,
+                                       const int64_t* __restrict__ cfg_pbase, const int64_t* __restrict__ cfg_obase,
+                                       const double* __restrict__ g_up, const double* __restrict__ g_dn,
+                                       double* __restrict__ out, double* __restrict__ out_ord,
+                                       double* __restrict__ single_tmp, int32_t* __restrict__ status,
+                                       const uint16_t* __restrict__ masks_by_pop, const int* __restrict__ pop_off,
+                                       double* base, int* s_flag) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
     const int n2 = n * n;
     const int list = fpm_list_size(n);
     const int ninv = n >= 2 ? (1 << (n - 2)) : 1;
-    double* base = gscratch ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
     double* mU = base;
     double* mD = mU + n2;
     double* dU = mD + n2;
@@ -386,105 +469,163 @@ __global__ __launch_bounds__(256) void k_fpm_cdet(KScalars ks, int n, int mode, 
     double* bufB = bufA + list;
     double* inv = bufB + list;
     double* red = inv + ninv;
-    const int* pop_off = pop_off_all + n * (kMaxOrder + 2);
-    const int64_t pb = 2 * cfg_pbase[c];
+    const bool same = ks.same_spin[cls] != 0;
+    const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
     const int64_t ob = cfg_obase[c];
-    if (tid == 0) s_status = 0;
+    // Vertex matrices (include/cdet/integrand_replica_randomly_sampled.h:751-761).
+    for (int idx = tid; idx < n2; idx += nt) {
+        const int i = idx / n;
+        const int j = idx - i * n;
+        double vu = g_up[pb + idx];
+        double vd = g_dn[pb + idx];
+        if (i == j) {
+            dU[i] = vu + ks.dup;
+            dD[i] = vd + ks.ddn;
+            vu += ks.dup;
+            vd += ks.ddn;
+        }
+        mU[idx] = vu * ks.scale;
+        mD[idx] = vd * ks.scale;
+    }
     __syncthreads();
-
-    for (int cls = 0; cls < 2; ++cls) {
-        const bool same = ks.same_spin[cls] != 0;
-        const double* gu = g_up + pb + cls * n2;
-        const double* gd = g_dn + pb + cls * n2;
-        // Vertex matrices (include/cdet/integrand_replica_randomly_sampled.h:751-761).
-        for (int idx = tid; idx < n2; idx += nt) {
-            const int i = idx / n;
-            const int j = idx - i * n;
-            double vu = gu[idx];
-            double vd = gd[idx];
-            if (i == j) {
-                dU[i] = vu + ks.dup;
-                dD[i] = vd + ks.ddn;
-                vu += ks.dup;
-                vd += ks.ddn;
-            }
-            mU[idx] = vu * ks.scale;
-            mD[idx] = vd * ks.scale;
+    if (n == 1) {  // order-1 single value (:173-213), no minors
+        if (tid == 0) {
+            const double v = ks.c1[cls] * (dU[0] * dD[0] - ks.counter);
+            single_tmp[4 * c + 2 * cls] = v;
+            single_tmp[4 * c + 2 * cls + 1] = 0.0;
         }
-        __syncthreads();
-        if (n == 1) {
-            // Order-1 single value (include/cdet/integrand_replica_randomly_sampled.h:173-213).
-            if (tid == 0) {
-                const double v = ks.c1[cls] * (dU[0] * dD[0] - ks.counter);
-                s_single[cls][0] = v;
-                s_single[cls][1] = 0.0;
-            }
-            __syncthreads();
-            continue;
-        }
-        int fb = fpm_block(mU, n, ks.tol, ks.shift, smU, bufA, bufB, inv, red, &s_flag);
-        if (same) {
+        return 0;
+    }
+    int need = 0;
+    if (REPAIR && (spin_brute & 1))
+        fpm_block<true>(mU, n, ks.tol, ks.shift, smU, bufA, bufB, inv, red, s_flag);
+    else if (fpm_block<false>(mU, n, ks.tol, ks.shift, smU, bufA, bufB, inv, red, s_flag))
+        need |= 1;
+    if (same) {
+        if (need) need |= 2;
+        else
             for (uint32_t m = tid; m < sc; m += nt) smD[m] = smU[m];
-            __syncthreads();
-            fb *= 2;
-        } else {
-            fb += fpm_block(mD, n, ks.tol, ks.shift, smD, bufA, bufB, inv, red, &s_flag);
-        }
-        if (fb && tid == 0) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(fallback_count), static_cast<unsigned long long>(fb));
-            s_status |= 2;
-        }
-        // a[mask] = pm_up * pm_down with a[0] = 0 (stored over smU).
-        for (uint32_t m = tid; m < sc; m += nt) smU[m] = (m == 0) ? 0.0 : smU[m] * smD[m];
-        __syncthreads();
-        if (mode == 0) {
-            ztransform_all_block(smU, smD, n, ks.thr, ks.filter != 0, masks_by_pop, pop_off);
-            // Assembly (:833-877) and twisted - 2*ordinary
-            // (include/cdet/cdet_integrand_evaluator.h:348-350).  Thread t
-            // owns the same masks in both classes.
-            const int rc = cls == 0 ? 2 : 1;
-            (void)rc;
-            bool bad = false;
-            for (uint32_t m = tid; m < sc; m += nt) {
-                double v;
-                if (m == 0) {
-                    v = 0.0;
-                } else if ((m & (m - 1)) == 0) {
-                    const int i = __builtin_ctz(m);
-                    v = ks.c1[cls] * (dU[i] * dD[i] - ks.counter);
-                } else {
-                    v = ks.pref[cls][__builtin_popcount(m)] * smD[m];
-                }
-                if (!isfinite(v)) bad = true;
-                if (cls == 0)
-                    out[ob + m] = v;
-                else
-                    out[ob + m] = out[ob + m] - 2.0 * v;
-            }
-            if (bad) atomicOr(&s_status, 1);
-            __syncthreads();
-        } else {
-            const double cres = ztransform_single_block(smU, smD, n, ks.thr, ks.filter != 0, masks_by_pop, pop_off);
-            if (tid == 0) {
-                const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
-                s_single[cls][0] = v.hi;
-                s_single[cls][1] = v.lo;
-            }
-            __syncthreads();
-        }
+    } else if (REPAIR && (spin_brute & 2)) {
+        fpm_block<true>(mD, n, ks.tol, ks.shift, smD, bufA, bufB, inv, red, s_flag);
+    } else if (fpm_block<false>(mD, n, ks.tol, ks.shift, smD, bufA, bufB, inv, red, s_flag)) {
+        need |= 2;
     }
-    if (mode == 1 && tid == 0) {
-        // twisted - 2*ordinary in double-double, then rounded
-        // (include/cdet/cdet_integrand_evaluator.h:262, :559).
-        const DD tw{s_single[0][0], s_single[0][1]};
-        const DD od{s_single[1][0], s_single[1][1]};
-        const DD r = dd_sub(tw, dd_mul(DD{2.0, 0.0}, od));
-        const double v = r.hi + r.lo;
-        if (!isfinite(tw.hi + tw.lo) || !isfinite(od.hi + od.lo) || !isfinite(v)) s_status |= 1;
-        out[ob] = v;
-    }
+    if (!REPAIR && need) return need;  // uniform across the workgroup
+    if (REPAIR && same && (spin_brute & 1))
+        for (uint32_t m = tid; m < sc; m += nt) smD[m] = smU[m];
     __syncthreads();
-    if (tid == 0 && status) status[c] = s_status;
+    // a[mask] = pm_up * pm_down with a[0] = 0 (stored over smU).
+    for (uint32_t m = tid; m < sc; m += nt) smU[m] = (m == 0) ? 0.0 : smU[m] * smD[m];
+    __syncthreads();
+    if (mode == 0 || mode == 2) {
+        // mode 2 writes the magnitude scale of each value instead of the value:
+        // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
+        double* rhs = mode == 2 ? bufA : nullptr;
+        ztransform_all_block(smU, smD, n, ks.thr, ks.filter != 0, masks_by_pop, pop_off, rhs);
+        // Assembly (:833-877).
+        double* dst = (cls == 0 ? out : out_ord) + ob;
+        bool bad = false;
+        for (uint32_t m = tid; m < sc; m += nt) {
+            double v;
+            if (m == 0) {
+                v = 0.0;
+            } else if ((m & (m - 1)) == 0) {
+                const int i = __builtin_ctz(m);
+                v = ks.c1[cls] * (dU[i] * dD[i] - ks.counter);
+                if (mode == 2) v = fabs(v);
+            } else {
+                v = mode == 2 ? fabs(ks.pref[cls][__builtin_popcount(m)]) * rhs[m]
+                              : ks.pref[cls][__builtin_popcount(m)] * smD[m];
+            }
+            if (!isfinite(v)) bad = true;
+            dst[m] = v;
+        }
+        if (bad && status) atomicOr(&status[c], 1);
+    } else {
+        const double cres = ztransform_single_block(smU, smD, n, ks.thr, ks.filter != 0, masks_by_pop, pop_off);
+        if (tid == 0) {
+            const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
+            single_tmp[4 * c + 2 * cls] = v.hi;
+            single_tmp[4 * c + 2 * cls + 1] = v.lo;
+            if (!isfinite(v.hi + v.lo) && status) atomicOr(&status[c], 1);
+        }
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(256) void k_minors(KScalars ks, int n, int mode, const int32_t* __restrict__ cfg_list,
+                                                const int64_t* __restrict__ cfg_pbase,
+                                                const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up,
+                                                const double* __restrict__ g_dn, double* __restrict__ out,
+                                                double* __restrict__ out_ord, double* __restrict__ single_tmp,
+                                                int32_t* __restrict__ status, int32_t* __restrict__ fb_list,
+                                                int32_t* __restrict__ fb_count,
+                                                const uint16_t* __restrict__ masks_by_pop,
+                                                const int* __restrict__ pop_off_all, double* __restrict__ gscratch,
+                                                size_t gscratch_stride) {
+    extern __shared__ double lds_dyn[];
+    __shared__ int s_flag;
+    const int li = blockIdx.x >> 1;
+    const int cls = blockIdx.x & 1;
+    const int c = cfg_list[li];
+    double* base = gscratch ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
+    const int need = minors_connected<false>(ks, n, mode, c, cls, 0, cfg_pbase, cfg_obase, g_up, g_dn, out, out_ord,
+                                             single_tmp, status, masks_by_pop, pop_off_all + n * (kMaxOrder + 2),
+                                             base, &s_flag);
+    if (need && threadIdx.x == 0) {
+        const int slot = atomicAdd(fb_count, 1);
+        fb_list[slot] = (li << 3) | (cls << 2) | need;
+    }
+}
+
+// Finishes the queued fallback spins with pivoted determinants
+// (include/cdet/fast_principle_minors.h:39-115, :180-186).  Fixed grid,
+// loops over the device-side queue.
+__global__ __launch_bounds__(256) void k_minors_repair(
+    KScalars ks, int n, int mode, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
+    const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up, const double* __restrict__ g_dn,
+    double* __restrict__ out, double* __restrict__ out_ord, double* __restrict__ single_tmp,
+    int32_t* __restrict__ status, const int32_t* __restrict__ fb_list, const int32_t* __restrict__ fb_count,
+    uint64_t* __restrict__ fallback_total, const uint16_t* __restrict__ masks_by_pop,
+    const int* __restrict__ pop_off_all, double* __restrict__ gscratch, size_t gscratch_stride) {
+    __shared__ int s_flag;
+    const int total = *fb_count;
+    double* base = gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride;
+    for (int e = blockIdx.x; e < total; e += gridDim.x) {
+        const int entry = fb_list[e];
+        const int li = entry >> 3;
+        const int cls = (entry >> 2) & 1;
+        const int spins = entry & 3;
+        const int c = cfg_list[li];
+        minors_connected<true>(ks, n, mode, c, cls, spins, cfg_pbase, cfg_obase, g_up, g_dn, out, out_ord,
+                               single_tmp, status, masks_by_pop, pop_off_all + n * (kMaxOrder + 2), base, &s_flag);
+        if (threadIdx.x == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(fallback_total),
+                      static_cast<unsigned long long>(__builtin_popcount(spins)));
+            if (status) atomicOr(&status[c], 2);
+        }
+        __syncthreads();
+    }
+}
+
+// values[mask] -= 2 * ordinary_values[mask]
+// (include/cdet/cdet_integrand_evaluator.h:348-350); with add set, the
+// magnitude scales of the two classes add instead.
+__global__ void k_combine_all(int64_t n, double* __restrict__ out, const double* __restrict__ out_ord, int add) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = add ? out[i] + 2.0 * out_ord[i] : out[i] - 2.0 * out_ord[i];
+}
+
+// twisted - DoubleDouble(2) * ordinary, rounded
+// (include/cdet/cdet_integrand_evaluator.h:262, :559).
+__global__ void k_combine_single(int64_t count, const int64_t* __restrict__ cfg_obase,
+                                 const double* __restrict__ single_tmp, double* __restrict__ out) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    const DD tw{single_tmp[4 * c], single_tmp[4 * c + 1]};
+    const DD od{single_tmp[4 * c + 2], single_tmp[4 * c + 3]};
+    const DD r = dd_sub(tw, dd_mul(DD{2.0, 0.0}, od));
+    out[cfg_obase[c]] = r.hi + r.lo;
 }
 
 // FP32 hi/lo split of a coefficient array: c = hi + lo + O(2^-48 |c|).
@@ -508,6 +649,12 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
 // ---------------------------------------------------------------------------
 struct cdet_batch_plan;
 
+// Device buffer that only grows (see minors_scratch).
+struct Scratch {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
 struct cdet_ctx {
     cdet_params p{};
     int device = 0;
@@ -524,10 +671,12 @@ struct cdet_ctx {
     double* d_gdn = nullptr;
     double* d_out = nullptr;
     int32_t* d_status = nullptr;
+    double* d_out_ord = nullptr;
+    double* d_single = nullptr;
+    int32_t* d_small_i32 = nullptr;
     int32_t* d_one = nullptr;   // cfg list {0}
     int64_t* d_zero64 = nullptr;  // pbase/obase {0}
-    double* d_gscratch = nullptr;
-    size_t gscratch_bytes = 0;
+    Scratch plain_scratch;
     // host-buffer batch path (driver callback)
     int batch_mode = CDET_MODE_EXACT;
     cdet_batch_plan* host_plan = nullptr;
@@ -672,37 +821,82 @@ int grow(void** p, size_t& cap, size_t need) {
 
 }  // namespace
 
-// Sizing helper: LDS bytes of k_fpm_cdet for order n.
-static size_t fpm_lds_bytes(int n) { return FpmLds::bytes(n); }
+// Sizing helper: LDS bytes of k_minors for order n.
+static size_t fpm_lds_bytes(int n) { return MinorsLds::bytes(n); }
 static constexpr size_t kMaxLds = 160 * 1024 - 2048;
+static constexpr int kRepairGrid = 64;
 
-// Launch the per-configuration kernel for one order bucket.
-static int launch_fpm(cdet_ctx* ctx, int n, int mode, const int32_t* cfg_list, int count, const int64_t* pbase,
-                      const int64_t* obase, const double* gup, const double* gdn, double* out, int32_t* status,
-                      hipStream_t stream) {
-    if (count <= 0) return 0;
+// Outputs and queues of one minors launch.
+struct MinorsOut {
+    double* out;
+    double* out_ord;
+    double* single_tmp;
+    int32_t* status;
+    int32_t* fb_list;
+    int32_t* fb_count;
+};
+
+// Global scratch of one minors launch: the repair grid's blocks first, then
+// (orders whose minors exceed LDS) one block per workgroup.  The buffer is
+// owned by the caller (a plan, or the context for the plain form) and only
+// grows; a plan sizes it at creation so that its captured graph never sees a
+// reallocation.
+static size_t minors_scratch_bytes(int n, int blocks) {
     const size_t lds = fpm_lds_bytes(n);
+    const size_t main_need = lds > kMaxLds ? lds * static_cast<size_t>(blocks) : 0;
+    return main_need + lds * kRepairGrid;
+}
+
+static int minors_scratch(Scratch& sc, int n, int blocks, double** repair_gs, double** main_gs, size_t* stride) {
+    *stride = MinorsLds::doubles(n);
+    int rc = grow(&sc.ptr, sc.cap, minors_scratch_bytes(n, blocks));
+    if (rc) return rc;
+    *repair_gs = static_cast<double*>(sc.ptr);
+    *main_gs = fpm_lds_bytes(n) > kMaxLds ? *repair_gs + static_cast<size_t>(kRepairGrid) * *stride : nullptr;
+    return 0;
+}
+
+// Launch the minors kernel for one order bucket and its repair pass.
+static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const int32_t* cfg_list, int count,
+                         const int64_t* pbase, const int64_t* obase, const double* gup, const double* gdn,
+                         const MinorsOut& o, hipStream_t stream) {
+    if (count <= 0) return 0;
     double* gs = nullptr;
+    double* rs = nullptr;
     size_t stride = 0;
-    size_t dyn = lds;
-    if (lds > kMaxLds) {
-        stride = lds / sizeof(double);
-        const size_t need = stride * sizeof(double) * static_cast<size_t>(count);
-        void* p = ctx->d_gscratch;
-        int rc = grow(&p, ctx->gscratch_bytes, need);
-        ctx->d_gscratch = static_cast<double*>(p);
-        if (rc) return rc;
-        gs = ctx->d_gscratch;
-        dyn = 0;
-    }
-    hipLaunchKernelGGL(k_fpm_cdet, dim3(count), dim3(256), dyn, stream, ctx->ks, n, mode, cfg_list, pbase, obase, gup,
-                       gdn, out, status, ctx->d_fallbacks, ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder),
-                       ctx->d_popoff, gs, stride);
+    int rc = minors_scratch(scratch, n, 2 * count, &rs, &gs, &stride);
+    if (rc) return rc;
+    const size_t dyn = gs ? 0 : fpm_lds_bytes(n);
+    const uint16_t* masks = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
+    hipLaunchKernelGGL(k_minors, dim3(2 * count), dim3(256), dyn, stream, ctx->ks, n, mode, cfg_list, pbase, obase,
+                       gup, gdn, o.out, o.out_ord, o.single_tmp, o.status, o.fb_list, o.fb_count, masks, ctx->d_popoff,
+                       gs, stride);
+    hipLaunchKernelGGL(k_minors_repair, dim3(kRepairGrid), dim3(256), 0, stream, ctx->ks, n, mode, cfg_list, pbase,
+                       obase, gup, gdn, o.out, o.out_ord, o.single_tmp, o.status, o.fb_list, o.fb_count,
+                       ctx->d_fallbacks, masks, ctx->d_popoff, rs, stride);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-static int mfma_tile_queries(const KScalars&) { return kMfmaTile; }
+static int launch_combine(int mode, int64_t count, int64_t out_total, const int64_t* obase, const MinorsOut& o,
+                          hipStream_t stream) {
+    if (mode == 0 || mode == 2) {
+        hipLaunchKernelGGL(k_combine_all, dim3(static_cast<unsigned>((out_total + 255) / 256)), dim3(256), 0, stream,
+                           out_total, o.out, o.out_ord, mode == 2 ? 1 : 0);
+    } else {
+        hipLaunchKernelGGL(k_combine_single, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, stream,
+                           count, obase, o.single_tmp, o.out);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// The 64-query FP64 kernel needs both coefficient counts to be multiples of
+// 16; other shapes use the 32-query kernel.
+static bool use_f64_v2(const KScalars& ks) { return ks.nu % 16 == 0 && ks.nv % 16 == 0; }
+static int mfma_tile_queries(const KScalars& ks, int mode) {
+    return (mode == CDET_MODE_MFMA64 && use_f64_v2(ks)) ? kF64Tile : kMfmaTile;
+}
 
 // FP32 hi/lo copies of every resident table (split-FP32 mode), built once.
 static int ensure_split_tables(cdet_ctx* ctx) {
@@ -731,7 +925,14 @@ static int ensure_split_tables(cdet_ctx* ctx) {
 static int contract_launch_mfma(const KScalars& ks, int mode, int grid, const int32_t* ntiles, const int32_t* tkey,
                                 const int32_t* tstart, const int32_t* tlen, const double* su, const double* sv,
                                 const int32_t* sdest, double* gup, double* gdn, hipStream_t s) {
-    if (mode == CDET_MODE_MFMA64) {
+    if (mode == CDET_MODE_MFMA64 && use_f64_v2(ks)) {
+        if (ks.same_spin[0] && ks.same_spin[1])
+            hipLaunchKernelGGL(k_contract_f64<true>, dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen, su,
+                               sv, sdest, gup, gdn);
+        else
+            hipLaunchKernelGGL(k_contract_f64<false>, dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen, su,
+                               sv, sdest, gup, gdn);
+    } else if (mode == CDET_MODE_MFMA64) {
         hipLaunchKernelGGL(k_contract_mfma<CDET_MODE_MFMA64>, dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart,
                            tlen, su, sv, sdest, gup, gdn);
     } else {
@@ -809,12 +1010,15 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
     HIP_TRY(hipMalloc(&ctx->d_gdn, sizeof(double) * 2 * Mx * Mx));
     HIP_TRY(hipMalloc(&ctx->d_out, sizeof(double) << Mx));
     HIP_TRY(hipMalloc(&ctx->d_status, sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&ctx->d_out_ord, sizeof(double) << Mx));
+    HIP_TRY(hipMalloc(&ctx->d_single, sizeof(double) * 4));
+    HIP_TRY(hipMalloc(&ctx->d_small_i32, sizeof(int32_t) * 4));
     HIP_TRY(hipMalloc(&ctx->d_one, sizeof(int32_t)));
     HIP_TRY(hipMalloc(&ctx->d_zero64, sizeof(int64_t)));
     HIP_TRY(hipMemset(ctx->d_one, 0, sizeof(int32_t)));
     HIP_TRY(hipMemset(ctx->d_zero64, 0, sizeof(int64_t)));
     // Allow the order-bucketed kernel the full LDS of a CU.
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm_cdet),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_minors),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     refresh_scalars(ctx);
     *out_ctx = ctx;
@@ -842,9 +1046,12 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_gdn);
     (void)hipFree(ctx->d_out);
     (void)hipFree(ctx->d_status);
+    (void)hipFree(ctx->d_out_ord);
+    (void)hipFree(ctx->d_single);
+    (void)hipFree(ctx->d_small_i32);
     (void)hipFree(ctx->d_one);
     (void)hipFree(ctx->d_zero64);
-    (void)hipFree(ctx->d_gscratch);
+    (void)hipFree(ctx->plain_scratch.ptr);
     if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
     (void)hipFree(ctx->d_hv);
     (void)hipFree(ctx->d_hout);
@@ -941,18 +1148,24 @@ static int plain_eval(cdet_ctx* ctx, const cdet_vertex* v, int n, int mode, doub
     if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(ctx->d_verts, v, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->d_small_i32, static_cast<int64_t>(4), 0);
     hipLaunchKernelGGL(k_pairs_plain, dim3(2 * n * n), dim3(64), 0, s, ctx->ks, ctx->d_verts, n, ctx->d_gup,
                        ctx->d_gdn);
     HIP_TRY(hipGetLastError());
-    int rc = launch_fpm(ctx, n, mode, ctx->d_one, 1, ctx->d_zero64, ctx->d_zero64, ctx->d_gup, ctx->d_gdn, ctx->d_out,
-                        ctx->d_status, s);
+    // d_small_i32: [0] status, [1] queue length, [2..3] queue
+    MinorsOut o{ctx->d_out, ctx->d_out_ord, ctx->d_single, ctx->d_small_i32, ctx->d_small_i32 + 2,
+                ctx->d_small_i32 + 1};
+    int rc = launch_minors(ctx, ctx->plain_scratch, n, mode, ctx->d_one, 1, ctx->d_zero64, ctx->d_zero64, ctx->d_gup,
+                           ctx->d_gdn, o, s);
+    if (rc) return rc;
+    rc = launch_combine(mode, 1, static_cast<int64_t>(out_count), ctx->d_zero64, o, s);
     if (rc) return rc;
     int32_t st = 0;
     HIP_TRY(hipMemcpyAsync(out, ctx->d_out, sizeof(double) * out_count, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&st, ctx->d_status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&st, ctx->d_small_i32, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     ctx->calls += 1;
-    if (st & 1) {
+    if ((st & 1) && n > 1) {
         if (mode == 1) return fail(CDET_ERR_NONFINITE, "Error (integrand): Value is not finite");
         return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
     }

@@ -43,19 +43,22 @@ __device__ inline void cheb_0_16(double x, double* t) {
     for (int m = 2; m <= 16; ++m) t[m] = tx * t[m - 1] - t[m - 2];
 }
 
-// Select t[4g + r] and t[|4g + r - 16|] (g in 0..3) without dynamic indexing.
+// Select a_g (g in 0..3) with two bit tests, branch-free.
 __device__ inline double sel4(int g, double a0, double a1, double a2, double a3) {
-    return g == 0 ? a0 : (g == 1 ? a1 : (g == 2 ? a2 : a3));
+    const double lo = (g & 1) ? a1 : a0;
+    const double hi = (g & 1) ? a3 : a2;
+    return (g & 2) ? hi : lo;
 }
 
 struct BasisK {  // lane's T_k(v) along K with k = 4s + g
     double cur, prev, w;
     __device__ inline void init(double v, int g) {
-        double t[17];
-        cheb_0_16(v, t);
-        cur = sel4(g, t[0], t[1], t[2], t[3]);
-        prev = sel4(g, t[4], t[3], t[2], t[1]);  // T_{4-g} = T_{g-4}
-        w = 2.0 * t[4];
+        const double t2 = 2.0 * v * v - 1.0;
+        const double t3 = 2.0 * v * t2 - v;
+        const double t4 = 2.0 * v * t3 - t2;
+        cur = sel4(g, 1.0, v, t2, t3);
+        prev = sel4(g, t4, t3, t2, v);  // T_{4-g} = T_{g-4}
+        w = 2.0 * t4;
     }
     __device__ inline void step() {
         const double nx = w * cur - prev;
@@ -246,6 +249,147 @@ __global__ __launch_bounds__(256) void k_contract_mfma(KScalars ks, const int32_
                 g_dn[d1] = same ? G1u : G1d;
             }
         }
+    }
+}
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+constexpr int kF64Tile = 64;  // queries per wave in k_contract_f64
+
+// One tile of k_contract_f64 with NT live 16-wide N tiles (16 (NT-1) < len <=
+// 16 NT).  K is visited in groups of 16; MFMA step r of group t uses
+// k = 16 t + 4 r + g on both operands, so each lane's K basis is one stride-4
+// Chebyshev sequence (BasisK) and its four coefficient loads of a group read
+// C[row][16 t + 4 r + g], r = 0..3 (32 contiguous bytes per row and step).
+// Accumulator register r holds row 16 m + 4 r + g, again a stride-4 sequence
+// (BasisK over u).  The next group's coefficients are loaded before the
+// current group's MFMAs; every N tile reuses each A fragment.
+template <bool SAME, int NT>
+__device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const double* __restrict__ cu,
+                                         const double* __restrict__ cd, int32_t start, int32_t len,
+                                         const double* __restrict__ s_u, const double* __restrict__ s_v,
+                                         const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
+                                         double* __restrict__ g_dn) {
+    const int nu = ks.nu, nv = ks.nv;
+    const int mslabs = nu >> 4;
+    const int groups = nv >> 4;
+    BasisK Au[NT], B0[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int q = 16 * nt + j;
+        const int p = start + (q < len ? q : 0);
+        B0[nt].init(s_v[p], g);
+        Au[nt].init(s_u[p], g);
+    }
+    double G[NT], Gd[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) G[nt] = Gd[nt] = 0.0;
+    for (int ms = 0; ms < mslabs; ++ms) {
+        const double* pu = cu + static_cast<int64_t>(16 * ms + j) * nv + g;
+        const double* pd = cd + static_cast<int64_t>(16 * ms + j) * nv + g;
+        d4 acc[NT], accd[NT];
+        BasisK Bv[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            acc[nt] = d4{0.0, 0.0, 0.0, 0.0};
+            accd[nt] = acc[nt];
+            Bv[nt] = B0[nt];
+        }
+        double a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            a[r] = pu[4 * r];
+            if (!SAME) b[r] = pd[4 * r];
+        }
+        for (int t = 0; t < groups; ++t) {
+            const int tn = 16 * ((t + 1 < groups) ? t + 1 : t);
+            double an[4], bn[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                an[r] = pu[tn + 4 * r];
+                if (!SAME) bn[r] = pd[tn + 4 * r];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], Bv[nt].cur, acc[nt], 0, 0, 0);
+                    if (!SAME) accd[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[r], Bv[nt].cur, accd[nt], 0, 0, 0);
+                    Bv[nt].step();
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                a[r] = an[r];
+                if (!SAME) b[r] = bn[r];
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                G[nt] = __builtin_fma(Au[nt].cur, acc[nt][r], G[nt]);
+                if (!SAME) Gd[nt] = __builtin_fma(Au[nt].cur, accd[nt][r], Gd[nt]);
+                Au[nt].step();
+            }
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        double x = G[nt];
+        x += __shfl_xor(x, 16);
+        x += __shfl_xor(x, 32);
+        double y = x;
+        if (!SAME) {
+            y = Gd[nt];
+            y += __shfl_xor(y, 16);
+            y += __shfl_xor(y, 32);
+        }
+        const int q = 16 * nt + j;
+        if (g == 0 && q < len) {
+            const int32_t d = s_dest[start + q];
+            g_up[d] = x;
+            g_dn[d] = y;
+        }
+    }
+}
+
+// FP64 matrix-core contraction for nu % 16 == 0 and nv % 16 == 0: one wave
+// per tile of up to 64 queries of one coefficient block.  SAME selects the
+// form for contexts whose up and down tables are one table for both classes.
+template <bool SAME>
+__global__ __launch_bounds__(256) void k_contract_f64(KScalars ks, const int32_t* __restrict__ n_tiles,
+                                                      const int32_t* __restrict__ tile_key,
+                                                      const int32_t* __restrict__ tile_start,
+                                                      const int32_t* __restrict__ tile_len,
+                                                      const double* __restrict__ s_u, const double* __restrict__ s_v,
+                                                      const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
+                                                      double* __restrict__ g_dn) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int total = *n_tiles;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    for (int t0 = lb * wpb + wave; t0 < total; t0 += gridDim.x * wpb) {
+        const int tile = __builtin_amdgcn_readfirstlane(t0);
+        const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
+        const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);
+        const int32_t len = __builtin_amdgcn_readfirstlane(tile_len[tile]);
+        const double* cu;
+        const double* cd;
+        bool same;
+        ks.key_tables(key, cu, cd, same);
+        const int nts = (len + 15) >> 4;
+        if (nts == 4)
+            contract_f64_tile<SAME, 4>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
+        else if (nts == 3)
+            contract_f64_tile<SAME, 3>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
+        else if (nts == 2)
+            contract_f64_tile<SAME, 2>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
+        else
+            contract_f64_tile<SAME, 1>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
     }
 }
 

@@ -37,6 +37,11 @@ struct cdet_batch_plan {
     double* d_su = nullptr;
     double* d_sv = nullptr;
     int32_t* d_sdest = nullptr;
+    double* d_out_ord = nullptr;
+    double* d_single = nullptr;
+    int32_t* d_fb_list = nullptr;
+    int32_t* d_fb_count = nullptr;  // one queue length per order
+    Scratch scratch;                // minors global scratch, sized at creation
     int contract_grid = 0;
     // graph cache
     hipGraph_t graph = nullptr;
@@ -53,7 +58,8 @@ static void plan_free(cdet_batch_plan* p) {
     if (p->graph) (void)hipGraphDestroy(p->graph);
     void* bufs[] = {p->d_order, p->d_vbase, p->d_pbase, p->d_obase, p->d_lists, p->d_gup, p->d_gdn, p->d_qkey,
                     p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
-                    p->d_ntiles, p->d_su, p->d_sv, p->d_sdest};
+                    p->d_ntiles, p->d_su, p->d_sv, p->d_sdest, p->d_out_ord, p->d_single, p->d_fb_list,
+                    p->d_fb_count, p->scratch.ptr};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -67,11 +73,32 @@ static int dev_alloc(T** p, size_t n) {
 
 static int contract_launch(cdet_batch_plan* p, hipStream_t s);
 
-static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
-                                hipStream_t s) {
+static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hipStream_t s) {
     cdet_ctx* ctx = p->ctx;
+    for (int n = kMaxOrder; n >= 1; --n) {
+        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
+        if (!cnt) continue;
+        const int off = p->bucket_off[static_cast<size_t>(n)];
+        MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 2 * off, p->d_fb_count + n};
+        int rc = launch_minors(ctx, p->scratch, n, p->eval_kind, p->d_lists + off, cnt, p->d_pbase, p->d_obase,
+                               p->d_gup, p->d_gdn, o, s);
+        if (rc) return rc;
+    }
+    MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, nullptr, nullptr};
+    return launch_combine(p->eval_kind, p->count, p->out_total, p->d_obase, o, s);
+}
+
+static void plan_reset(cdet_batch_plan* p, int32_t* d_status, hipStream_t s) {
     const int64_t nk = p->nkeys;
     hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, s, p->d_hist, nk, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, p->d_fb_count, static_cast<int64_t>(kMaxOrder + 1), 0);
+    if (d_status)
+        hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((p->count + 255) / 256)), dim3(256), 0, s, d_status,
+                           p->count, 0);
+}
+
+static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, hipStream_t s) {
+    cdet_ctx* ctx = p->ctx;
     hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
                        reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
                        p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist);
@@ -79,15 +106,16 @@ static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertice
                        p->d_tstart, p->d_tlen, p->d_ntiles);
     hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>((p->nq + 255) / 256)), dim3(256), 0, s, p->nq, p->d_qkey,
                        p->d_qrank, p->d_qu, p->d_qv, p->d_keyoff, p->d_su, p->d_sv, p->d_sdest);
+}
+
+static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                                hipStream_t s) {
+    plan_reset(p, d_status, s);
+    plan_group(p, d_vertices, s);
     int rc = contract_launch(p, s);
     if (rc) return rc;
-    for (int n = kMaxOrder; n >= 1; --n) {
-        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
-        if (!cnt) continue;
-        rc = launch_fpm(ctx, n, p->eval_kind, p->d_lists + p->bucket_off[static_cast<size_t>(n)], cnt, p->d_pbase,
-                        p->d_obase, p->d_gup, p->d_gdn, d_out, d_status, s);
-        if (rc) return rc;
-    }
+    rc = plan_minors(p, d_out, d_status, s);
+    if (rc) return rc;
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -120,18 +148,22 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     auto* p = new cdet_batch_plan();
     p->ctx = ctx;
     p->contract_mode = mode;
-    p->eval_kind = eval_kind ? 1 : 0;
+    if (eval_kind < 0 || eval_kind > 2) {
+        delete p;
+        return fail(CDET_ERR_ARGUMENT, "unknown evaluation kind");
+    }
+    p->eval_kind = eval_kind;
     p->count = count;
-    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks);
+    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks, mode);
     std::vector<int64_t> vb(static_cast<size_t>(count)), pb(static_cast<size_t>(count)), ob(static_cast<size_t>(count));
     p->bucket_cnt.assign(kMaxOrder + 1, 0);
     p->bucket_off.assign(kMaxOrder + 2, 0);
     int64_t v = 0, pp = 0, o = 0;
     for (int64_t c = 0; c < count; ++c) {
         const int n = orders[c];
-        const int nmin = p->eval_kind ? 1 : 2;
+        const int nmin = p->eval_kind == 1 ? 1 : 2;
         if (n < nmin || n > kMaxOrder) {
-            const bool order_error = n <= 1 && !p->eval_kind;
+            const bool order_error = n <= 1 && p->eval_kind != 1;
             plan_free(p);
             if (order_error)
                 return fail(CDET_ERR_ORDER, "The all-subset replica integrand is defined only for envelope order >= 2");
@@ -142,7 +174,7 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
         ob[static_cast<size_t>(c)] = o;
         v += n;
         pp += static_cast<int64_t>(n) * n;
-        o += p->eval_kind ? 1 : (int64_t(1) << n);
+        o += p->eval_kind == 1 ? 1 : (int64_t(1) << n);
         p->bucket_cnt[static_cast<size_t>(n)] += 1;
     }
     p->vert_total = v;
@@ -182,6 +214,10 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     rc |= dev_alloc(&p->d_su, p->nq);
     rc |= dev_alloc(&p->d_sv, p->nq);
     rc |= dev_alloc(&p->d_sdest, p->nq);
+    rc |= dev_alloc(&p->d_out_ord, p->eval_kind == 1 ? 1 : p->out_total);
+    rc |= dev_alloc(&p->d_single, p->eval_kind == 1 ? 4 * count : 1);
+    rc |= dev_alloc(&p->d_fb_list, 2 * count);
+    rc |= dev_alloc(&p->d_fb_count, kMaxOrder + 1);
     if (rc) {
         plan_free(p);
         return CDET_ERR_DEVICE;
@@ -201,18 +237,16 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     int64_t grid = std::min<int64_t>(std::max<int64_t>(wgs, 1), 256 * 8);
     grid = ((grid + 7) / 8) * 8;
     p->contract_grid = static_cast<int>(grid);
-    // Global scratch of the orders whose minors do not fit in LDS, sized now
-    // so that runs never allocate (and can be graph-captured).
-    for (int n = 1; n <= kMaxOrder; ++n) {
-        if (!p->bucket_cnt[static_cast<size_t>(n)] || fpm_lds_bytes(n) <= kMaxLds) continue;
-        const size_t need = fpm_lds_bytes(n) * static_cast<size_t>(p->bucket_cnt[static_cast<size_t>(n)]);
-        void* g = ctx->d_gscratch;
-        rc = grow(&g, ctx->gscratch_bytes, need);
-        ctx->d_gscratch = static_cast<double*>(g);
-        if (rc) {
-            plan_free(p);
-            return rc;
-        }
+    // Global scratch (orders whose minors exceed LDS, and the repair grid),
+    // sized now so that runs never allocate and can be graph-captured.
+    size_t scratch_need = 0;
+    for (int n = 1; n <= kMaxOrder; ++n)
+        if (p->bucket_cnt[static_cast<size_t>(n)])
+            scratch_need = std::max(scratch_need, minors_scratch_bytes(n, 2 * p->bucket_cnt[static_cast<size_t>(n)]));
+    rc = grow(&p->scratch.ptr, p->scratch.cap, scratch_need);
+    if (rc) {
+        plan_free(p);
+        return rc;
     }
     if (mode == CDET_MODE_SPLIT32) {
         rc = ensure_split_tables(ctx);
@@ -274,30 +308,17 @@ int cdet_batch_profile(cdet_batch_plan* p, const cdet_vertex* d_vertices, double
                        void* stream, float* stage_ms) {
     if (!p || !d_vertices || !d_out || !stage_ms) return fail(CDET_ERR_ARGUMENT, "cdet_batch_profile: bad argument");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
-    cdet_ctx* ctx = p->ctx;
     hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    const int64_t nk = p->nkeys;
     HIP_TRY(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, s, p->d_hist, nk, 0);
-    hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
-                       reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
-                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist);
-    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, s, p->d_hist, p->nkeys, p->tile, p->d_keyoff, p->d_tkey,
-                       p->d_tstart, p->d_tlen, p->d_ntiles);
-    hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>((p->nq + 255) / 256)), dim3(256), 0, s, p->nq, p->d_qkey,
-                       p->d_qrank, p->d_qu, p->d_qv, p->d_keyoff, p->d_su, p->d_sv, p->d_sdest);
+    plan_reset(p, d_status, s);
+    plan_group(p, d_vertices, s);
     HIP_TRY(hipEventRecord(ev[1], s));
     int rc = contract_launch(p, s);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ev[2], s));
-    for (int n = kMaxOrder; n >= 1; --n) {
-        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
-        if (!cnt) continue;
-        rc = launch_fpm(ctx, n, p->eval_kind, p->d_lists + p->bucket_off[static_cast<size_t>(n)], cnt, p->d_pbase,
-                        p->d_obase, p->d_gup, p->d_gdn, d_out, d_status, s);
-        if (rc) return rc;
-    }
+    rc = plan_minors(p, d_out, d_status, s);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(ev[3], s));
     HIP_TRY(hipEventSynchronize(ev[3]));
     for (int k = 0; k < 3; ++k) HIP_TRY(hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]));
@@ -353,7 +374,7 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     }
     QueryScratch& q = query_scratch(ctx);
     const int32_t nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
-    const int tile = mode == CDET_MODE_EXACT ? 64 : kMfmaTile;
+    const int tile = mode == CDET_MODE_EXACT ? 64 : mfma_tile_queries(ctx->ks, mode);
     const int64_t max_tiles = count / tile + std::min<int64_t>(nkeys, count) + 1;
     if (q.cap < count || q.tiles_cap < max_tiles || !q.hist) {
         void* bufs[] = {q.gup, q.gdn, q.qu, q.qv, q.su, q.sv, q.qkey, q.qrank, q.sdest, q.hist, q.keyoff,

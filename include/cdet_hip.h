@@ -180,7 +180,12 @@ int cdet_eval_all_subsets(cdet_ctx* ctx, const cdet_vertex* vertices, int envelo
  * all-subset kind, outputs [sum_{c'<c} 2^n_c', ... + 2^n_c) (out[mask] as
  * cdet_eval_all_subsets); for the single-order kind one output per
  * configuration.  mode is one of CDET_MODE_*; eval_kind 0 = all subsets
- * (orders 2..CDET_MAX_ORDER), 1 = single order (orders 1..CDET_MAX_ORDER).
+ * (orders 2..CDET_MAX_ORDER), 1 = single order (orders 1..CDET_MAX_ORDER),
+ * 2 = conditioning scale of every all-subset value (same layout as kind 0):
+ * |prefactor| * (|a[mask]| + sum of |subtracted terms|) of the subset
+ * recursion, summed as twisted + 2*ordinary; a value computed with relative
+ * perturbations of size e in its inputs can differ from the exact one by
+ * about e times this scale, however small the value itself.
  * Batched form of the reference's per-chain calls
  * (include/cdet/mcmcmc_sampler.h:538, :624-625; cdet_integrand_evaluator.h:542-583).
  */

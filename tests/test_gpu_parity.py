@@ -89,10 +89,10 @@ def test_reference_envelopes_on_gpu(gpu_available):
             assert abs(allv[m] - d) <= 2e-9 + 2e-9 * max(1.0, abs(allv[m]), abs(d))
 
 
-def run_batch(ctx, configs, mode, single=False, graph=False, repeats=1):
+def run_batch(ctx, configs, mode, single=False, graph=False, repeats=1, kind=None):
     import torch
     orders = [len(v) for v in configs]
-    plan = ctx.plan(orders, mode=mode, single=single)
+    plan = ctx.plan(orders, mode=mode, single=single, kind=kind)
     verts = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
     out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
     status = torch.zeros(len(orders), dtype=torch.int32, device="cuda")
@@ -145,7 +145,7 @@ def rel_err(got, want):
     return np.abs(got - want) / scale
 
 
-@pytest.mark.parametrize("name", ["ref2x2", "c1"])
+@pytest.mark.parametrize("name", ["ref2x2", "c1", "c1_neq"])
 def test_batched_mfma64_within_tolerance(gpu_available, name):
     # FP64 matrix-core contraction: table values agree to ~1e-15 relative; the
     # integrand is checked at 1e-9 relative per mask, measured against the
@@ -165,6 +165,26 @@ def test_batched_mfma64_within_tolerance(gpu_available, name):
         worst = max(worst, err)
     say(f"mfma64 {name} worst scaled error {worst:.3e}")
     assert worst < 1e-9
+
+
+@pytest.mark.parametrize("name", ["c1", "c1_neq"])
+def test_conditioning_scale_bounds_values_and_mfma_error(gpu_available, name):
+    # The conditioning scale of a value is the sum of |a[mask]| and |terms| of
+    # its subset chain, combined as |twisted| + 2 |ordinary|; it bounds |value|
+    # and, times a few ulps of the operands, the error of any FP64 evaluation
+    # order of the same terms.
+    ctx = gpu_ctx(name)
+    orders = [2, 4, 6, 8, 9]
+    configs = helpers.seeded_configs(5100 + len(name), orders)
+    say(f"conditioning scales {name}")
+    (cond,), ob, _ = run_batch(ctx, configs, "exact", kind=2)
+    (ex,), _, _ = run_batch(ctx, configs, "exact")
+    (mf,), _, _ = run_batch(ctx, configs, "mfma64")
+    assert np.all(np.isfinite(cond)) and np.all(cond >= 0)
+    assert np.all(cond >= np.abs(ex) * (1 - 1e-12))
+    worst = float(np.max(np.abs(mf - ex) / np.maximum(cond, 1e-300)))
+    say(f"{name} max |mfma64 - exact| / conditioning = {worst:.3e}")
+    assert worst < 1e-11
 
 
 def test_split32_runs_and_reports_error(gpu_available):
@@ -231,10 +251,9 @@ def test_nonfinite_tables_raise_reference_message(gpu_available):
     ctx.close()
 
 
-@pytest.mark.parametrize("mode", ["exact", "mfma64"])
-def test_green_pair_eval(gpu_available, mode):
+@pytest.mark.parametrize("mode,name", [("exact", "c1"), ("mfma64", "c1"), ("mfma64", "c1_neq")])
+def test_green_pair_eval(gpu_available, mode, name):
     import torch
-    name = "c1"
     ctx = gpu_ctx(name)
     orc = helpers.oracle_of(name)
     rng = np.random.default_rng(3)

@@ -8,19 +8,19 @@ namespace cdet {
 
 // One mask of the subset recursion: acc = a[mask] - sum over the
 // anchor-containing proper subsets `anchor | s` (s over the proper subsets of
-// rest, descending) of dp[sub] * a[mask ^ sub], with the running |term| sum,
-// exactly the reference's sequence
-// (include/cdet/integrand_replica_randomly_sampled.h:326-343).  The operand
-// loads of eight consecutive terms are issued before their dependent
-// subtractions so that LDS latency overlaps; the arithmetic order is
+// rest, descending) of dp[sub] * a[mask ^ sub], with the running |term| sum
+// in rhs_out, exactly the reference's sequence
+// (include/cdet/integrand_replica_randomly_sampled.h:326-343).  Operands are
+// read eight terms at a time, one chunk ahead of the dependent subtractions,
+// so that LDS latency overlaps the serial chain; the arithmetic order is
 // unchanged.
 __device__ inline double connected_chain(const double* a, const double* dp, uint32_t mask, uint32_t anchor,
-                                         uint32_t rest, int terms, double thr, bool filter, double* rhs_out = nullptr) {
+                                         uint32_t rest, int terms, double thr, bool filter, double& rhs_out) {
     double acc = a[mask];
-    double rhs = fabs(a[mask]);
+    double rhs = fabs(acc);
     uint32_t s = (rest - 1u) & rest;
     int done = 0;
-    for (; done + 8 <= terms; done += 8) {
+    if (terms >= 8) {
         double dv[8], av[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -29,22 +29,55 @@ __device__ inline double connected_chain(const double* a, const double* dp, uint
             av[u] = a[mask ^ sub];
             s = (s - 1u) & rest;
         }
+        for (; done + 16 <= terms; done += 8) {
+            double dn[8], an[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t sub = anchor | s;
+                dn[u] = dp[sub];
+                an[u] = a[mask ^ sub];
+                s = (s - 1u) & rest;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double term = dv[u] * av[u];
+                acc = acc - term;
+                rhs += fabs(term);
+                dv[u] = dn[u];
+                av[u] = an[u];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const double term = dv[u] * av[u];
             acc = acc - term;
             rhs += fabs(term);
         }
+        done += 8;
     }
-    for (; done < terms; ++done) {
-        const uint32_t sub = anchor | s;
-        const double term = dp[sub] * a[mask ^ sub];
-        acc = acc - term;
-        rhs += fabs(term);
-        s = (s - 1u) & rest;
+    const int tail = terms - done;  // < 8
+    if (tail > 0) {
+        double dv[7], av[7];
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
+            if (u < tail) {
+                const uint32_t sub = anchor | s;
+                dv[u] = dp[sub];
+                av[u] = a[mask ^ sub];
+                s = (s - 1u) & rest;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
+            if (u < tail) {
+                const double term = dv[u] * av[u];
+                acc = acc - term;
+                rhs += fabs(term);
+            }
+        }
     }
     if (filter && fabs(acc) < thr * rhs) acc = 0.0;
-    if (rhs_out) *rhs_out = rhs;
+    rhs_out = rhs;
     return acc;
 }
 
@@ -70,8 +103,8 @@ __device__ inline void ztransform_all_block(const double* a, double* dp, int n, 
         for (int it = beg + tid; it < end; it += nt) {
             const uint32_t mask = masks_by_pop[it];
             const uint32_t anchor = mask & (~mask + 1u);
-            double r = 0.0;
-            dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, terms, thr, filter, rhs ? &r : nullptr);
+            double r;
+            dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, terms, thr, filter, r);
             if (rhs) rhs[mask] = r;
         }
         __syncthreads();
@@ -93,7 +126,8 @@ __device__ inline double ztransform_single_block(const double* a, double* dp, in
         const int terms = (1 << (bits - 1)) - 1;
         for (int it = beg + tid; it < end; it += nt) {
             const uint32_t mask = masks_by_pop[it];
-            dp[mask] = (mask & 1u) ? connected_chain(a, dp, mask, 1u, mask ^ 1u, terms, thr, filter) : a[mask];
+            double r;
+            dp[mask] = (mask & 1u) ? connected_chain(a, dp, mask, 1u, mask ^ 1u, terms, thr, filter, r) : a[mask];
         }
         __syncthreads();
     }

@@ -198,62 +198,82 @@ struct UClenshaw {
 // Scratch: bufA/bufB hold max_l 2^(l+1)(n-l-1)^2 doubles (and n*n), inv holds
 // 2^(n-2) doubles, red holds blockDim doubles.
 // ---------------------------------------------------------------------------
-__device__ inline double det_minor_brute(const double* A, int n, uint32_t mask) {
-    int idx[kMaxOrder];
-    int k = 0;
-    for (int i = 0; i < n; ++i)
-        if (mask & (1u << i)) idx[k++] = i;
+// Pivoted determinant of A[mask, mask] with the row elimination of
+// include/cdet/fast_principle_minors.h:39-115.  The k x k working matrix of
+// the calling thread lives at w with element (r, c) at w[(r k + c) ws], so a
+// workgroup's threads interleave their matrices (ws = blockDim) in LDS.
+__device__ inline double det_minor_brute(const double* A, int n, uint32_t mask, double* w, int ws) {
+    const int k = __builtin_popcount(mask);
     if (k == 0) return 1.0;
-    double m[kMaxOrder * kMaxOrder];
-    for (int r = 0; r < k; ++r)
-        for (int c = 0; c < k; ++c) m[r * k + c] = A[idx[r] * n + idx[c]];
+    int r = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!(mask & (1u << i))) continue;
+        int cc = 0;
+        for (int j = 0; j < n; ++j) {
+            if (!(mask & (1u << j))) continue;
+            w[(r * k + cc) * ws] = A[i * n + j];
+            ++cc;
+        }
+        ++r;
+    }
+    auto W = [&](int rr, int c) -> double& { return w[(rr * k + c) * ws]; };
     double det = 1.0;
     int sign = 1;
     for (int col = 0; col < k; ++col) {
         int prow = col;
-        double pabs = fabs(m[col * k + col]);
-        for (int r = col + 1; r < k; ++r) {
-            const double cand = fabs(m[r * k + col]);
+        double pabs = fabs(W(col, col));
+        for (int rr = col + 1; rr < k; ++rr) {
+            const double cand = fabs(W(rr, col));
             if (cand > pabs) {
                 pabs = cand;
-                prow = r;
+                prow = rr;
             }
         }
         if (pabs == 0.0) return 0.0;
         if (prow != col) {
             for (int c = col; c < k; ++c) {
-                const double t = m[col * k + c];
-                m[col * k + c] = m[prow * k + c];
-                m[prow * k + c] = t;
+                const double t = W(col, c);
+                W(col, c) = W(prow, c);
+                W(prow, c) = t;
             }
             sign = -sign;
         }
-        const double piv = m[col * k + col];
+        const double piv = W(col, col);
         det *= piv;
-        for (int r = col + 1; r < k; ++r) {
-            const double f = m[r * k + col] / piv;
+        for (int rr = col + 1; rr < k; ++rr) {
+            const double f = W(rr, col) / piv;
             if (f == 0.0) continue;
-            m[r * k + col] = 0.0;
-            for (int c = col + 1; c < k; ++c) m[r * k + c] = __builtin_fma(-f, m[col * k + c], m[r * k + c]);
+            W(rr, col) = 0.0;
+            for (int c = col + 1; c < k; ++c) W(rr, c) = __builtin_fma(-f, W(col, c), W(rr, c));
         }
     }
     return sign < 0 ? -det : det;
 }
 
+// Maximum over the workgroup: butterfly within each wave, then across the
+// waves through red[] (exact for any order).
 __device__ inline double block_max(double v, double* red) {
-    const int tid = threadIdx.x;
-    red[tid] = v;
-    __syncthreads();
-    for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
-        if (tid < s) {
-            const double o = red[tid + s];
-            if (o > red[tid]) red[tid] = o;
-        }
-        __syncthreads();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = o > v ? o : v;
     }
-    const double r = red[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int w = 1; w < nw; ++w) r = red[w] > r ? red[w] : r;
     __syncthreads();
     return r;
+}
+
+// x / d for 0 <= x < 2^22 and 1 <= d < 2^10, with inv_d = 1.0f / d: the float
+// quotient is within one of the true one and is corrected exactly.
+__device__ inline int div_small(int x, int d, float inv_d) {
+    int q = static_cast<int>(static_cast<float>(x) * inv_d);
+    if (q * d > x) --q;
+    else if ((q + 1) * d <= x) ++q;
+    return q;
 }
 
 // out[mask] = det(A[mask, mask]) for every mask (out[0] = 1).  With
@@ -261,11 +281,12 @@ __device__ inline double block_max(double v, double* red) {
 // leaving out undefined, as soon as a pivot |A00 + p| falls to
 // tol * max|A| (the reference then switches to pivoted determinants,
 // include/cdet/fast_principle_minors.h:180-186); with brute = true the
-// pivoted determinants are formed directly.  Scratch: bufA/bufB hold
-// fpm_list_size(n) >= 2^n doubles, inv 2^(n-2), red blockDim.
+// pivoted determinants, formed beforehand by k_fallback_dets, are taken
+// from pre[mask].  Scratch: bufA/bufB hold fpm_list_size(n) >= 2^n doubles,
+// inv 2^(n-2), red blockDim.
 template <bool BRUTE>
 __device__ inline int fpm_block(const double* A, int n, double tol, double thresh, double* out, double* bufA,
-                                double* bufB, double* inv, double* red, int* flag) {
+                                double* bufB, double* inv, double* red, int* flag, const double* pre = nullptr) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
@@ -285,7 +306,7 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
         return 0;
     }
     if (BRUTE) {
-        for (uint32_t m = tid; m < sc; m += nt) out[m] = (m == 0) ? 1.0 : det_minor_brute(A, n, m);
+        for (uint32_t m = tid; m < sc; m += nt) out[m] = (m == 0) ? 1.0 : pre[m];
         __syncthreads();
         return 0;
     }
@@ -317,10 +338,12 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
         __syncthreads();
         if (*flag) return 1;
         const int items = sets * nms;
+        const float inv_nms = 1.0f / static_cast<float>(nms);
+        const float inv_ns = 1.0f / static_cast<float>(ns);
         for (int it = tid; it < items; it += nt) {
-            const int s = it / nms;
+            const int s = div_small(it, nms, inv_nms);
             const int rem = it - s * nms;
-            const int r = rem / ns;
+            const int r = div_small(rem, ns, inv_ns);
             const int c = rem - r * ns;
             const int src = s * oms;
             const double f = oldl[src + (r + 1) * os] * inv[s];

@@ -387,7 +387,7 @@ __device__ inline double cheb2d_exact(const double* __restrict__ c, int nu, int 
 // Exact batched contraction: one wave per tile of <= 64 queries sharing a
 // coefficient block; lane = query.  The block address is wave-uniform, so the
 // coefficients stream through the scalar cache and are broadcast to all
-// lanes.  Tiles of one block are consecutive and stay on one XCD (L2).
+// lanes.
 __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32_t* __restrict__ n_tiles,
                                                         const int32_t* __restrict__ tile_key,
                                                         const int32_t* __restrict__ tile_start,
@@ -399,8 +399,7 @@ __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int waves_per_block = blockDim.x >> 6;
     const int total = *n_tiles;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    for (int t = lb * waves_per_block + wave; t < total; t += gridDim.x * waves_per_block) {
+    for (int t = blockIdx.x * waves_per_block + wave; t < total; t += gridDim.x * waves_per_block) {
         const int tile = __builtin_amdgcn_readfirstlane(t);
         const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
         const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);
@@ -423,187 +422,228 @@ __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32
     }
 }
 
-// Order-bucketed vertex matrices -> principal minors -> connected values ->
-// assembly.  One workgroup per (configuration, table class): blockIdx.x =
-// 2 * list index + class, class 0 = twisted (rc = 2), 1 = ordinary (rc = 1).
+// Order-bucketed principal minors and connected values, two kernels per
+// order bucket (list index li = position in the bucket's configuration list):
+//
+// k_fpm   one workgroup per spin matrix, work item w = 4 li + 2 class + spin
+//         (class 0 = twisted, rc = 2; 1 = ordinary, rc = 1).  Builds the
+//         vertex matrix (include/cdet/integrand_replica_randomly_sampled.h:751-761),
+//         runs the Schur-complement tree and writes the principal minors to
+//         pm[w 2^n + mask].  A matrix whose pivot needs the reference's
+//         fallback is queued instead (entry = (li << 3) | (class << 2) | spin
+//         mask) and k_fallback_dets writes its pivoted determinants to the same
+//         place.  Identical up and down tables (same_spin) use spin 0 for both.
+// k_cdet  one workgroup per (li, class), w = 2 li + class:
+//         a = pm_up * pm_down, subset recursion, assembly.
 //   mode 0 (all subsets): class values into out (twisted) / out_ord
 //          (ordinary) at [obase, obase + 2^n); k_combine_all forms
 //          twisted - 2 * ordinary.
 //   mode 1 (single order): class value as a double-double pair into
 //          single_tmp[4 c + 2 class + {0, 1}]; k_combine_single forms the
 //          double-double difference.
-// A spin matrix whose Schur pivot needs the reference's brute-force fallback
-// is queued in fb_list (entry = (list index << 3) | (class << 2) | spin mask)
-// and finished by k_minors_repair, so this kernel carries no per-lane
-// scratch.
-struct MinorsLds {
+//   mode 2: conditioning scales of the mode-0 values, same layout.
+// With GSCR (orders whose workspace exceeds LDS) the workspace is global
+// scratch and a fixed grid loops over the work items; GSCR is a template
+// argument so that LDS accesses compile to ds_read/ds_write.
+struct FpmLds {
     static __host__ __device__ inline size_t doubles(int n) {
         const size_t sc = size_t(1) << n;
-        const size_t list = static_cast<size_t>(fpm_list_size(n));
         const size_t inv = n >= 2 ? (size_t(1) << (n - 2)) : 1;
-        return 2 * n * n + 2 * n + 2 * sc + 2 * list + inv + 256;
+        return static_cast<size_t>(n) * n + sc + 2 * static_cast<size_t>(fpm_list_size(n)) + inv + 64;
     }
-    static __host__ __device__ inline size_t bytes(int n) { return sizeof(double) * doubles(n) + 64; }
+    static __host__ __device__ inline size_t bytes(int n) { return sizeof(double) * doubles(n); }
+};
+struct CdetLds {
+    static __host__ __device__ inline size_t doubles(int n, int mode) {
+        const size_t sc = size_t(1) << n;
+        return (mode == 2 ? 3 : 2) * sc + 2 * static_cast<size_t>(n) + sc / 4 + 1;
+    }
+    static __host__ __device__ inline size_t bytes(int n, int mode) { return sizeof(double) * doubles(n, mode); }
 };
 
-template <bool REPAIR>
-__device__ inline int minors_connected(const KScalars& ks, int n, int mode, int c, int cls, int spin_brute,
-                                       const int64_t* __restrict__ cfg_pbase, const int64_t* __restrict__ cfg_obase,
-                                       const double* __restrict__ g_up, const double* __restrict__ g_dn,
-                                       double* __restrict__ out, double* __restrict__ out_ord,
-                                       double* __restrict__ single_tmp, int32_t* __restrict__ status,
-                                       const uint16_t* __restrict__ masks_by_pop, const int* __restrict__ pop_off,
-                                       double* base, int* s_flag) {
+template <bool GSCR>
+__global__ __launch_bounds__(256) void k_fpm(KScalars ks, int n, int items, const int32_t* __restrict__ cfg_list,
+                                             const int64_t* __restrict__ cfg_pbase, const double* __restrict__ g_up,
+                                             const double* __restrict__ g_dn, double* __restrict__ pm,
+                                             int32_t* __restrict__ fb_list, int32_t* __restrict__ fb_count,
+                                             double* __restrict__ gscratch, size_t gscratch_stride) {
+    extern __shared__ double lds_dyn[];
+    __shared__ int s_flag;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
     const int n2 = n * n;
     const int list = fpm_list_size(n);
-    const int ninv = n >= 2 ? (1 << (n - 2)) : 1;
-    double* mU = base;
-    double* mD = mU + n2;
-    double* dU = mD + n2;
-    double* dD = dU + n;
-    double* smU = dD + n;
-    double* smD = smU + sc;
-    double* bufA = smD + sc;
+    double* A = GSCR ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
+    double* fo = A + n2;
+    double* bufA = fo + sc;
     double* bufB = bufA + list;
     double* inv = bufB + list;
-    double* red = inv + ninv;
-    const bool same = ks.same_spin[cls] != 0;
-    const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
-    const int64_t ob = cfg_obase[c];
-    // Vertex matrices (include/cdet/integrand_replica_randomly_sampled.h:751-761).
-    for (int idx = tid; idx < n2; idx += nt) {
-        const int i = idx / n;
-        const int j = idx - i * n;
-        double vu = g_up[pb + idx];
-        double vd = g_dn[pb + idx];
-        if (i == j) {
-            dU[i] = vu + ks.dup;
-            dD[i] = vd + ks.ddn;
-            vu += ks.dup;
-            vd += ks.ddn;
+    double* red = inv + (n >= 2 ? (1 << (n - 2)) : 1);
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        const int li = w >> 2;
+        const int cls = (w >> 1) & 1;
+        const int spin = w & 1;
+        const bool same = ks.same_spin[cls] != 0;
+        if (spin == 1 && same) continue;  // uniform across the workgroup
+        const int c = cfg_list[li];
+        const double* g = (spin ? g_dn : g_up) + 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
+        const double shift = spin ? ks.ddn : ks.dup;
+        for (int idx = tid; idx < n2; idx += nt) {
+            const int i = idx / n;
+            double v = g[idx];
+            if (i * n + i == idx) v += shift;
+            A[idx] = v * ks.scale;
         }
-        mU[idx] = vu * ks.scale;
-        mD[idx] = vd * ks.scale;
-    }
-    __syncthreads();
-    if (n == 1) {  // order-1 single value (:173-213), no minors
-        if (tid == 0) {
-            const double v = ks.c1[cls] * (dU[0] * dD[0] - ks.counter);
-            single_tmp[4 * c + 2 * cls] = v;
-            single_tmp[4 * c + 2 * cls + 1] = 0.0;
-        }
-        return 0;
-    }
-    int need = 0;
-    if (REPAIR && (spin_brute & 1))
-        fpm_block<true>(mU, n, ks.tol, ks.shift, smU, bufA, bufB, inv, red, s_flag);
-    else if (fpm_block<false>(mU, n, ks.tol, ks.shift, smU, bufA, bufB, inv, red, s_flag))
-        need |= 1;
-    if (same) {
-        if (need) need |= 2;
-        else
-            for (uint32_t m = tid; m < sc; m += nt) smD[m] = smU[m];
-    } else if (REPAIR && (spin_brute & 2)) {
-        fpm_block<true>(mD, n, ks.tol, ks.shift, smD, bufA, bufB, inv, red, s_flag);
-    } else if (fpm_block<false>(mD, n, ks.tol, ks.shift, smD, bufA, bufB, inv, red, s_flag)) {
-        need |= 2;
-    }
-    if (!REPAIR && need) return need;  // uniform across the workgroup
-    if (REPAIR && same && (spin_brute & 1))
-        for (uint32_t m = tid; m < sc; m += nt) smD[m] = smU[m];
-    __syncthreads();
-    // a[mask] = pm_up * pm_down with a[0] = 0 (stored over smU).
-    for (uint32_t m = tid; m < sc; m += nt) smU[m] = (m == 0) ? 0.0 : smU[m] * smD[m];
-    __syncthreads();
-    if (mode == 0 || mode == 2) {
-        // mode 2 writes the magnitude scale of each value instead of the value:
-        // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
-        double* rhs = mode == 2 ? bufA : nullptr;
-        ztransform_all_block(smU, smD, n, ks.thr, ks.filter != 0, masks_by_pop, pop_off, rhs);
-        // Assembly (:833-877).
-        double* dst = (cls == 0 ? out : out_ord) + ob;
-        bool bad = false;
-        for (uint32_t m = tid; m < sc; m += nt) {
-            double v;
-            if (m == 0) {
-                v = 0.0;
-            } else if ((m & (m - 1)) == 0) {
-                const int i = __builtin_ctz(m);
-                v = ks.c1[cls] * (dU[i] * dD[i] - ks.counter);
-                if (mode == 2) v = fabs(v);
-            } else {
-                v = mode == 2 ? fabs(ks.pref[cls][__builtin_popcount(m)]) * rhs[m]
-                              : ks.pref[cls][__builtin_popcount(m)] * smD[m];
+        __syncthreads();
+        if (fpm_block<false>(A, n, ks.tol, ks.shift, fo, bufA, bufB, inv, red, &s_flag)) {
+            if (tid == 0) {
+                const int slot = atomicAdd(fb_count, 1);
+                fb_list[slot] = (li << 3) | (cls << 2) | (same ? 3 : (1 << spin));
             }
-            if (!isfinite(v)) bad = true;
-            dst[m] = v;
+        } else {
+            double* dst = pm + static_cast<size_t>(w) * sc;
+            for (uint32_t m = tid; m < sc; m += nt) dst[m] = fo[m];
         }
-        if (bad && status) atomicOr(&status[c], 1);
-    } else {
-        const double cres = ztransform_single_block(smU, smD, n, ks.thr, ks.filter != 0, masks_by_pop, pop_off);
-        if (tid == 0) {
-            const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
-            single_tmp[4 * c + 2 * cls] = v.hi;
-            single_tmp[4 * c + 2 * cls + 1] = v.lo;
-            if (!isfinite(v.hi + v.lo) && status) atomicOr(&status[c], 1);
-        }
+        __syncthreads();
     }
-    return 0;
 }
 
-__global__ __launch_bounds__(256) void k_minors(KScalars ks, int n, int mode, const int32_t* __restrict__ cfg_list,
-                                                const int64_t* __restrict__ cfg_pbase,
-                                                const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up,
-                                                const double* __restrict__ g_dn, double* __restrict__ out,
-                                                double* __restrict__ out_ord, double* __restrict__ single_tmp,
-                                                int32_t* __restrict__ status, int32_t* __restrict__ fb_list,
-                                                int32_t* __restrict__ fb_count,
-                                                const uint16_t* __restrict__ masks_by_pop,
-                                                const int* __restrict__ pop_off_all, double* __restrict__ gscratch,
-                                                size_t gscratch_stride) {
+// Pivoted determinants of every principal submatrix of the queued fallback
+// spin matrices (include/cdet/fast_principle_minors.h:39-115, :180-186),
+// spread over a fixed grid: work item = (queue entry, spin, chunk of 64
+// masks), one mask per thread, the thread's k x k working matrix interleaved
+// in LDS.  Counts the fallbacks (include/cdet/runtime_counters.h:27) and
+// flags the configuration's status.
+constexpr int kDetThreads = 64;
+__global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
+    KScalars ks, int n, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
+    const double* __restrict__ g_up, const double* __restrict__ g_dn, const int32_t* __restrict__ fb_list,
+    const int32_t* __restrict__ fb_count, double* __restrict__ pm, int32_t* __restrict__ status,
+    uint64_t* __restrict__ fallback_total) {
     extern __shared__ double lds_dyn[];
-    __shared__ int s_flag;
-    const int li = blockIdx.x >> 1;
-    const int cls = blockIdx.x & 1;
-    const int c = cfg_list[li];
-    double* base = gscratch ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
-    const int need = minors_connected<false>(ks, n, mode, c, cls, 0, cfg_pbase, cfg_obase, g_up, g_dn, out, out_ord,
-                                             single_tmp, status, masks_by_pop, pop_off_all + n * (kMaxOrder + 2),
-                                             base, &s_flag);
-    if (need && threadIdx.x == 0) {
-        const int slot = atomicAdd(fb_count, 1);
-        fb_list[slot] = (li << 3) | (cls << 2) | need;
-    }
-}
-
-// Finishes the queued fallback spins with pivoted determinants
-// (include/cdet/fast_principle_minors.h:39-115, :180-186).  Fixed grid,
-// loops over the device-side queue.
-__global__ __launch_bounds__(256) void k_minors_repair(
-    KScalars ks, int n, int mode, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
-    const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up, const double* __restrict__ g_dn,
-    double* __restrict__ out, double* __restrict__ out_ord, double* __restrict__ single_tmp,
-    int32_t* __restrict__ status, const int32_t* __restrict__ fb_list, const int32_t* __restrict__ fb_count,
-    uint64_t* __restrict__ fallback_total, const uint16_t* __restrict__ masks_by_pop,
-    const int* __restrict__ pop_off_all, double* __restrict__ gscratch, size_t gscratch_stride) {
-    __shared__ int s_flag;
-    const int total = *fb_count;
-    double* base = gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride;
-    for (int e = blockIdx.x; e < total; e += gridDim.x) {
+    const int tid = threadIdx.x;
+    const int n2 = n * n;
+    const uint32_t sc = 1u << n;
+    const int chunks = static_cast<int>((sc + kDetThreads - 1) / kDetThreads);
+    double* A = lds_dyn;
+    double* ws = lds_dyn + n2;
+    const int total = *fb_count * 2 * chunks;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const int e = w / (2 * chunks);
+        const int spin = (w / chunks) & 1;
+        const int chunk = w - (w / chunks) * chunks;
         const int entry = fb_list[e];
         const int li = entry >> 3;
         const int cls = (entry >> 2) & 1;
         const int spins = entry & 3;
+        // spin 1 of identical spin tables is never read (the up minors serve both)
+        if (!((spins >> spin) & 1) || (spin == 1 && ks.same_spin[cls])) continue;
         const int c = cfg_list[li];
-        minors_connected<true>(ks, n, mode, c, cls, spins, cfg_pbase, cfg_obase, g_up, g_dn, out, out_ord,
-                               single_tmp, status, masks_by_pop, pop_off_all + n * (kMaxOrder + 2), base, &s_flag);
-        if (threadIdx.x == 0) {
+        if (chunk == 0 && tid == 0 && spin == __builtin_ctz(spins)) {
             atomicAdd(reinterpret_cast<unsigned long long*>(fallback_total),
                       static_cast<unsigned long long>(__builtin_popcount(spins)));
             if (status) atomicOr(&status[c], 2);
+        }
+        const double* g = (spin == 0 ? g_up : g_dn) + 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
+        const double shift = spin == 0 ? ks.dup : ks.ddn;
+        __syncthreads();
+        for (int idx = tid; idx < n2; idx += kDetThreads) {
+            const int i = idx / n;
+            double v = g[idx];
+            if (i * n + i == idx) v += shift;
+            A[idx] = v * ks.scale;
+        }
+        __syncthreads();
+        const uint32_t m = static_cast<uint32_t>(chunk) * kDetThreads + tid;
+        if (m < sc) {
+            const double det = m == 0 ? 1.0 : det_minor_brute(A, n, m, ws + tid, kDetThreads);
+            pm[static_cast<size_t>(4 * li + 2 * cls + spin) * sc + m] = det;
+        }
+    }
+}
+
+template <bool GSCR>
+__global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int items,
+                                              const int32_t* __restrict__ cfg_list,
+                                              const int64_t* __restrict__ cfg_pbase,
+                                              const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up,
+                                              const double* __restrict__ g_dn, const double* __restrict__ pm,
+                                              double* __restrict__ out, double* __restrict__ out_ord,
+                                              double* __restrict__ single_tmp, int32_t* __restrict__ status,
+                                              const uint16_t* __restrict__ masks_by_pop,
+                                              const int* __restrict__ pop_off, double* __restrict__ gscratch,
+                                              size_t gscratch_stride) {
+    extern __shared__ double lds_dyn[];
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const uint32_t sc = 1u << n;
+    const int n2 = n * n;
+    double* a = GSCR ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
+    double* dp = a + sc;
+    double* rhs = dp + sc;  // mode 2 only
+    double* dU = rhs + (mode == 2 ? sc : 0);
+    double* dD = dU + n;
+    uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + n);
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        const int li = w >> 1;
+        const int cls = w & 1;
+        const int c = cfg_list[li];
+        const bool same = ks.same_spin[cls] != 0;
+        const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
+        // unscaled shifted diagonal entries for the singletons (:751-758)
+        for (int i = tid; i < n; i += nt) {
+            dU[i] = g_up[pb + i * n + i] + ks.dup;
+            dD[i] = g_dn[pb + i * n + i] + ks.ddn;
+        }
+        if (n == 1) {  // order-1 single value (:173-213), no minors
+            __syncthreads();
+            if (tid == 0) {
+                single_tmp[4 * c + 2 * cls] = ks.c1[cls] * (dU[0] * dD[0] - ks.counter);
+                single_tmp[4 * c + 2 * cls + 1] = 0.0;
+            }
+            __syncthreads();
+            continue;
+        }
+        const double* pu = pm + static_cast<size_t>(2 * w) * sc;
+        const double* pd = same ? pu : pu + sc;
+        // a[mask] = pm_up * pm_down with a[0] = 0
+        for (uint32_t m = tid; m < sc; m += nt) {
+            a[m] = (m == 0) ? 0.0 : pu[m] * pd[m];
+            lmask[m] = masks_by_pop[m];
+        }
+        __syncthreads();
+        if (mode == 0 || mode == 2) {
+            // mode 2 writes the magnitude scale of each value instead of the value:
+            // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
+            ztransform_all_block(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off, mode == 2 ? rhs : nullptr);
+            // Assembly (:833-877).
+            double* dst = (cls == 0 ? out : out_ord) + cfg_obase[c];
+            bool bad = false;
+            for (uint32_t m = tid; m < sc; m += nt) {
+                double v;
+                if (m == 0) {
+                    v = 0.0;
+                } else if ((m & (m - 1)) == 0) {
+                    const int i = __builtin_ctz(m);
+                    v = ks.c1[cls] * (dU[i] * dD[i] - ks.counter);
+                    if (mode == 2) v = fabs(v);
+                } else {
+                    v = mode == 2 ? fabs(ks.pref[cls][__builtin_popcount(m)]) * rhs[m]
+                                  : ks.pref[cls][__builtin_popcount(m)] * dp[m];
+                }
+                if (!isfinite(v)) bad = true;
+                dst[m] = v;
+            }
+            if (bad && status) atomicOr(&status[c], 1);
+        } else {
+            const double cres = ztransform_single_block(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off);
+            if (tid == 0) {
+                const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
+                single_tmp[4 * c + 2 * cls] = v.hi;
+                single_tmp[4 * c + 2 * cls + 1] = v.lo;
+                if (!isfinite(v.hi + v.lo) && status) atomicOr(&status[c], 1);
+            }
         }
         __syncthreads();
     }
@@ -822,10 +862,11 @@ int grow(void** p, size_t& cap, size_t need) {
 
 }  // namespace
 
-// Sizing helper: LDS bytes of k_minors for order n.
-static size_t fpm_lds_bytes(int n) { return MinorsLds::bytes(n); }
 static constexpr size_t kMaxLds = 160 * 1024 - 2048;
-static constexpr int kRepairGrid = 64;
+static constexpr int kDetGrid = 512;
+static constexpr int kGscrGrid = 1024;  // workgroups of the global-scratch forms
+static bool fpm_gscr(int n) { return FpmLds::bytes(n) > kMaxLds; }
+static bool cdet_gscr(int n, int mode) { return CdetLds::bytes(n, mode) > kMaxLds; }
 
 // Outputs and queues of one minors launch.
 struct MinorsOut {
@@ -833,48 +874,61 @@ struct MinorsOut {
     double* out_ord;
     double* single_tmp;
     int32_t* status;
-    int32_t* fb_list;
+    int32_t* fb_list;   // capacity 4 * count entries
     int32_t* fb_count;
 };
 
-// Global scratch of one minors launch: the repair grid's blocks first, then
-// (orders whose minors exceed LDS) one block per workgroup.  The buffer is
-// owned by the caller (a plan, or the context for the plain form) and only
-// grows; a plan sizes it at creation so that its captured graph never sees a
-// reallocation.
-static size_t minors_scratch_bytes(int n, int blocks) {
-    const size_t lds = fpm_lds_bytes(n);
-    const size_t main_need = lds > kMaxLds ? lds * static_cast<size_t>(blocks) : 0;
-    return main_need + lds * kRepairGrid;
+// Global scratch of one order bucket: the principal minors of every spin
+// matrix (4 per configuration), then the k_fpm and k_cdet workspaces of the
+// global-scratch forms.  The buffer is owned by the caller (a plan, or the
+// context for the plain form) and only grows; a plan sizes it at creation so
+// that its captured graph never sees a reallocation.
+static size_t minors_scratch_bytes(int n, int count, int mode) {
+    const size_t sc = size_t(1) << n;
+    size_t b = sizeof(double) * 4 * static_cast<size_t>(count) * sc;
+    if (fpm_gscr(n)) b += FpmLds::bytes(n) * std::min<size_t>(4 * static_cast<size_t>(count), kGscrGrid);
+    if (cdet_gscr(n, mode)) b += CdetLds::bytes(n, mode) * std::min<size_t>(2 * static_cast<size_t>(count), kGscrGrid);
+    return b;
 }
 
-static int minors_scratch(Scratch& sc, int n, int blocks, double** repair_gs, double** main_gs, size_t* stride) {
-    *stride = MinorsLds::doubles(n);
-    int rc = grow(&sc.ptr, sc.cap, minors_scratch_bytes(n, blocks));
-    if (rc) return rc;
-    *repair_gs = static_cast<double*>(sc.ptr);
-    *main_gs = fpm_lds_bytes(n) > kMaxLds ? *repair_gs + static_cast<size_t>(kRepairGrid) * *stride : nullptr;
-    return 0;
-}
-
-// Launch the minors kernel for one order bucket and its repair pass.
+// Launch the minors and connected-value kernels of one order bucket.
 static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const int32_t* cfg_list, int count,
                          const int64_t* pbase, const int64_t* obase, const double* gup, const double* gdn,
                          const MinorsOut& o, hipStream_t stream) {
     if (count <= 0) return 0;
-    double* gs = nullptr;
-    double* rs = nullptr;
-    size_t stride = 0;
-    int rc = minors_scratch(scratch, n, 2 * count, &rs, &gs, &stride);
+    int rc = grow(&scratch.ptr, scratch.cap, minors_scratch_bytes(n, count, mode));
     if (rc) return rc;
-    const size_t dyn = gs ? 0 : fpm_lds_bytes(n);
+    const size_t sc = size_t(1) << n;
+    double* pm = static_cast<double*>(scratch.ptr);
+    double* next = pm + 4 * static_cast<size_t>(count) * sc;
     const uint16_t* masks = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
-    hipLaunchKernelGGL(k_minors, dim3(2 * count), dim3(256), dyn, stream, ctx->ks, n, mode, cfg_list, pbase, obase,
-                       gup, gdn, o.out, o.out_ord, o.single_tmp, o.status, o.fb_list, o.fb_count, masks, ctx->d_popoff,
-                       gs, stride);
-    hipLaunchKernelGGL(k_minors_repair, dim3(kRepairGrid), dim3(256), 0, stream, ctx->ks, n, mode, cfg_list, pbase,
-                       obase, gup, gdn, o.out, o.out_ord, o.single_tmp, o.status, o.fb_list, o.fb_count,
-                       ctx->d_fallbacks, masks, ctx->d_popoff, rs, stride);
+    const int* popoff = ctx->d_popoff + n * (kMaxOrder + 2);
+    if (n >= 2) {
+        const int items = 4 * count;
+        if (fpm_gscr(n)) {
+            const int grid = std::min(items, kGscrGrid);
+            hipLaunchKernelGGL(k_fpm<true>, dim3(grid), dim3(256), 0, stream, ctx->ks, n, items, cfg_list, pbase, gup,
+                               gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
+            next += static_cast<size_t>(grid) * FpmLds::doubles(n);
+        } else {
+            hipLaunchKernelGGL(k_fpm<false>, dim3(items), dim3(256), FpmLds::bytes(n), stream, ctx->ks, n, items,
+                               cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
+        }
+        const size_t det_lds = sizeof(double) * (kDetThreads + 1) * static_cast<size_t>(n) * n;
+        hipLaunchKernelGGL(k_fallback_dets, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n, cfg_list,
+                           pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+    }
+    const int items = 2 * count;
+    if (cdet_gscr(n, mode)) {
+        const int grid = std::min(items, kGscrGrid);
+        hipLaunchKernelGGL(k_cdet<true>, dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items, cfg_list, pbase,
+                           obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks, popoff, next,
+                           CdetLds::doubles(n, mode));
+    } else {
+        hipLaunchKernelGGL(k_cdet<false>, dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks, n, mode,
+                           items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
+                           masks, popoff, nullptr, 0);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1013,13 +1067,17 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
     HIP_TRY(hipMalloc(&ctx->d_status, sizeof(int32_t)));
     HIP_TRY(hipMalloc(&ctx->d_out_ord, sizeof(double) << Mx));
     HIP_TRY(hipMalloc(&ctx->d_single, sizeof(double) * 4));
-    HIP_TRY(hipMalloc(&ctx->d_small_i32, sizeof(int32_t) * 4));
+    HIP_TRY(hipMalloc(&ctx->d_small_i32, sizeof(int32_t) * 8));
     HIP_TRY(hipMalloc(&ctx->d_one, sizeof(int32_t)));
     HIP_TRY(hipMalloc(&ctx->d_zero64, sizeof(int64_t)));
     HIP_TRY(hipMemset(ctx->d_one, 0, sizeof(int32_t)));
     HIP_TRY(hipMemset(ctx->d_zero64, 0, sizeof(int64_t)));
     // Allow the order-bucketed kernel the full LDS of a CU.
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_minors),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     refresh_scalars(ctx);
     *out_ctx = ctx;
@@ -1149,11 +1207,11 @@ static int plain_eval(cdet_ctx* ctx, const cdet_vertex* v, int n, int mode, doub
     if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(ctx->d_verts, v, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->d_small_i32, static_cast<int64_t>(4), 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->d_small_i32, static_cast<int64_t>(8), 0);
     hipLaunchKernelGGL(k_pairs_plain, dim3(2 * n * n), dim3(64), 0, s, ctx->ks, ctx->d_verts, n, ctx->d_gup,
                        ctx->d_gdn);
     HIP_TRY(hipGetLastError());
-    // d_small_i32: [0] status, [1] queue length, [2..3] queue
+    // d_small_i32: [0] status, [1] queue length, [2..5] queue
     MinorsOut o{ctx->d_out, ctx->d_out_ord, ctx->d_single, ctx->d_small_i32, ctx->d_small_i32 + 2,
                 ctx->d_small_i32 + 1};
     int rc = launch_minors(ctx, ctx->plain_scratch, n, mode, ctx->d_one, 1, ctx->d_zero64, ctx->d_zero64, ctx->d_gup,

@@ -79,7 +79,7 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
         const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
         if (!cnt) continue;
         const int off = p->bucket_off[static_cast<size_t>(n)];
-        MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 2 * off, p->d_fb_count + n};
+        MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n};
         int rc = launch_minors(ctx, p->scratch, n, p->eval_kind, p->d_lists + off, cnt, p->d_pbase, p->d_obase,
                                p->d_gup, p->d_gdn, o, s);
         if (rc) return rc;
@@ -216,7 +216,7 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     rc |= dev_alloc(&p->d_sdest, p->nq);
     rc |= dev_alloc(&p->d_out_ord, p->eval_kind == 1 ? 1 : p->out_total);
     rc |= dev_alloc(&p->d_single, p->eval_kind == 1 ? 4 * count : 1);
-    rc |= dev_alloc(&p->d_fb_list, 2 * count);
+    rc |= dev_alloc(&p->d_fb_list, 4 * count);
     rc |= dev_alloc(&p->d_fb_count, kMaxOrder + 1);
     if (rc) {
         plan_free(p);
@@ -237,12 +237,13 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     int64_t grid = std::min<int64_t>(std::max<int64_t>(wgs, 1), 256 * 8);
     grid = ((grid + 7) / 8) * 8;
     p->contract_grid = static_cast<int>(grid);
-    // Global scratch (orders whose minors exceed LDS, and the repair grid),
-    // sized now so that runs never allocate and can be graph-captured.
+    // Global scratch (minors of every spin matrix, workspaces of large
+    // orders), sized now so that runs never allocate and can be graph-captured.
     size_t scratch_need = 0;
     for (int n = 1; n <= kMaxOrder; ++n)
         if (p->bucket_cnt[static_cast<size_t>(n)])
-            scratch_need = std::max(scratch_need, minors_scratch_bytes(n, 2 * p->bucket_cnt[static_cast<size_t>(n)]));
+            scratch_need = std::max(scratch_need,
+                                    minors_scratch_bytes(n, p->bucket_cnt[static_cast<size_t>(n)], p->eval_kind));
     rc = grow(&p->scratch.ptr, p->scratch.cap, scratch_need);
     if (rc) {
         plan_free(p);

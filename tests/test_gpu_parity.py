@@ -219,6 +219,20 @@ def test_fpm_fallback_path_bit_identical(gpu_available):
     assert orc.fpm_fallbacks > 0
 
 
+def test_batched_fallback_repair_bit_identical(gpu_available):
+    # Batched form of the fallback: matrices whose Schur pivot trips TOL_FPM
+    # are queued and redone with pivoted determinants, in LDS up to order 11
+    # and in global scratch above.
+    ctx = gpu_ctx("c1", tol_fpm=0.9)
+    orc = oracle_for("c1", tol_fpm=0.9)
+    configs = helpers.seeded_configs(7100, [3, 5, 6, 8, 11, 12, 9, 11])
+    say("batched fallback repair")
+    (out,), ob, status = run_batch(ctx, configs, "exact")
+    assert np.count_nonzero(status & 2) > 0
+    for k, v in enumerate(configs):
+        assert bits_equal(out[ob[k]:ob[k + 1]], orc.all_subsets(v)), k
+
+
 def test_high_orders_global_scratch_bit_identical(gpu_available):
     ctx = gpu_ctx("c1")
     orc = helpers.oracle_of("c1")

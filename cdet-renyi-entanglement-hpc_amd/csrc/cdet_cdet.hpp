@@ -14,7 +14,7 @@ namespace cdet {
 // read eight terms at a time, one chunk ahead of the dependent subtractions,
 // so that LDS latency overlaps the serial chain; the arithmetic order is
 // unchanged.
-__device__ inline double connected_chain(const double* a, const double* dp, uint32_t mask, uint32_t anchor,
+__device__ __forceinline__ double connected_chain(const double* a, const double* dp, uint32_t mask, uint32_t anchor,
                                          uint32_t rest, int terms, double thr, bool filter, double& rhs_out) {
     double acc = a[mask];
     double rhs = fabs(acc);
@@ -81,59 +81,136 @@ __device__ inline double connected_chain(const double* a, const double* dp, uint
     return acc;
 }
 
-// All-subset recursion (include/cdet/integrand_replica_randomly_sampled.h:292-347).
-// On entry a[mask] = pm_up * pm_down (a[0] = 0); on exit dp[mask] holds the
-// connected value.  masks_by_pop lists every mask ordered by popcount with
-// pop_off[k] the first index of popcount k; masks of one popcount are
-// independent and run in parallel.
-// With rhs non-null, rhs[mask] receives the sum of |a[mask]| and |terms| of
-// the mask's chain (the scale against which the round-off filter judges the
-// connected value, :330-341).
-__device__ inline void ztransform_all_block(const double* a, double* dp, int n, double thr, bool filter,
-                                            const uint16_t* masks_by_pop, const int* pop_off, double* rhs = nullptr) {
-    const int tid = threadIdx.x;
-    const int nt = blockDim.x;
-    const uint32_t sc = 1u << n;
-    for (uint32_t m = tid; m < sc; m += nt) dp[m] = (m != 0 && (m & (m - 1)) == 0) ? a[m] : 0.0;
-    __syncthreads();
-    for (int bits = 2; bits <= n; ++bits) {
-        const int beg = pop_off[bits];
-        const int end = pop_off[bits + 1];
-        const int terms = (1 << (bits - 1)) - 1;
-        for (int it = beg + tid; it < end; it += nt) {
-            const uint32_t mask = masks_by_pop[it];
-            const uint32_t anchor = mask & (~mask + 1u);
-            double r;
-            dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, terms, thr, filter, r);
-            if (rhs) rhs[mask] = r;
-        }
-        __syncthreads();
-    }
-}
+constexpr int kFoldSteps = 2;  // steps per exchange of the wave-wide chains
 
-// Single-anchor recursion (include/cdet/integrand_replica_randomly_sampled.h:226-290):
-// only masks containing bit 0 recurse; returns dp[full] (valid on all threads).
-__device__ inline double ztransform_single_block(const double* a, double* dp, int n, double thr, bool filter,
-                                                 const uint16_t* masks_by_pop, const int* pop_off) {
+// Whether a popcount layer of M masks with chains of R - 1 terms runs as
+// wave-wide chains: only when every mask can have a wave of its own and the
+// chains are long enough that one lane per mask would leave the wave idle.
+__device__ inline bool layer_wave_chains(int M, int R, int waves) { return M <= waves && R >= 128; }
+
+// Subset recursion over all popcount layers, workgroup-cooperative.
+//   all-subset (single = false; include/cdet/integrand_replica_randomly_sampled.h:292-347):
+//     every mask recurses with anchor = its lowest bit;
+//   single-anchor (single = true; :226-290): only masks containing bit 0
+//     recurse (anchor = bit 0), the others keep dp = a.
+// On entry a[mask] = pm_up * pm_down (a[0] = 0); on exit dp[mask] holds the
+// connected value and, with rhs non-null, rhs[mask] the sum of |a[mask]| and
+// |terms| of its chain (the round-off filter's scale, :330-341).  masks_by_pop
+// lists every mask ordered by popcount, pop_off[k] the first index of
+// popcount k; masks of one popcount are independent.
+//
+// A layer with many masks runs one chain per lane (connected_chain).  A layer
+// with at most one mask per wave and long chains (the top layers) runs each
+// chain on a whole wave: the chain's terms are enumerated as x = 64 X + l
+// (descending x is descending X, then descending l), lane l holding s_lo =
+// the l-th subset of the lowest six bits of the rest and stepping the high
+// part s_hi = (s_hi - 1) & rest_hi; every kFoldSteps steps the lanes' products
+// go to tb[] and lane 0 folds them in descending order, so the arithmetic
+// sequence is the reference's.  The one excluded subset (s = rest, the mask
+// itself) is the first term of the first step and enters as 0, which leaves
+// acc and rhs unchanged.  tb holds kFoldSteps doubles per thread.
+__device__ __forceinline__ void ztransform_layers(const double* a, double* dp, int n, double thr, bool filter,
+                                         const uint16_t* masks_by_pop, const int* pop_off, double* rhs, bool single,
+                                         double* tb) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
     for (uint32_t m = tid; m < sc; m += nt) dp[m] = (m != 0 && (m & (m - 1)) == 0) ? a[m] : 0.0;
     __syncthreads();
+    const int lane = tid & 63;
     for (int bits = 2; bits <= n; ++bits) {
         const int beg = pop_off[bits];
         const int end = pop_off[bits + 1];
-        const int terms = (1 << (bits - 1)) - 1;
-        for (int it = beg + tid; it < end; it += nt) {
-            const uint32_t mask = masks_by_pop[it];
-            double r;
-            dp[mask] = (mask & 1u) ? connected_chain(a, dp, mask, 1u, mask ^ 1u, terms, thr, filter, r) : a[mask];
+        const int R = 1 << (bits - 1);
+        if (!layer_wave_chains(end - beg, R, nt >> 6)) {
+            for (int it = beg + tid; it < end; it += nt) {
+                const uint32_t mask = masks_by_pop[it];
+                if (single && !(mask & 1u)) {
+                    dp[mask] = a[mask];
+                    continue;
+                }
+                const uint32_t anchor = mask & (~mask + 1u);
+                double r;
+                dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, R - 1, thr, filter, r);
+                if (rhs) rhs[mask] = r;
+            }
+        } else {
+            // one mask per wave; lane l holds s_lo = the l-th subset of the
+            // lowest six bits of the rest, steps walk the high bits downwards
+            const int it = beg + (tid >> 6);
+            const bool have = it < end;
+            const uint32_t mask = have ? masks_by_pop[it] : 1u;
+            const bool rec = have && (!single || (mask & 1u));
+            const uint32_t anchor = mask & (~mask + 1u);
+            const uint32_t rest = mask ^ anchor;
+            uint32_t rl = 0, rr = rest;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const uint32_t b = rr & (~rr + 1u);
+                rl |= b;
+                rr ^= b;
+            }
+            const uint32_t rh = rest ^ rl;
+            uint32_t slo = 0;
+            {
+                uint32_t bl = rl;
+                int v = lane;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const uint32_t b = bl & (~bl + 1u);
+                    if (v & 1) slo |= b;
+                    v >>= 1;
+                    bl ^= b;
+                }
+            }
+            const uint32_t base_lo = anchor | slo;
+            uint32_t shi = rh;
+            double acc = rec ? a[mask] : 0.0;
+            double rs = fabs(acc);
+            const int steps = R >> 6;
+            double* wtb = tb + (tid & ~63);
+            for (int st0 = 0; st0 < steps; st0 += kFoldSteps) {
+                // every index below is a subset of the mask, so the loads are
+                // in bounds for all lanes; invalid terms are replaced by 0
+#pragma unroll
+                for (int u = 0; u < kFoldSteps; ++u) {
+                    const int st = st0 + u;
+                    const uint32_t sub = base_lo | shi;
+                    const double prod = dp[sub] * a[mask ^ sub];
+                    const bool valid = rec && st < steps && !(st == 0 && lane == 63);
+                    wtb[u * nt + lane] = valid ? prod : 0.0;
+                    shi = (shi - 1u) & rh;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (rec && lane == 0) {
+#pragma unroll
+                    for (int u = 0; u < kFoldSteps; ++u) {
+#pragma unroll
+                        for (int q = 63; q >= 0; --q) {
+                            const double v = wtb[u * nt + q];
+                            acc = acc - v;
+                            rs += fabs(v);
+                        }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (have && lane == 0) {
+                if (rec) {
+                    if (filter && fabs(acc) < thr * rs) acc = 0.0;
+                    dp[mask] = acc;
+                    if (rhs) rhs[mask] = rs;
+                } else {
+                    dp[mask] = a[mask];
+                }
+            }
         }
         __syncthreads();
     }
-    const double r = dp[sc - 1];
-    __syncthreads();
-    return r;
 }
 
 }  // namespace cdet

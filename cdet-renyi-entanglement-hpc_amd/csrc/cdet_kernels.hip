@@ -425,11 +425,11 @@ __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32
 // Order-bucketed principal minors and connected values, two kernels per
 // order bucket (list index li = position in the bucket's configuration list):
 //
-// k_fpm   one workgroup per spin matrix, work item w = 4 li + 2 class + spin
-//         (class 0 = twisted, rc = 2; 1 = ordinary, rc = 1).  Builds the
-//         vertex matrix (include/cdet/integrand_replica_randomly_sampled.h:751-761),
-//         runs the Schur-complement tree and writes the principal minors to
-//         pm[w 2^n + mask].  A matrix whose pivot needs the reference's
+// k_fpm   one workgroup per spin matrix (class 0 = twisted, rc = 2; 1 =
+//         ordinary, rc = 1).  Builds the vertex matrix
+//         (include/cdet/integrand_replica_randomly_sampled.h:751-761), runs
+//         the Schur-complement tree and writes the principal minors to
+//         pm[(4 li + 2 class + spin) 2^n + mask].  A matrix whose pivot needs the reference's
 //         fallback is queued instead (entry = (li << 3) | (class << 2) | spin
 //         mask) and k_fallback_dets writes its pivoted determinants to the same
 //         place.  Identical up and down tables (same_spin) use spin 0 for both.
@@ -480,10 +480,13 @@ __global__ __launch_bounds__(256) void k_fpm(KScalars ks, int n, int items, cons
     double* bufB = bufA + list;
     double* inv = bufB + list;
     double* red = inv + (n >= 2 ? (1 << (n - 2)) : 1);
+    // Work items: 2 (li, class) pairs x 2 spins, or x 1 when both classes have
+    // identical spin tables.
+    const bool both_same = ks.same_spin[0] && ks.same_spin[1];
     for (int w = blockIdx.x; w < items; w += gridDim.x) {
-        const int li = w >> 2;
-        const int cls = (w >> 1) & 1;
-        const int spin = w & 1;
+        const int li = both_same ? (w >> 1) : (w >> 2);
+        const int cls = both_same ? (w & 1) : ((w >> 1) & 1);
+        const int spin = both_same ? 0 : (w & 1);
         const bool same = ks.same_spin[cls] != 0;
         if (spin == 1 && same) continue;  // uniform across the workgroup
         const int c = cfg_list[li];
@@ -502,7 +505,7 @@ __global__ __launch_bounds__(256) void k_fpm(KScalars ks, int n, int items, cons
                 fb_list[slot] = (li << 3) | (cls << 2) | (same ? 3 : (1 << spin));
             }
         } else {
-            double* dst = pm + static_cast<size_t>(w) * sc;
+            double* dst = pm + static_cast<size_t>(4 * li + 2 * cls + spin) * sc;
             for (uint32_t m = tid; m < sc; m += nt) dst[m] = fo[m];
         }
         __syncthreads();
@@ -575,6 +578,7 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
                                               const int* __restrict__ pop_off, double* __restrict__ gscratch,
                                               size_t gscratch_stride) {
     extern __shared__ double lds_dyn[];
+    __shared__ double tb[kFoldSteps * 256];  // wave-wide chain terms of the subset recursion
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
@@ -616,7 +620,7 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
         if (mode == 0 || mode == 2) {
             // mode 2 writes the magnitude scale of each value instead of the value:
             // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
-            ztransform_all_block(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off, mode == 2 ? rhs : nullptr);
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off, mode == 2 ? rhs : nullptr, false, tb);
             // Assembly (:833-877).
             double* dst = (cls == 0 ? out : out_ord) + cfg_obase[c];
             bool bad = false;
@@ -637,7 +641,8 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
             }
             if (bad && status) atomicOr(&status[c], 1);
         } else {
-            const double cres = ztransform_single_block(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off);
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off, nullptr, true, tb);
+            const double cres = dp[sc - 1];
             if (tid == 0) {
                 const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
                 single_tmp[4 * c + 2 * cls] = v.hi;
@@ -866,7 +871,9 @@ static constexpr size_t kMaxLds = 160 * 1024 - 2048;
 static constexpr int kDetGrid = 512;
 static constexpr int kGscrGrid = 1024;  // workgroups of the global-scratch forms
 static bool fpm_gscr(int n) { return FpmLds::bytes(n) > kMaxLds; }
-static bool cdet_gscr(int n, int mode) { return CdetLds::bytes(n, mode) > kMaxLds; }
+// k_cdet's static LDS (wave-wide chain terms) comes out of its dynamic budget.
+static constexpr size_t kCdetMaxDyn = kMaxLds - sizeof(double) * kFoldSteps * 256;
+static bool cdet_gscr(int n, int mode) { return CdetLds::bytes(n, mode) > kCdetMaxDyn; }
 
 // Outputs and queues of one minors launch.
 struct MinorsOut {
@@ -904,7 +911,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     const uint16_t* masks = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
     const int* popoff = ctx->d_popoff + n * (kMaxOrder + 2);
     if (n >= 2) {
-        const int items = 4 * count;
+        const int items = (ctx->ks.same_spin[0] && ctx->ks.same_spin[1] ? 2 : 4) * count;
         if (fpm_gscr(n)) {
             const int grid = std::min(items, kGscrGrid);
             hipLaunchKernelGGL(k_fpm<true>, dim3(grid), dim3(256), 0, stream, ctx->ks, n, items, cfg_list, pbase, gup,
@@ -1076,7 +1083,7 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     refresh_scalars(ctx);

@@ -21,6 +21,9 @@ FLAGS = [
     "-ffp-contract=off",
     # Keep packed FP32 instructions out of the code object.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: the default AGPR form copied every loop-
+    # carried accumulator through v_accvgpr moves each K group.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
 ]
 

@@ -387,7 +387,7 @@ __device__ inline double cheb2d_exact(const double* __restrict__ c, int nu, int 
 // Exact batched contraction: one wave per tile of <= 64 queries sharing a
 // coefficient block; lane = query.  The block address is wave-uniform, so the
 // coefficients stream through the scalar cache and are broadcast to all
-// lanes.
+// lanes.  Tiles are split evenly over the XCDs in contiguous ranges.
 __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32_t* __restrict__ n_tiles,
                                                         const int32_t* __restrict__ tile_key,
                                                         const int32_t* __restrict__ tile_start,
@@ -398,8 +398,8 @@ __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int waves_per_block = blockDim.x >> 6;
-    const int total = *n_tiles;
-    for (int t = blockIdx.x * waves_per_block + wave; t < total; t += gridDim.x * waves_per_block) {
+    const TileRange tr = xcd_tiles(*n_tiles, wave, waves_per_block);
+    for (int t = tr.first; t < tr.end; t += tr.stride) {
         const int tile = __builtin_amdgcn_readfirstlane(t);
         const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
         const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);

@@ -61,7 +61,7 @@ struct BasisK {  // lane's T_k(v) along K with k = 4s + g
         w = 2.0 * t4;
     }
     __device__ inline void step() {
-        const double nx = w * cur - prev;
+        const double nx = __builtin_fma(w, cur, -prev);
         prev = cur;
         cur = nx;
     }
@@ -94,20 +94,28 @@ struct BasisA {  // lane's T_a(u) for its four rows a = 16 m + row_in_slab(g, r)
     __device__ inline void step() {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const double nx = w * cur[r] - prev[r];
+            const double nx = __builtin_fma(w, cur[r], -prev[r]);
             prev[r] = cur[r];
             cur[r] = nx;
         }
     }
 };
 
-__device__ inline int xcd_remap(int b, int grid) {
-    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
-    // contiguous range of logical blocks so tiles of one coefficient block
-    // share that XCD's L2.
-    const int per = grid / 8;
-    if (per == 0 || grid % 8 != 0) return b;
-    return (b % 8) * per + b / 8;
+// Work split for the persistent contraction kernels.  Workgroups are dealt
+// round-robin to the 8 XCDs (workgroup b runs on XCD b % 8), so the tiles are
+// cut into 8 equal contiguous ranges, one per XCD: every XCD gets the same
+// share and the tiles of one coefficient block (consecutive) share an L2.
+// Within its XCD a wave strides over the range.
+struct TileRange {
+    int first, end, stride;
+};
+__device__ inline TileRange xcd_tiles(int total, int wave, int wpb) {
+    const int grid = gridDim.x;
+    if (grid % 8 != 0) return TileRange{static_cast<int>(blockIdx.x) * wpb + wave, total, grid * wpb};
+    const int x = blockIdx.x & 7;
+    const int chunk = (total + 7) >> 3;
+    const int end = min(total, (x + 1) * chunk);
+    return TileRange{x * chunk + static_cast<int>(blockIdx.x >> 3) * wpb + wave, end, (grid >> 3) * wpb};
 }
 
 template <int MODE>
@@ -123,12 +131,11 @@ __global__ __launch_bounds__(256) void k_contract_mfma(KScalars ks, const int32_
     const int j = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
-    const int total = *n_tiles;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const TileRange tr = xcd_tiles(*n_tiles, wave, wpb);
     const int nu = ks.nu, nv = ks.nv;
     const int mslabs = (nu + 15) >> 4;
     const int ksteps = (nv + 3) >> 2;
-    for (int t = lb * wpb + wave; t < total; t += gridDim.x * wpb) {
+    for (int t = tr.first; t < tr.end; t += tr.stride) {
         const int tile = __builtin_amdgcn_readfirstlane(t);
         const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
         const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);
@@ -295,34 +302,47 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
             accd[nt] = acc[nt];
             Bv[nt] = B0[nt];
         }
-        double a[4], b[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            a[r] = pu[4 * r];
-            if (!SAME) b[r] = pd[4 * r];
-        }
-        for (int t = 0; t < groups; ++t) {
-            const int tn = 16 * ((t + 1 < groups) ? t + 1 : t);
-            double an[4], bn[4];
+        // Two register sets: the coefficients of group t + 1 are in flight
+        // while the MFMAs of group t run.
+        double a0[4], b0[4], a1[4], b1[4];
+        auto load = [&](int t, double* x, double* y) {
+            const int tt = 16 * t;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                an[r] = pu[tn + 4 * r];
-                if (!SAME) bn[r] = pd[tn + 4 * r];
+                x[r] = pu[tt + 4 * r];
+                if (!SAME) y[r] = pd[tt + 4 * r];
             }
+        };
+        auto group = [&](const double* x, const double* y) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
-                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], Bv[nt].cur, acc[nt], 0, 0, 0);
-                    if (!SAME) accd[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[r], Bv[nt].cur, accd[nt], 0, 0, 0);
+                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[r], Bv[nt].cur, acc[nt], 0, 0, 0);
+                    if (!SAME) accd[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[r], Bv[nt].cur, accd[nt], 0, 0, 0);
                     Bv[nt].step();
                 }
             }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                a[r] = an[r];
-                if (!SAME) b[r] = bn[r];
-            }
+        };
+        // Loads are unconditional and fenced from the scheduler so that they
+        // stay one group ahead of the MFMAs that consume them.
+        load(0, a0, b0);
+        int t = 0;
+        for (; t + 3 <= groups; t += 2) {
+            load(t + 1, a1, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            group(a0, b0);
+            load(t + 2, a0, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            group(a1, b1);
+        }
+        if (groups - t == 2) {
+            load(t + 1, a1, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            group(a0, b0);
+            group(a1, b1);
+        } else {
+            group(a0, b0);
         }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
@@ -370,9 +390,8 @@ __global__ __launch_bounds__(256) void k_contract_f64(KScalars ks, const int32_t
     const int j = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
-    const int total = *n_tiles;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    for (int t0 = lb * wpb + wave; t0 < total; t0 += gridDim.x * wpb) {
+    const TileRange tr = xcd_tiles(*n_tiles, wave, wpb);
+    for (int t0 = tr.first; t0 < tr.end; t0 += tr.stride) {
         const int tile = __builtin_amdgcn_readfirstlane(t0);
         const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
         const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);

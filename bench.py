@@ -17,10 +17,9 @@ Printed JSON line fields beyond the driver contract:
   cpu_baseline  the oracle (CPU restatement of the reference) on the host
                 cores, bounded sample of the same order-10 configurations
   plain_fp64    one-configuration-per-launch FP64 GPU form, evals/s
-  errors        against the bit-identical FP64 form: max relative error on
-                masks >= 1e-6 of the configuration's largest value, on
-                well-conditioned masks (sum of |terms| <= 1e4 |value|), and
-                max |error| / sum of |terms| over all masks
+  errors        against the bit-identical FP64 form (error_metrics), beside
+                the same figures for that FP64 form on tables moved by one
+                ulp (the spread any FP64 evaluation order shows)
 """
 import argparse
 import json
@@ -55,6 +54,8 @@ def parse():
     ap.add_argument("--plain-evals", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
+    ap.add_argument("--no-ulp", action="store_true",
+                    help="skip the FP64 one-ulp table perturbation reference for the error report")
     return ap.parse_args()
 
 
@@ -66,6 +67,56 @@ def traffic_from_profiles(kernel):
             return json.load(f).get(kernel)
     except (OSError, ValueError):
         return None
+
+
+def error_metrics(got, ref, cond):
+    """Errors of `got` against the bit-identical FP64 values `ref` over the
+    masks >= 1e-6 of their configuration's largest |value| (smaller values sit
+    at the round-off floor of the principal minors): max relative error, the
+    same on well-conditioned masks (sum of |terms| <= 1e4 |value|), and max
+    |error| / sum of |terms|."""
+    scale = np.maximum(np.abs(ref), 1e-300)
+    rel = np.abs(got - ref) / scale
+    big = np.abs(ref) >= 1e-6 * np.abs(ref).max(axis=1, keepdims=True)
+    use = big & (cond > 0)
+    well = use & (cond <= 1e4 * np.abs(ref))
+    return {
+        "max_rel_err": float(np.max(rel[big])) if np.any(big) else 0.0,
+        "max_rel_err_well_conditioned": float(np.max(rel[well])) if np.any(well) else 0.0,
+        "well_conditioned_fraction": round(float(np.count_nonzero(well) / max(1, np.count_nonzero(use))), 4),
+        "max_err_over_conditioning": float(np.max(np.abs(got - ref)[use] / cond[use])) if np.any(use) else 0.0,
+    }
+
+
+def ulp_reference(ctx, params, d_verts, count, order, ref, cond, dev, sptr):
+    """The same error metrics for the bit-identical FP64 path evaluated on the
+    tables with every coefficient moved by one ulp (seeded random direction):
+    the spread any FP64 evaluation order of these tables can show."""
+    import torch
+    import cdet_amd
+    r = np.random.default_rng(7)
+    pert = []
+    for kind in (0, 2):
+        arrs = ctx.download_table(kind)
+        out = []
+        for a in arrs:
+            d = np.where(r.random(a.shape) < 0.5, -np.inf, np.inf)
+            out.append(np.nextafter(a, d))
+        pert.append(out)
+    class T:
+        def __init__(self, a):
+            self.upper, self.lower, self.ucorner, self.lcorner = a
+    t0, t2 = T(pert[0]), T(pert[1])
+    c2 = cdet_amd.Context(params)
+    c2.upload_tables([t0, t0, t2, t2])
+    plan = c2.plan([order] * count, mode="exact")
+    out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
+    plan.run(d_verts, out, None, use_graph=False, stream=sptr)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(count, -1)
+    plan.close()
+    c2.close()
+    return error_metrics(got, ref, cond)
 
 
 def cpu_baseline(tables, params_gpu, verts, seconds):
@@ -160,17 +211,8 @@ def main():
     # accuracy of this mode against the FP64 exact form
     got = d_out.cpu().numpy().reshape(count, -1)
     ref = d_ref.cpu().numpy().reshape(count, -1)
-    scale = np.maximum(np.abs(ref), 1e-300)
-    rel = np.abs(got - ref) / scale
-    big = np.abs(ref) >= 1e-6 * np.abs(ref).max(axis=1, keepdims=True)
-    max_rel = float(np.max(rel[big])) if np.any(big) else 0.0
     cond = d_cond.cpu().numpy().reshape(count, -1)
-    diff = np.abs(got - ref)
-    pos = cond > 0
-    err_vs_cond = float(np.max(diff[pos] / cond[pos])) if np.any(pos) else 0.0
-    well = pos & (np.abs(ref) > 0) & (cond <= 1e4 * np.abs(ref))
-    max_rel_well = float(np.max(rel[well])) if np.any(well) else 0.0
-    well_frac = float(np.count_nonzero(well) / max(1, np.count_nonzero(pos)))
+    errors = error_metrics(got, ref, cond)
     st = d_status.cpu().numpy()
     status_bad = int(np.count_nonzero(st & 1))
     fallbacks = int(np.count_nonzero(st & 2))
@@ -193,6 +235,8 @@ def main():
                 "flops_per_launch": flops, "kernel_ms": round(stage[1], 4),
                 "stage_ms": {"plan_group": round(stage[0], 4), "contract": round(stage[1], 4),
                              "fpm_cdet": round(stage[2], 4)}}
+
+    ulp = None if args.no_ulp else ulp_reference(ctx, params, d_verts, count, w["order"], ref, cond, dev, sptr)
 
     # plain one-configuration-per-launch FP64 form
     nplain = min(args.plain_evals, count)
@@ -226,10 +270,8 @@ def main():
         "cpu_baseline": cpu,
         "plain_fp64_evals_per_s": round(plain, 1),
         "speedup_vs_plain_fp64": round(value / plain, 2),
-        "max_rel_err_vs_fp64_exact": max_rel,
-        "max_rel_err_well_conditioned": max_rel_well,
-        "well_conditioned_fraction": round(well_frac, 4),
-        "max_err_over_conditioning": err_vs_cond,
+        "errors_vs_fp64_exact": errors,
+        "errors_fp64_one_ulp_tables": ulp,
         "nonfinite_configs": status_bad,
         "fpm_fallback_configs": fallbacks,
     }

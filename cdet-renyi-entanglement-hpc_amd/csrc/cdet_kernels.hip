@@ -113,16 +113,22 @@ __global__ __launch_bounds__(64) void k_pairs_plain(KScalars ks, const double* _
 // Batched planning: one workgroup per configuration.
 //   cfg_order[c], cfg_vbase[c] (first vertex), cfg_pbase[c] (sum of n^2 before c)
 // Query slot of pair t (t < 2 n^2) is 2*pbase + t; G slot is the same index.
+// Sub-counters per coefficient block in the lookup histogram.
+constexpr int kKeySub = 16;
+
 __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restrict__ verts,
                                               const int32_t* __restrict__ cfg_order,
                                               const int64_t* __restrict__ cfg_vbase,
                                               const int64_t* __restrict__ cfg_pbase, double* __restrict__ g_up,
                                               double* __restrict__ g_dn, int32_t* __restrict__ q_key,
                                               int32_t* __restrict__ q_rank, double* __restrict__ q_u,
-                                              double* __restrict__ q_v, int32_t* __restrict__ hist) {
+                                              double* __restrict__ q_v, int32_t* __restrict__ hist,
+                                              int32_t* __restrict__ status) {
     __shared__ Vtx vs[2][kMaxOrder];
     const int c = blockIdx.x;
     const int n = cfg_order[c];
+    if (status && threadIdx.x == 0) status[c] = 0;
+    const int sub = c & (kKeySub - 1);
     const int64_t vb = cfg_vbase[c];
     const int64_t qb = 2 * cfg_pbase[c];
     if (threadIdx.x < 2 * n) {
@@ -151,8 +157,9 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
             continue;
         }
         const int32_t key = static_cast<int32_t>((static_cast<int64_t>(cls) * 2 + br) * ks.SP + d.sitepair);
-        q_key[q] = key;
-        q_rank[q] = atomicAdd(&hist[key], 1);
+        const int32_t slot = key * kKeySub + sub;
+        q_key[q] = slot;
+        q_rank[q] = atomicAdd(&hist[slot], 1);
         q_u[q] = d.u;
         q_v[q] = d.v;
     }
@@ -191,8 +198,9 @@ __global__ __launch_bounds__(256) void k_plan_queries(KScalars ks, int64_t nq, c
         return;
     }
     const int32_t key = static_cast<int32_t>((static_cast<int64_t>(cls) * 2 + br) * ks.SP + d.sitepair);
-    q_key[q] = key;
-    q_rank[q] = atomicAdd(&hist[key], 1);
+    const int32_t slot = key * kKeySub + (blockIdx.x & (kKeySub - 1));
+    q_key[q] = slot;
+    q_rank[q] = atomicAdd(&hist[slot], 1);
     q_u[q] = d.u;
     q_v[q] = d.v;
 }
@@ -206,51 +214,134 @@ __global__ void k_interleave(int64_t n, const double* __restrict__ a, const doub
 }
 
 // Exclusive scan of the per-block histogram and tile list, one workgroup.
-__global__ __launch_bounds__(1024) void k_scan_keys(const int32_t* __restrict__ hist, int32_t nkeys, int32_t tile,
-                                                    int32_t* __restrict__ key_off, int32_t* __restrict__ tile_key,
-                                                    int32_t* __restrict__ tile_start, int32_t* __restrict__ tile_len,
-                                                    int32_t* __restrict__ n_tiles) {
-    __shared__ int32_t s_q[1024];
-    __shared__ int32_t s_t[1024];
-    const int tid = threadIdx.x;
-    const int nt = blockDim.x;
-    const int chunk = (nkeys + nt - 1) / nt;
-    const int k0 = min(nkeys, tid * chunk);
-    const int k1 = min(nkeys, k0 + chunk);
-    int sq = 0, st = 0;
-    for (int k = k0; k < k1; ++k) {
-        const int h = hist[k];
-        sq += h;
-        st += (h + tile - 1) / tile;
-    }
-    s_q[tid] = sq;
-    s_t[tid] = st;
-    __syncthreads();
-    for (int off = 1; off < nt; off <<= 1) {
-        int aq = 0, at = 0;
-        if (tid >= off) {
-            aq = s_q[tid - off];
-            at = s_t[tid - off];
+// Offsets of the lookup groups and the tile list.  Each coefficient block
+// (key) counts its lookups in kKeySub sub-counters (hist[key * kKeySub + sub])
+// so that the atomics of the lookup planners spread over kKeySub addresses;
+// the sub-groups of a key are laid out consecutively, so a key's lookups stay
+// contiguous.  Tiles of <= tile lookups are cut per key.  The histogram is
+// cleared after use (it starts cleared at allocation), and the per-XCD work
+// counters are cleared for the contraction.
+// Exclusive scan of (a, b) pairs over the workgroup: wave shuffles, then
+// the wave totals.  Returns the workgroup totals.
+__device__ inline int2 block_scan2(int a, int b, int& ea, int& eb, int2* s_w) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    int ia = a, ib = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int ta = __shfl_up(ia, off);
+        const int tb = __shfl_up(ib, off);
+        if (lane >= off) {
+            ia += ta;
+            ib += tb;
         }
-        __syncthreads();
-        s_q[tid] += aq;
-        s_t[tid] += at;
-        __syncthreads();
     }
-    int oq = s_q[tid] - sq;
-    int ot = s_t[tid] - st;
-    for (int k = k0; k < k1; ++k) {
-        const int h = hist[k];
-        key_off[k] = oq;
+    if (lane == 63) s_w[w] = make_int2(ia, ib);
+    __syncthreads();
+    int pa = 0, pb = 0;
+    int2 tot = make_int2(0, 0);
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; ++k) {
+        const int2 v = s_w[k];
+        if (k < w) {
+            pa += v.x;
+            pb += v.y;
+        }
+        tot.x += v.x;
+        tot.y += v.y;
+    }
+    __syncthreads();
+    ea = pa + ia - a;
+    eb = pb + ib - b;
+    return tot;
+}
+
+// Lookups of key k: the sum of its kKeySub sub-counters (read as int4).
+__device__ inline int key_count(const int32_t* hist, int k, int4* hv) {
+    const int4* src = reinterpret_cast<const int4*>(hist + static_cast<int64_t>(k) * kKeySub);
+    int h = 0;
+#pragma unroll
+    for (int u = 0; u < kKeySub / 4; ++u) {
+        hv[u] = src[u];
+        h += hv[u].x + hv[u].y + hv[u].z + hv[u].w;
+    }
+    return h;
+}
+
+// Grouping, pass 1: per workgroup of kScanKeys keys, the total lookups and
+// tiles (wg_tot[blockIdx]).
+constexpr int kScanKeys = 256;
+__global__ __launch_bounds__(kScanKeys) void k_key_totals(const int32_t* __restrict__ hist, int32_t nkeys,
+                                                          int32_t tile, int2* __restrict__ wg_tot) {
+    static_assert(kKeySub % 4 == 0, "sub-counters are read as int4");
+    __shared__ int2 s_w[kScanKeys / 64];
+    const int k = blockIdx.x * kScanKeys + threadIdx.x;
+    int4 hv[kKeySub / 4];
+    const int h = k < nkeys ? key_count(hist, k, hv) : 0;
+    int eq, et;
+    const int2 tot = block_scan2(h, (h + tile - 1) / tile, eq, et, s_w);
+    if (threadIdx.x == 0) wg_tot[blockIdx.x] = tot;
+}
+
+// Grouping, pass 2: offsets of the lookup groups and the tile list.  Each
+// coefficient block (key) counts its lookups in kKeySub sub-counters
+// (hist[key * kKeySub + sub]) so that the atomics of the lookup planners
+// spread over kKeySub addresses; the sub-groups of a key are laid out
+// consecutively, so a key's lookups stay contiguous.  Tiles of <= tile
+// lookups are cut per key.  The histogram is cleared after use (it starts
+// cleared at allocation).
+__global__ __launch_bounds__(kScanKeys) void k_scan_keys(int32_t* __restrict__ hist, int32_t nkeys, int32_t tile,
+                                                         const int2* __restrict__ wg_tot,
+                                                         int32_t* __restrict__ key_off, int32_t* __restrict__ tile_key,
+                                                         int32_t* __restrict__ tile_start,
+                                                         int32_t* __restrict__ tile_len, int32_t* __restrict__ n_tiles) {
+    __shared__ int2 s_w[kScanKeys / 64];
+    __shared__ int2 s_base;
+    const int tid = threadIdx.x;
+    // offsets of this workgroup: totals of the workgroups before it
+    int bq = 0, bt = 0;
+    for (int w = tid; w < static_cast<int>(blockIdx.x); w += kScanKeys) {
+        const int2 v = wg_tot[w];
+        bq += v.x;
+        bt += v.y;
+    }
+    int eq0, et0;
+    const int2 pre = block_scan2(bq, bt, eq0, et0, s_w);
+    if (tid == 0) s_base = pre;
+    __syncthreads();
+    const int cq = s_base.x, ct = s_base.y;
+    const int k = blockIdx.x * kScanKeys + tid;
+    int4 hv[kKeySub / 4];
+    const int h = k < nkeys ? key_count(hist, k, hv) : 0;
+    int eq, et;
+    const int2 tot = block_scan2(h, (h + tile - 1) / tile, eq, et, s_w);
+    if (k < nkeys) {
+        int off = cq + eq;
+        int4* dst = reinterpret_cast<int4*>(key_off + static_cast<int64_t>(k) * kKeySub);
+        int4* hz = reinterpret_cast<int4*>(hist + static_cast<int64_t>(k) * kKeySub);
+#pragma unroll
+        for (int u = 0; u < kKeySub / 4; ++u) {
+            int4 o;
+            o.x = off;
+            off += hv[u].x;
+            o.y = off;
+            off += hv[u].y;
+            o.z = off;
+            off += hv[u].z;
+            o.w = off;
+            off += hv[u].w;
+            dst[u] = o;
+            hz[u] = make_int4(0, 0, 0, 0);
+        }
+        int ot = ct + et;
         for (int s = 0; s < h; s += tile) {
             tile_key[ot] = k;
-            tile_start[ot] = oq + s;
+            tile_start[ot] = cq + eq + s;
             tile_len[ot] = min(tile, h - s);
             ++ot;
         }
-        oq += h;
     }
-    if (tid == nt - 1) *n_tiles = s_t[tid];
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) *n_tiles = ct + tot.y;
 }
 
 __global__ __launch_bounds__(256) void k_scatter(int64_t nq, const int32_t* __restrict__ q_key,
@@ -576,9 +667,12 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
                                               double* __restrict__ single_tmp, int32_t* __restrict__ status,
                                               const uint16_t* __restrict__ masks_by_pop,
                                               const int* __restrict__ pop_off, double* __restrict__ gscratch,
-                                              size_t gscratch_stride) {
+                                              size_t gscratch_stride, int32_t* __restrict__ fb_count) {
     extern __shared__ double lds_dyn[];
-    __shared__ double tb[kFoldSteps * 256];  // wave-wide chain terms of the subset recursion
+    __shared__ double tb[kFoldSteps * 256];
+    // The bucket's fallback queue has been consumed (k_fallback_dets ran
+    // before); clear its length for the next run.
+    if (fb_count && blockIdx.x == 0 && threadIdx.x == 0) *fb_count = 0;  // wave-wide chain terms of the subset recursion
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
@@ -704,6 +798,7 @@ struct Scratch {
 struct cdet_ctx {
     cdet_params p{};
     int device = 0;
+    int ncu = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     DevTable tab[4];
     KScalars ks{};
@@ -930,11 +1025,11 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         const int grid = std::min(items, kGscrGrid);
         hipLaunchKernelGGL(k_cdet<true>, dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items, cfg_list, pbase,
                            obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks, popoff, next,
-                           CdetLds::doubles(n, mode));
+                           CdetLds::doubles(n, mode), o.fb_count);
     } else {
         hipLaunchKernelGGL(k_cdet<false>, dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks, n, mode,
                            items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
-                           masks, popoff, nullptr, 0);
+                           masks, popoff, nullptr, 0, o.fb_count);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1006,6 +1101,14 @@ static int contract_launch_mfma(const KScalars& ks, int mode, int grid, const in
     return 0;
 }
 
+// Grid of a persistent contraction launch: one wave per tile up to 8
+// workgroups per CU, a multiple of the 8 XCDs.
+static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles) {
+    int64_t g = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), static_cast<int64_t>(ctx->ncu) * 8);
+    g = ((g + 7) / 8) * 8;
+    return static_cast<int>(g);
+}
+
 static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner);
 static int alloc_table(cdet_ctx* ctx, int kind);
 
@@ -1053,6 +1156,9 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
         delete ctx;
         return fail(CDET_ERR_DEVICE, "no HIP device");
     }
+    if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+        ctx->ncu <= 0)
+        ctx->ncu = 256;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return fail(CDET_ERR_DEVICE, "hipStreamCreate failed");

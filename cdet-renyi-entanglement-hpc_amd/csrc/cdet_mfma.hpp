@@ -130,8 +130,7 @@ __global__ __launch_bounds__(256) void k_contract_mfma(KScalars ks, const int32_
     const int g = lane >> 4;
     const int j = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wpb = blockDim.x >> 6;
-    const TileRange tr = xcd_tiles(*n_tiles, wave, wpb);
+    const TileRange tr = xcd_tiles(*n_tiles, wave, blockDim.x >> 6);
     const int nu = ks.nu, nv = ks.nv;
     const int mslabs = (nu + 15) >> 4;
     const int ksteps = (nv + 3) >> 2;
@@ -280,79 +279,89 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
     const int nu = ks.nu, nv = ks.nv;
     const int mslabs = nu >> 4;
     const int groups = nv >> 4;
-    BasisK Au[NT], B0[NT];
+    BasisK Au[NT];
+    double v[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int q = 16 * nt + j;
         const int p = start + (q < len ? q : 0);
-        B0[nt].init(s_v[p], g);
+        v[nt] = s_v[p];
         Au[nt].init(s_u[p], g);
     }
     double G[NT], Gd[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) G[nt] = Gd[nt] = 0.0;
-    for (int ms = 0; ms < mslabs; ++ms) {
-        const double* pu = cu + static_cast<int64_t>(16 * ms + j) * nv + g;
-        const double* pd = cd + static_cast<int64_t>(16 * ms + j) * nv + g;
-        d4 acc[NT], accd[NT];
-        BasisK Bv[NT];
+    // The (slab, group) pairs form one stream of mslabs * groups steps; the
+    // coefficients of later steps are loaded (rotating registers, fenced from
+    // the scheduler) while step k runs, across slab boundaries too.
+    const int total = mslabs * groups;
+    int64_t loff = static_cast<int64_t>(j) * nv + g;  // next load: row 16 m + j, column 16 t + g
+    int lk = 0, lt = 0;
+    auto load_next = [&](double* x, double* y) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            acc[nt] = d4{0.0, 0.0, 0.0, 0.0};
-            accd[nt] = acc[nt];
-            Bv[nt] = B0[nt];
+        for (int r = 0; r < 4; ++r) {
+            x[r] = cu[loff + 4 * r];
+            if (!SAME) y[r] = cd[loff + 4 * r];
         }
-        // Two register sets: the coefficients of group t + 1 are in flight
-        // while the MFMAs of group t run.
-        double a0[4], b0[4], a1[4], b1[4];
-        auto load = [&](int t, double* x, double* y) {
-            const int tt = 16 * t;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                x[r] = pu[tt + 4 * r];
-                if (!SAME) y[r] = pd[tt + 4 * r];
+        if (lk + 1 < total) {  // past the end the last step is read again
+            ++lk;
+            loff += 16;
+            if (++lt == groups) {
+                lt = 0;
+                loff += 15 * static_cast<int64_t>(nv);
             }
-        };
-        auto group = [&](const double* x, const double* y) {
+        }
+    };
+    d4 acc[NT], accd[NT];
+    BasisK Bv[NT];
+    int ct = 0;
+    auto compute = [&](const double* x, const double* y) {
+        if (ct == 0) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int nt = 0; nt < NT; ++nt) {
+                acc[nt] = d4{0.0, 0.0, 0.0, 0.0};
+                accd[nt] = acc[nt];
+                Bv[nt].init(v[nt], g);
+            }
+        }
+        // Four recurrence steps per lane and N tile with the two basis
+        // registers alternating roles (no register moves): steps r = 0, 2 read
+        // cur and overwrite prev with T_{k+4}; r = 1, 3 read prev and
+        // overwrite cur, leaving (cur, prev) = (T_{k+16}, T_{k+12}).
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[r], Bv[nt].cur, acc[nt], 0, 0, 0);
-                    if (!SAME) accd[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[r], Bv[nt].cur, accd[nt], 0, 0, 0);
-                    Bv[nt].step();
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                double& rd = (r & 1) ? Bv[nt].prev : Bv[nt].cur;
+                double& wr = (r & 1) ? Bv[nt].cur : Bv[nt].prev;
+                acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[r], rd, acc[nt], 0, 0, 0);
+                if (!SAME) accd[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[r], rd, accd[nt], 0, 0, 0);
+                wr = __builtin_fma(Bv[nt].w, rd, -wr);
+            }
+        }
+        if (++ct == groups) {  // slab done: rows 16 m + 4 r + g of the accumulators
+            ct = 0;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    G[nt] = __builtin_fma(Au[nt].cur, acc[nt][r], G[nt]);
+                    if (!SAME) Gd[nt] = __builtin_fma(Au[nt].cur, accd[nt][r], Gd[nt]);
+                    Au[nt].step();
                 }
             }
-        };
-        // Loads are unconditional and fenced from the scheduler so that they
-        // stay one group ahead of the MFMAs that consume them.
-        load(0, a0, b0);
-        int t = 0;
-        for (; t + 3 <= groups; t += 2) {
-            load(t + 1, a1, b1);
-            __builtin_amdgcn_sched_barrier(0);
-            group(a0, b0);
-            load(t + 2, a0, b0);
-            __builtin_amdgcn_sched_barrier(0);
-            group(a1, b1);
         }
-        if (groups - t == 2) {
-            load(t + 1, a1, b1);
-            __builtin_amdgcn_sched_barrier(0);
-            group(a0, b0);
-            group(a1, b1);
-        } else {
-            group(a0, b0);
-        }
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                G[nt] = __builtin_fma(Au[nt].cur, acc[nt][r], G[nt]);
-                if (!SAME) Gd[nt] = __builtin_fma(Au[nt].cur, accd[nt][r], Gd[nt]);
-                Au[nt].step();
-            }
-        }
+    };
+    // Two register sets: step k + 1 is loaded while step k runs.
+    double a0[4], b0[4], a1[4], b1[4];
+    load_next(a0, b0);
+    for (int k = 0; k < total; k += 2) {
+        load_next(a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(a0, b0);
+        load_next(a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (k + 1 < total) compute(a1, b1);
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -375,22 +384,21 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
 }
 
 // FP64 matrix-core contraction for nu % 16 == 0 and nv % 16 == 0: one wave
-// per tile of up to 64 queries of one coefficient block.  SAME selects the
-// form for contexts whose up and down tables are one table for both classes.
+// per tile of up to 64 queries of one coefficient block (tiles split over the
+// XCDs by xcd_tiles).  SAME selects the form for contexts whose up and down
+// tables are one table for both classes; it is held to 128 registers so that
+// four waves share each SIMD.
 template <bool SAME>
-__global__ __launch_bounds__(256) void k_contract_f64(KScalars ks, const int32_t* __restrict__ n_tiles,
-                                                      const int32_t* __restrict__ tile_key,
-                                                      const int32_t* __restrict__ tile_start,
-                                                      const int32_t* __restrict__ tile_len,
-                                                      const double* __restrict__ s_u, const double* __restrict__ s_v,
-                                                      const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
-                                                      double* __restrict__ g_dn) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SAME ? 4 : 2))) void k_contract_f64(
+    KScalars ks, const int32_t* __restrict__ n_tiles, const int32_t* __restrict__ tile_key,
+    const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_len, const double* __restrict__ s_u,
+    const double* __restrict__ s_v, const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
+    double* __restrict__ g_dn) {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4;
     const int j = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wpb = blockDim.x >> 6;
-    const TileRange tr = xcd_tiles(*n_tiles, wave, wpb);
+    const TileRange tr = xcd_tiles(*n_tiles, wave, blockDim.x >> 6);
     for (int t0 = tr.first; t0 < tr.end; t0 += tr.stride) {
         const int tile = __builtin_amdgcn_readfirstlane(t0);
         const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);

@@ -30,6 +30,7 @@ struct cdet_batch_plan {
     double* d_qv = nullptr;
     int32_t* d_hist = nullptr;
     int32_t* d_keyoff = nullptr;
+    int2* d_wgtot = nullptr;  // per-workgroup totals of the key scan
     int32_t* d_tkey = nullptr;
     int32_t* d_tstart = nullptr;
     int32_t* d_tlen = nullptr;
@@ -59,7 +60,7 @@ static void plan_free(cdet_batch_plan* p) {
     void* bufs[] = {p->d_order, p->d_vbase, p->d_pbase, p->d_obase, p->d_lists, p->d_gup, p->d_gdn, p->d_qkey,
                     p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
                     p->d_ntiles, p->d_su, p->d_sv, p->d_sdest, p->d_out_ord, p->d_single, p->d_fb_list,
-                    p->d_fb_count, p->scratch.ptr};
+                    p->d_fb_count, p->d_wgtot, p->scratch.ptr};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -88,30 +89,26 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
     return launch_combine(p->eval_kind, p->count, p->out_total, p->d_obase, o, s);
 }
 
-static void plan_reset(cdet_batch_plan* p, int32_t* d_status, hipStream_t s) {
-    const int64_t nk = p->nkeys;
-    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, s, p->d_hist, nk, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, p->d_fb_count, static_cast<int64_t>(kMaxOrder + 1), 0);
-    if (d_status)
-        hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((p->count + 255) / 256)), dim3(256), 0, s, d_status,
-                           p->count, 0);
-}
-
-static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, hipStream_t s) {
+// Lookup planning and grouping.  The run needs no clearing launches: k_plan
+// clears the status words, k_scan_keys clears the histogram after reading it
+// and k_cdet clears each bucket's fallback-queue length after use (all start
+// cleared at plan creation).
+static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, int32_t* d_status, hipStream_t s) {
     cdet_ctx* ctx = p->ctx;
     hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
                        reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
-                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist);
-    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, s, p->d_hist, p->nkeys, p->tile, p->d_keyoff, p->d_tkey,
-                       p->d_tstart, p->d_tlen, p->d_ntiles);
+                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist, d_status);
+    const unsigned kwg = static_cast<unsigned>((p->nkeys + kScanKeys - 1) / kScanKeys);
+    hipLaunchKernelGGL(k_key_totals, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot);
+    hipLaunchKernelGGL(k_scan_keys, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot,
+                       p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen, p->d_ntiles);
     hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>((p->nq + 255) / 256)), dim3(256), 0, s, p->nq, p->d_qkey,
                        p->d_qrank, p->d_qu, p->d_qv, p->d_keyoff, p->d_su, p->d_sv, p->d_sdest);
 }
 
 static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
                                 hipStream_t s) {
-    plan_reset(p, d_status, s);
-    plan_group(p, d_vertices, s);
+    plan_group(p, d_vertices, d_status, s);
     int rc = contract_launch(p, s);
     if (rc) return rc;
     rc = plan_minors(p, d_out, d_status, s);
@@ -205,8 +202,9 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     rc |= dev_alloc(&p->d_qrank, p->nq);
     rc |= dev_alloc(&p->d_qu, p->nq);
     rc |= dev_alloc(&p->d_qv, p->nq);
-    rc |= dev_alloc(&p->d_hist, p->nkeys);
-    rc |= dev_alloc(&p->d_keyoff, p->nkeys);
+    rc |= dev_alloc(&p->d_hist, static_cast<int64_t>(p->nkeys) * kKeySub);
+    rc |= dev_alloc(&p->d_keyoff, static_cast<int64_t>(p->nkeys) * kKeySub);
+    rc |= dev_alloc(&p->d_wgtot, (p->nkeys + kScanKeys - 1) / kScanKeys);
     rc |= dev_alloc(&p->d_tkey, p->max_tiles);
     rc |= dev_alloc(&p->d_tstart, p->max_tiles);
     rc |= dev_alloc(&p->d_tlen, p->max_tiles);
@@ -218,6 +216,9 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     rc |= dev_alloc(&p->d_single, p->eval_kind == 1 ? 4 * count : 1);
     rc |= dev_alloc(&p->d_fb_list, 4 * count);
     rc |= dev_alloc(&p->d_fb_count, kMaxOrder + 1);
+    if (!rc && (hipMemset(p->d_hist, 0, sizeof(int32_t) * static_cast<size_t>(p->nkeys) * kKeySub) != hipSuccess ||
+                hipMemset(p->d_fb_count, 0, sizeof(int32_t) * (kMaxOrder + 1)) != hipSuccess))
+        rc = 1;
     if (rc) {
         plan_free(p);
         return CDET_ERR_DEVICE;
@@ -231,12 +232,7 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
         plan_free(p);
         return fail(CDET_ERR_DEVICE, "plan upload failed");
     }
-    // Persistent contraction grid: enough workgroups to fill 256 CUs several
-    // times, a multiple of the 8 XCDs.
-    const int64_t wgs = (p->max_tiles + 3) / 4;
-    int64_t grid = std::min<int64_t>(std::max<int64_t>(wgs, 1), 256 * 8);
-    grid = ((grid + 7) / 8) * 8;
-    p->contract_grid = static_cast<int>(grid);
+    p->contract_grid = contract_grid(ctx, p->max_tiles);
     // Global scratch (minors of every spin matrix, workspaces of large
     // orders), sized now so that runs never allocate and can be graph-captured.
     size_t scratch_need = 0;
@@ -312,8 +308,7 @@ int cdet_batch_profile(cdet_batch_plan* p, const cdet_vertex* d_vertices, double
     hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], s));
-    plan_reset(p, d_status, s);
-    plan_group(p, d_vertices, s);
+    plan_group(p, d_vertices, d_status, s);
     HIP_TRY(hipEventRecord(ev[1], s));
     int rc = contract_launch(p, s);
     if (rc) return rc;
@@ -351,6 +346,7 @@ struct QueryScratch {
     double *gup = nullptr, *gdn = nullptr, *qu = nullptr, *qv = nullptr, *su = nullptr, *sv = nullptr;
     int32_t *qkey = nullptr, *qrank = nullptr, *sdest = nullptr, *hist = nullptr, *keyoff = nullptr;
     int32_t *tkey = nullptr, *tstart = nullptr, *tlen = nullptr, *ntiles = nullptr;
+    int2* wgtot = nullptr;
 };
 
 static QueryScratch& query_scratch(cdet_ctx* ctx) {
@@ -379,7 +375,7 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     const int64_t max_tiles = count / tile + std::min<int64_t>(nkeys, count) + 1;
     if (q.cap < count || q.tiles_cap < max_tiles || !q.hist) {
         void* bufs[] = {q.gup, q.gdn, q.qu, q.qv, q.su, q.sv, q.qkey, q.qrank, q.sdest, q.hist, q.keyoff,
-                        q.tkey, q.tstart, q.tlen, q.ntiles};
+                        q.tkey, q.tstart, q.tlen, q.ntiles, q.wgtot};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         q = QueryScratch();
@@ -393,8 +389,9 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
         rc |= dev_alloc(&q.qkey, count);
         rc |= dev_alloc(&q.qrank, count);
         rc |= dev_alloc(&q.sdest, count);
-        rc |= dev_alloc(&q.hist, nkeys);
-        rc |= dev_alloc(&q.keyoff, nkeys);
+        rc |= dev_alloc(&q.hist, static_cast<int64_t>(nkeys) * kKeySub);
+        rc |= dev_alloc(&q.keyoff, static_cast<int64_t>(nkeys) * kKeySub);
+        rc |= dev_alloc(&q.wgtot, (nkeys + kScanKeys - 1) / kScanKeys);
         rc |= dev_alloc(&q.tkey, max_tiles);
         rc |= dev_alloc(&q.tstart, max_tiles);
         rc |= dev_alloc(&q.tlen, max_tiles);
@@ -405,16 +402,17 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     }
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     const unsigned gq = static_cast<unsigned>((count + 255) / 256);
-    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nkeys + 255) / 256)), dim3(256), 0, s, q.hist,
-                       static_cast<int64_t>(nkeys), 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nkeys * kKeySub + 255) / 256)), dim3(256), 0, s,
+                       q.hist, static_cast<int64_t>(nkeys) * kKeySub, 0);
     hipLaunchKernelGGL(k_plan_queries, dim3(gq), dim3(256), 0, s, ctx->ks, count, query_ints, query_taus, q.gup, q.gdn,
                        q.qkey, q.qrank, q.qu, q.qv, q.hist);
-    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, s, q.hist, nkeys, tile, q.keyoff, q.tkey, q.tstart, q.tlen,
-                       q.ntiles);
+    const unsigned kwg = static_cast<unsigned>((nkeys + kScanKeys - 1) / kScanKeys);
+    hipLaunchKernelGGL(k_key_totals, dim3(kwg), dim3(kScanKeys), 0, s, q.hist, nkeys, tile, q.wgtot);
+    hipLaunchKernelGGL(k_scan_keys, dim3(kwg), dim3(kScanKeys), 0, s, q.hist, nkeys, tile, q.wgtot, q.keyoff, q.tkey,
+                       q.tstart, q.tlen, q.ntiles);
     hipLaunchKernelGGL(k_scatter, dim3(gq), dim3(256), 0, s, count, q.qkey, q.qrank, q.qu, q.qv, q.keyoff, q.su, q.sv,
                        q.sdest);
-    int64_t grid = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), 256 * 8);
-    grid = ((grid + 7) / 8) * 8;
+    const int grid = contract_grid(ctx, max_tiles);
     if (mode == CDET_MODE_EXACT) {
         hipLaunchKernelGGL(k_contract_exact, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, ctx->ks, q.ntiles,
                            q.tkey, q.tstart, q.tlen, q.su, q.sv, q.sdest, q.gup, q.gdn);

@@ -110,6 +110,9 @@ __global__ __launch_bounds__(256) void k_node_green(ProdArgs a, int node_begin, 
         Rr[e] = 0.0;
         Ri[e] = 0.0;
     }
+    // the diagonal of R is set below by other threads than the ones that
+    // cleared it above
+    __syncthreads();
     for (int r = tid; r < S; r += nt) {
         const double x = a.beta * a.evals[r];
         const double sx = upper ? x : -x;
@@ -443,10 +446,13 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
     DevTable& t = ctx->tab[kind];
     size_t ncoef, ncorner;
     table_sizes(ctx, ncoef, ncorner);
-    HIP_TRY(hipMemset(t.upper, 0, ncoef * sizeof(double)));
-    HIP_TRY(hipMemset(t.lower, 0, ncoef * sizeof(double)));
-    HIP_TRY(hipMemset(t.ucorner, 0, ncorner * sizeof(double)));
-    HIP_TRY(hipMemset(t.lcorner, 0, ncorner * sizeof(double)));
+    // cleared on the producer stream so that the clears are ordered before
+    // the node kernels (the context stream does not synchronise with the
+    // null stream)
+    HIP_TRY(hipMemsetAsync(t.upper, 0, ncoef * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(t.lower, 0, ncoef * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(t.ucorner, 0, ncorner * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(t.lcorner, 0, ncorner * sizeof(double), ctx->stream));
 
     const size_t lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
     if (lds > 64 * 1024)

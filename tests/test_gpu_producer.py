@@ -35,3 +35,23 @@ def test_gpu_tables_match_oracle(gpu_available, name):
         sc = np.maximum(np.abs(ref), 1e-6 * np.max(np.abs(ref)))
         assert np.max(np.abs(got - ref) / sc) < 1e-9
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "c1_neq"])
+def test_gpu_tables_bitwise_reproducible(gpu_available, name):
+    # Two builds of the same tables on one device are bit-identical (the node
+    # kernels and the transform have no order-dependent arithmetic).
+    s = helpers.SETS[name]
+    p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"])
+    got = []
+    for rep in range(3):
+        ctx = cdet_amd.Context(p)
+        print(f"producer build {rep} {name}", flush=True)
+        ctx.build_tables(pbc=True, mask=s["mask"], mask_param=s["mask_param"],
+                         node_chunk=0 if rep != 1 else 5 * s["nc"] + 1)
+        got.append([ctx.download_table(k) for k in range(4)])
+        ctx.close()
+    for rep in (1, 2):
+        for k in range(4):
+            for a, b in zip(got[0][k], got[rep][k]):
+                assert np.array_equal(a.view(np.int64), b.view(np.int64)), (name, rep, k)

@@ -56,6 +56,10 @@ def parse():
                     help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: order-10 integrand batch (headline); c3: 2^20 table lookups on the 8x8 lattice; "
+                         "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice")
+    ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
     return ap.parse_args()
 
 
@@ -155,8 +159,184 @@ def cpu_baseline(tables, params_gpu, verts, seconds):
             "sample": f"{n} order-10 all-subset evaluations over {dt:.1f} s wall on {cores} threads"}
 
 
+C3 = dict(N1=8, N2=8, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="origin_square", radius=2,
+          lookups=1 << 20, base_seed=2026101401, tag=0xC3, block=1024)
+
+
+def c3_queries():
+    """2^20 twisted-table lookups on the 8x8 lattice: block i of 1024 lookups
+    draws from mt19937_64(derive_mcmcmc_seed(2026101401, 0xC3, i)), per lookup
+    six U[0,1) numbers in the order site_i, site_j, replica_a, replica_b, tau,
+    tau' (site = floor(64 u) with x = site mod 8, y = site div 8; replica =
+    floor(2 u); tau = beta u)."""
+    import rng
+    w = C3
+    nb = w["lookups"] // w["block"]
+    eng = rng.MT19937_64([rng.derive_mcmcmc_seed(w["base_seed"], w["tag"], i) for i in range(nb)])
+    u = np.stack([eng.uniform() for _ in range(6 * w["block"])], axis=1).reshape(nb, w["block"], 6)
+    u = u.reshape(-1, 6)
+    S = w["N1"] * w["N2"]
+    si = np.minimum((S * u[:, 0]).astype(np.int32), S - 1)
+    sj = np.minimum((S * u[:, 1]).astype(np.int32), S - 1)
+    qi = np.zeros((len(u), 7), dtype=np.int32)
+    qi[:, 1] = np.minimum((2 * u[:, 2]).astype(np.int32), 1)
+    qi[:, 2] = np.minimum((2 * u[:, 3]).astype(np.int32), 1)
+    qi[:, 3], qi[:, 4] = si % w["N1"], si // w["N1"]
+    qi[:, 5], qi[:, 6] = sj % w["N1"], sj // w["N1"]
+    qt = np.stack([w["beta"] * u[:, 4], w["beta"] * u[:, 5]], axis=1)
+    return qi, qt
+
+
+def main_c3(args):
+    """Table evaluation only (configuration C3): lookups/s of the FP64 matrix-
+    core contraction with the FP64 Clenshaw form timed beside it, and the
+    relative error of the former against the latter."""
+    import ctypes
+    import torch
+    import cdet_amd
+    w = C3
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    params = cdet_amd.make_params(w["N1"], w["N2"], w["nc"], w["beta"], w["U"], w["mu"], w["alpha"], w["alpha"])
+    ctx = cdet_amd.Context(params)
+    t0 = time.time()
+    print("c3: building tables", file=sys.stderr, flush=True)
+    ctx.build_tables(pbc=True, mask=w["mask"], mask_param=w["radius"])
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    print(f"c3: tables built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    qi, qt = c3_queries()
+    n = len(qi)
+    d_qi = torch.tensor(qi, device=dev)
+    d_qt = torch.tensor(qt, device=dev)
+    sptr = torch.cuda.current_stream().cuda_stream
+    outs = {}
+    rates = {}
+    for mode in ("exact", "mfma64"):
+        out = torch.empty(2 * n, dtype=torch.float64, device=dev)
+
+        def call():
+            rc = cdet_amd.lib().cdet_green_pair_eval(ctx.handle, ctypes.c_void_p(d_qi.data_ptr()),
+                                                     ctypes.c_void_p(d_qt.data_ptr()), n,
+                                                     ctypes.c_void_p(out.data_ptr()), cdet_amd.MODES[mode],
+                                                     ctypes.c_void_p(sptr))
+            if rc != 0:
+                raise RuntimeError(cdet_amd._err())
+        for _ in range(max(1, args.warmup)):
+            call()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            call()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        rates[mode] = (n / dt, dt)
+        outs[mode] = out.cpu().numpy().reshape(n, 2)
+        print(f"c3: {mode} {n / dt:.3e} lookups/s", file=sys.stderr, flush=True)
+    ref, got = outs["exact"], outs["mfma64"]
+    floor = 1e-12 * np.max(np.abs(ref))
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), floor)
+    flops = n * (2.0 * w["nc"] * w["nc"] + 2.0 * w["nc"])
+    value, ms = rates["mfma64"][0], rates["mfma64"][1] * 1e3
+    line = {
+        "metric": "C3 table lookups/s (8x8, beta=8, Nc=128, twisted table, FP64 matrix-core vs FP64 Clenshaw)",
+        "value": round(value, 1), "unit": "lookups/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "8x8 Hubbard, beta=8, 4x4 corner subsystem, 128x128 Chebyshev; 2^20 seeded "
+                               "twisted-table lookups per step (planning, grouping, contraction)",
+                   "table_build_s": round(build_s, 2)},
+        "fp64_clenshaw_lookups_per_s": round(rates["exact"][0], 1),
+        "speedup_vs_fp64_clenshaw": round(rates["mfma64"][0] / rates["exact"][0], 2),
+        "whole_call_fp64_tflops": round(flops / rates["mfma64"][1] / 1e12, 3),
+        "rel_err_vs_fp64_clenshaw": {"max": float(np.max(rel)), "mean": float(np.mean(rel))},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+C4 = dict(N1=6, N2=6, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="left_columns", width=3,
+          chains=256, envelope=12, base_seed=2026101401)
+
+
+def main_c4(args):
+    """Batched MCMCMC sweeps (configuration C4): 256 chains advanced in lockstep
+    by the C++ driver, every chain's all-subset envelope evaluated in one GPU
+    batch per sweep (host buffers through cdet_eval_batch_host), the heat-bath
+    steps on the host.  All chains use envelope order 12 (orders 1..11), the
+    top of C4's per-chain range."""
+    import ctypes
+    import torch
+    import cdet_amd
+    import mcmcmc
+    w = C4
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    params = cdet_amd.make_params(w["N1"], w["N2"], w["nc"], w["beta"], w["U"], w["mu"], w["alpha"], w["alpha"])
+    ctx = cdet_amd.Context(params)
+    t0 = time.time()
+    print("c4: building tables", file=sys.stderr, flush=True)
+    ctx.build_tables(pbc=True, mask=w["mask"], mask_param=w["width"])
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    print(f"c4: tables built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    mode = "mfma64" if args.mode == "split32" else args.mode
+    cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[mode])
+    E = w["envelope"]
+    c = mcmcmc.MCMCMCConfig()
+    c.order_min, c.order_max, c.envelope_order, c.normalization_order = 1, E - 1, E, 1
+    for o in range(1, E):
+        c.lambdas[o] = 1.0
+    warm = max(1, args.sweeps // 10)
+    c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = warm, max(1, warm // 5), 0.25
+    c.production_steps, c.chain_count, c.base_seed = args.sweeps - warm, w["chains"], w["base_seed"]
+    c.seed_mode, c.checkpoint_enabled, c.resume_mode = 0, 0, 0
+    c.checkpoint_dir = b"/tmp/cdet_c4_unused"
+    fn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    # GPU batch alone: one envelope batch of all chains, graph replays
+    import rng
+    orders = [E] * w["chains"]
+    _, verts = rng.synthetic_configs(w["base_seed"], 0xC4, w["chains"], E, E)
+    plan = ctx.plan(orders, mode=mode)
+    d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device=dev)
+    d_o = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
+    sptr = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(20):
+        plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t) / 20 * 1e3
+    plan.close()
+    print(f"c4: GPU batch of {w['chains']} order-{E} envelopes {batch_ms:.3f} ms", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    results, ratios, _ = mcmcmc.run(c, fn, ctx.handle)
+    dt = time.perf_counter() - t
+    sweeps = args.sweeps
+    line = {
+        "metric": "C4 MCMCMC sweeps/s (6x6, beta=8, Nc=128, 256 chains, envelope 12, lockstep batches)",
+        "value": round(sweeps / dt, 2), "unit": "sweeps/s", "n_gpus": 1, "steps": sweeps, "warmup": warm,
+        "ms_per_step": round(dt / sweeps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "6x6 Hubbard, beta=8, x<3 half-strip subsystem, 128x128 Chebyshev; 256 chains, "
+                               "orders 1..11 in envelope 12, uniform seed, one GPU batch per sweep",
+                   "mode": mode, "table_build_s": round(build_s, 2)},
+        "envelope_evals_per_s": round(sweeps * w["chains"] / dt, 1),
+        "gpu_batch_ms": round(batch_ms, 4),
+        "ratio_order2_over_order1": float(ratios[2]) if len(ratios) > 2 else None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
+    if args.gpus == 1 and args.workload == "c3":
+        return main_c3(args)
+    if args.gpus == 1 and args.workload == "c4":
+        return main_c4(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
               "(chains shard across GPUs with replicated tables; see the design notes)", file=sys.stderr)
@@ -228,7 +408,7 @@ def main():
     flops = queries * (2.0 * nc * nc + 2.0 * nc)
     contract_s = stage[1] * 1e-3
     achieved = flops / contract_s / 1e12
-    kname = {"exact": "k_contract_exact", "mfma64": "k_contract_f64<true>", "split32": "k_contract_mfma<2>"}[args.mode]
+    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true>", "split32": "cdet::k_contract_mfma<2>"}[args.mode]
     roofline = {"bound": "mfma" if args.mode != "exact" else "fp64-valu", "achieved": round(achieved, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                 "traffic": traffic_from_profiles(kname), "kernel": kname,

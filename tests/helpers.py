@@ -9,7 +9,8 @@ from oracle import oracle as O
 # (tests/test_integrand_all_subsets.cpp:11-39); "c1" the 3x3 workload
 # (t=1, half filling, beta=4, one-column subsystem, 64 coefficients);
 # "c1_neq" the same with unequal spin shifts (distinct up and down tables);
-# "c2" the 4x4 workload (beta=6, 2x2 corner block, 96 coefficients).
+# "p8x8" the largest lattice of the LDS producer (64 sites) at a small
+# coefficient count; "c2" the 4x4 workload (beta=6, 2x2 corner block, 96 coefficients).
 SETS = {
     "ref2x2": dict(N1=2, N2=2, beta=2.5, U=1.7, mu=0.3, alpha_up=0.55, alpha_down=0.4, nc=18,
                    mask="first_row", mask_param=0),
@@ -19,6 +20,8 @@ SETS = {
                mask="first_column", mask_param=0),
     "c1_neq": dict(N1=3, N2=3, beta=4.0, U=2.0, mu=1.0, alpha_up=1.15, alpha_down=0.85, nc=64,
                    mask="first_column", mask_param=0),
+    "p8x8": dict(N1=8, N2=8, beta=8.0, U=2.0, mu=1.0, alpha_up=1.2, alpha_down=0.8, nc=16,
+                 mask="origin_square", mask_param=2),
     "c2": dict(N1=4, N2=4, beta=6.0, U=2.0, mu=1.0, alpha_up=1.0, alpha_down=1.0, nc=96,
                mask="origin_square", mask_param=1),
 }

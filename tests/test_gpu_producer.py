@@ -11,7 +11,7 @@ import helpers
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["ref2x2", "ref2x2_eq", "c1"])
+@pytest.mark.parametrize("name", ["ref2x2", "ref2x2_eq", "c1", "p8x8"])
 def test_gpu_tables_match_oracle(gpu_available, name):
     s = helpers.SETS[name]
     p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"])
@@ -28,12 +28,15 @@ def test_gpu_tables_match_oracle(gpu_available, name):
         assert np.max(np.abs(uc - w.ucorner)) < 1e-12, (name, kind)
         assert np.max(np.abs(lc - w.lcorner)) < 1e-12, (name, kind)
     # integrand on producer tables agrees with the oracle on oracle tables
-    orc = helpers.oracle_of(name)
-    for v in helpers.seeded_configs(99, [3, 5, 6]):
-        got = ctx.evaluate_all_subset_integrands(v)
-        ref = orc.all_subsets(v)
-        sc = np.maximum(np.abs(ref), 1e-6 * np.max(np.abs(ref)))
-        assert np.max(np.abs(got - ref) / sc) < 1e-9
+    # (the 64-site lattice is checked at the table level only: its 16-term
+    # expansion is a coverage case for the producer, not an integrand workload)
+    if name != "p8x8":
+        orc = helpers.oracle_of(name)
+        for v in helpers.seeded_configs(99, [3, 5, 6]):
+            got = ctx.evaluate_all_subset_integrands(v)
+            ref = orc.all_subsets(v)
+            sc = np.maximum(np.abs(ref), 1e-6 * np.max(np.abs(ref)))
+            assert np.max(np.abs(got - ref) / sc) < 1e-9
     ctx.close()
 
 

@@ -325,7 +325,7 @@ def main_c4(args):
                    "mode": mode, "table_build_s": round(build_s, 2)},
         "envelope_evals_per_s": round(sweeps * w["chains"] / dt, 1),
         "gpu_batch_ms": round(batch_ms, 4),
-        "ratio_order2_over_order1": float(ratios[2]) if len(ratios) > 2 else None,
+        "ratio_order2_over_order1": (float(ratios[2]) if len(ratios) > 2 and np.isfinite(ratios[2]) else None),
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -31,7 +31,7 @@ FLAGS = [
 DRIVER_SRC = os.path.join(HERE, "driver", "cdet_mcmcmc.cpp")
 DRIVER_OUT = os.path.join(HERE, "libcdet_mcmcmc.so")
 # Host driver: g++/libstdc++ (the reference's toolchain), generic x86-64.
-DRIVER_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off", "-Wall"]
+DRIVER_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-Wall"]
 
 
 def build_driver(verbose=False):

@@ -34,7 +34,63 @@ class MCMCMCConfig(ctypes.Structure):
         ("checkpoint_enabled", ctypes.c_int32), ("checkpoint_interval_steps", ctypes.c_uint64),
         ("resume_mode", ctypes.c_int32), ("checkpoint_dir", ctypes.c_char * 1024),
         ("stop_after_steps", ctypes.c_uint64),
+        ("N1", ctypes.c_int32), ("N2", ctypes.c_int32), ("beta", ctypes.c_double),
+        ("histogram_learning_steps", ctypes.c_int32), ("histogram_tau_bins", ctypes.c_int32),
+        ("histogram_uniform_mix", ctypes.c_double), ("histogram_pseudo_count", ctypes.c_double),
+        ("histogram_learning_only", ctypes.c_int32), ("histogram_output_prefix", ctypes.c_char * 1024),
+        ("host_threads", ctypes.c_int32),
     ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        # The histogram settings are validated in every seed mode; start from
+        # the reference's defaults (include/cdet/mcmcmc_sampler.h:92-95).
+        self.N1 = self.N2 = 1
+        self.beta = 1.0
+        self.histogram_tau_bins = 16
+        self.histogram_uniform_mix = 0.05
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class HistogramSeed(ctypes.Structure):
+    """cdet_histogram_seed (include/cdet_mcmcmc.h); the numpy tables are kept
+    alive by the instance."""
+
+    _fields_ = [
+        ("tau_bins", ctypes.c_int32), ("N1", ctypes.c_int32), ("N2", ctypes.c_int32),
+        ("uniform_mix", ctypes.c_double), ("pseudo_count", ctypes.c_double),
+        ("tau_prob", ctypes.POINTER(ctypes.c_double)), ("space_prob", ctypes.POINTER(ctypes.c_double)),
+        ("replica_prob", ctypes.c_double * 4),
+    ]
+
+    @classmethod
+    def make(cls, tau_bins, N1, N2, tau=None, space=None, replica=None, uniform_mix=0.0, pseudo_count=0.0):
+        h = cls()
+        h.tau_bins, h.N1, h.N2 = tau_bins, N1, N2
+        h.uniform_mix, h.pseudo_count = uniform_mix, pseudo_count
+        S = N1 * N2
+        h._tau = np.ascontiguousarray(np.zeros((tau_bins, tau_bins)) if tau is None else tau, dtype=np.float64)
+        h._space = np.ascontiguousarray(np.zeros((S, S)) if space is None else space, dtype=np.float64)
+        assert h._tau.shape == (tau_bins, tau_bins) and h._space.shape == (S, S)
+        h.tau_prob = h._tau.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h.space_prob = h._space.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if replica is not None:
+            for i, x in enumerate(np.asarray(replica, dtype=np.float64).ravel()):
+                h.replica_prob[i] = x
+        return h
+
+    @property
+    def tau(self):
+        return self._tau
+
+    @property
+    def space(self):
+        return self._space
+
+    @property
+    def replica(self):
+        return np.array(self.replica_prob[:]).reshape(2, 2)
 
 
 class ChainResult(ctypes.Structure):
@@ -62,6 +118,27 @@ def lib():
                                       ctypes.POINTER(ChainResult), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
         L.cdet_mcmcmc_last_error.restype = ctypes.c_char_p
+        L.cdet_mcmcmc_config_defaults.restype = None
+        L.cdet_mcmcmc_config_defaults.argtypes = [ctypes.POINTER(MCMCMCConfig)]
+        L.cdet_mcmcmc_run_seeded.restype = ctypes.c_int
+        L.cdet_mcmcmc_run_seeded.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.POINTER(HistogramSeed),
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ChainResult),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.c_int64]
+        L.cdet_histogram_learn.restype = ctypes.c_int
+        L.cdet_histogram_learn.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.POINTER(HistogramSeed), ctypes.POINTER(ctypes.c_double)]
+        L.cdet_histogram_export.restype = ctypes.c_int
+        L.cdet_histogram_export.argtypes = [ctypes.POINTER(HistogramSeed), ctypes.c_double, ctypes.c_char_p]
+        L.cdet_seed_conditional_probabilities.restype = ctypes.c_int
+        L.cdet_seed_conditional_probabilities.argtypes = [ctypes.POINTER(HistogramSeed), ctypes.c_void_p,
+                                                          ctypes.c_int, ctypes.c_void_p]
+        L.cdet_seed_sample_children.restype = ctypes.c_int
+        L.cdet_seed_sample_children.argtypes = [ctypes.POINTER(HistogramSeed), ctypes.c_void_p, ctypes.c_int64,
+                                                ctypes.c_uint64, ctypes.c_void_p]
+        L.cdet_seed_density.restype = ctypes.c_int
+        L.cdet_seed_density.argtypes = [ctypes.POINTER(HistogramSeed), ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -87,6 +164,10 @@ def _one(params, key, default=None):
     return default
 
 
+def _bool(text):
+    return text.strip().lower() in ("1", "true", "yes", "on")
+
+
 def config_from_params(params, **over):
     """MCMCMCConfig from params.txt keys and defaults (src/main.cpp:461-535)."""
     omin = int(_one(params, "MCMCMC_ORDER_MIN", "2"))
@@ -107,14 +188,29 @@ def config_from_params(params, **over):
     c.chain_count = int(_one(params, "MCMCMC_CHAIN_COUNT", "8"))
     c.base_seed = int(_one(params, "MCMCMC_BASE_SEED", "998875432210"))
     c.seed_mode = {"uniform": 0, "histogram": 1}[_one(params, "MCMCMC_SEED_MODE", "uniform").lower()]
-    c.checkpoint_enabled = int(_one(params, "MCMCMC_CHECKPOINT_ENABLED", "false").lower() in ("1", "true", "yes", "on"))
+    c.checkpoint_enabled = int(_bool(_one(params, "MCMCMC_CHECKPOINT_ENABLED", "false")))
     c.checkpoint_interval_steps = int(_one(params, "MCMCMC_CHECKPOINT_INTERVAL_STEPS", "10000000"))
     rm = _one(params, "MCMCMC_RESUME", "off").lower()
     c.resume_mode = {"off": 0, "auto": 1, "require": 2, "required": 2}[rm]
     c.checkpoint_dir = _one(params, "MCMCMC_CHECKPOINT_DIR", "mcmcmc_checkpoints").encode()
+    # Histogram seed keys (src/main.cpp:495-509).
+    c.N1 = int(_one(params, "N1", "1"))
+    c.N2 = int(_one(params, "N2", "1"))
+    c.beta = float(_one(params, "BETA", "1"))
+    c.histogram_learning_steps = int(_one(params, "MCMCMC_SEED_HISTOGRAM_LEARNING_STEPS", "0"))
+    c.histogram_tau_bins = int(_one(params, "MCMCMC_SEED_HISTOGRAM_TAU_BINS", "16"))
+    c.histogram_uniform_mix = float(_one(params, "MCMCMC_SEED_HISTOGRAM_UNIFORM_MIX", "0.05"))
+    c.histogram_pseudo_count = float(_one(params, "MCMCMC_SEED_HISTOGRAM_PSEUDO_COUNT", "0"))
+    c.histogram_learning_only = int(_bool(_one(params, "MCMCMC_SEED_HISTOGRAM_LEARNING_ONLY", "false")))
+    c.histogram_output_prefix = _one(params, "MCMCMC_SEED_HISTOGRAM_OUTPUT_PREFIX", "").encode()
     for k, v in over.items():
-        setattr(c, k, v.encode() if k == "checkpoint_dir" else v)
+        setattr(c, k, v.encode() if k in ("checkpoint_dir", "histogram_output_prefix") else v)
     return c
+
+
+def compare_uniform_seed(params):
+    """MCMCMC_COMPARE_UNIFORM_SEED (src/main.cpp:508-509)."""
+    return _bool(_one(params, "MCMCMC_COMPARE_UNIFORM_SEED", "false"))
 
 
 def integrand_scalars_from_params(params):
@@ -148,3 +244,66 @@ def run(config, eval_fn, eval_user, trace_steps=0):
     if rc != 0:
         raise RuntimeError(lib().cdet_mcmcmc_last_error().decode())
     return list(results), np.array(ratios[:config.envelope_order + 1]), (trace if trace_steps else None)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().cdet_mcmcmc_last_error().decode())
+
+
+def run_seeded(config, seed, eval_fn, eval_user, trace_steps=0):
+    """Run the chains with a given seed kernel (None = uniform)."""
+    n = config.chain_count
+    results = (ChainResult * n)()
+    ratios = (ctypes.c_double * (MAXE + 1))()
+    trace = np.zeros((n, max(trace_steps, 1)), dtype=np.uint64)
+    fn = ctypes.cast(eval_fn, ctypes.c_void_p)
+    _check(lib().cdet_mcmcmc_run_seeded(
+        ctypes.byref(config), ctypes.byref(seed) if seed is not None else None, fn, eval_user, results, ratios,
+        trace.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if trace_steps else None, trace_steps))
+    return list(results), np.array(ratios[:config.envelope_order + 1]), (trace if trace_steps else None)
+
+
+def learn_histogram(config, eval_fn, eval_user):
+    """Learn the factorized histogram seed; returns (HistogramSeed, total pair weight)."""
+    h = HistogramSeed.make(config.histogram_tau_bins, config.N1, config.N2)
+    tot = ctypes.c_double()
+    _check(lib().cdet_histogram_learn(ctypes.byref(config), ctypes.cast(eval_fn, ctypes.c_void_p), eval_user,
+                                      ctypes.byref(h), ctypes.byref(tot)))
+    return h, tot.value
+
+
+def export_histogram(seed, beta, prefix):
+    _check(lib().cdet_histogram_export(ctypes.byref(seed), beta, prefix.encode()))
+
+
+def conditional_probabilities(seed, vertices):
+    v = np.ascontiguousarray(vertices, dtype=np.float64)
+    E = v.shape[0]
+    out = np.zeros(1 << E)
+    _check(lib().cdet_seed_conditional_probabilities(ctypes.byref(seed) if seed is not None else None,
+                                                     v.ctypes.data, E, out.ctypes.data))
+    return out
+
+
+def sample_children(seed, parents, rng_seed):
+    p = np.ascontiguousarray(parents, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros_like(p)
+    _check(lib().cdet_seed_sample_children(ctypes.byref(seed) if seed is not None else None, p.ctypes.data,
+                                           p.shape[0], rng_seed, out.ctypes.data))
+    return out
+
+
+def seed_density(seed, child, parent):
+    c = np.ascontiguousarray(child, dtype=np.float64)
+    p = np.ascontiguousarray(parent, dtype=np.float64)
+    out = np.zeros(1)
+    _check(lib().cdet_seed_density(ctypes.byref(seed) if seed is not None else None, c.ctypes.data,
+                                   p.ctypes.data, out.ctypes.data))
+    return float(out[0])
+
+
+def defaults():
+    c = MCMCMCConfig()
+    lib().cdet_mcmcmc_config_defaults(ctypes.byref(c))
+    return c

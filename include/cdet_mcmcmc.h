@@ -55,7 +55,42 @@ typedef struct cdet_mcmcmc_config {
      * production) in this process, as an interrupted run would; 0 = run to
      * the end.  Used to rehearse kill-and-resume. */
     uint64_t stop_after_steps;
+    /* Histogram seed (MCMCMCConfig histogram_* fields, mcmcmc_sampler.h:92-96;
+     * lattice extents N1, N2 and BETA from the integrand parameters). */
+    int32_t N1;
+    int32_t N2;
+    double beta;                     /* recorded in the histogram export only */
+    int32_t histogram_learning_steps;
+    int32_t histogram_tau_bins;
+    double histogram_uniform_mix;
+    double histogram_pseudo_count;
+    int32_t histogram_learning_only; /* MCMCMC_SEED_HISTOGRAM_LEARNING_ONLY */
+    char histogram_output_prefix[1024]; /* MCMCMC_SEED_HISTOGRAM_OUTPUT_PREFIX ("" = no export) */
+    /* Host threads for the per-chain step work after each batched
+     * evaluation (0 = min(16, hardware threads)); results do not depend on it. */
+    int32_t host_threads;
 } cdet_mcmcmc_config;
+
+/* Fill the reference's MCMCMCConfig defaults (mcmcmc_sampler.h:79-101) plus
+ * N1 = N2 = 1, beta = 1. */
+void cdet_mcmcmc_config_defaults(cdet_mcmcmc_config* config);
+
+/*
+ * Factorized histogram seed (FactorizedHistogramSeed,
+ * include/cdet/mcmcmc_seed.h:63-229): three row-stochastic tables indexed
+ * [parent bin][child bin], row-major, over imaginary-time bins, lattice sites
+ * (site = x * N2 + y) and the two replicas.  Arrays are owned by the caller.
+ */
+typedef struct cdet_histogram_seed {
+    int32_t tau_bins;
+    int32_t N1;
+    int32_t N2;
+    double uniform_mix;     /* learning settings, recorded by the export */
+    double pseudo_count;
+    double* tau_prob;       /* tau_bins * tau_bins */
+    double* space_prob;     /* (N1 * N2) * (N1 * N2) */
+    double replica_prob[4]; /* 2 * 2 */
+} cdet_histogram_seed;
 
 /*
  * Integrand callback: for `count` envelopes packed in h_vertices (envelope c
@@ -82,7 +117,12 @@ typedef struct cdet_mcmcmc_chain_result {
 } cdet_mcmcmc_chain_result;
 
 /*
- * Run all chains (run_mcmcmc_chains_parallel, include/cdet/mcmcmc_sampler.h:1090-1145).
+ * The MCMCMC flow of the reference's main (src/main.cpp:999-1107): validate
+ * the configuration; with seed_mode = CDET_SEED_HISTOGRAM learn the histogram
+ * seed (cdet_histogram_learn), export it when histogram_output_prefix is set,
+ * and stop there when histogram_learning_only is set (results untouched);
+ * then run all chains (run_mcmcmc_chains_parallel,
+ * include/cdet/mcmcmc_sampler.h:1090-1145) with that seed.
  * results: chain_count entries.  ratios: envelope_order + 1 entries,
  * cross-chain order-signal mean ratio to the normalization order.
  * trace (may be NULL): chain_count * trace_stride chosen masks, step k of chain
@@ -91,6 +131,47 @@ typedef struct cdet_mcmcmc_chain_result {
  */
 int cdet_mcmcmc_run(const cdet_mcmcmc_config* config, cdet_batch_eval_fn eval, void* eval_user,
                     cdet_mcmcmc_chain_result* results, double* ratios, uint64_t* trace, int64_t trace_stride);
+
+/*
+ * Run all chains with a given seed kernel (run_mcmcmc_chains_parallel with
+ * seed_kernel, mcmcmc_sampler.h:1090): seed = NULL is the uniform seed, else
+ * the histogram seed.  config->seed_mode is still validated and written to
+ * checkpoints as is, so seed = NULL with seed_mode = histogram is the
+ * reference's MCMCMC_COMPARE_UNIFORM_SEED rerun (src/main.cpp:1109-1120).
+ */
+int cdet_mcmcmc_run_seeded(const cdet_mcmcmc_config* config, const cdet_histogram_seed* seed,
+                           cdet_batch_eval_fn eval, void* eval_user, cdet_mcmcmc_chain_result* results,
+                           double* ratios, uint64_t* trace, int64_t trace_stride);
+
+/*
+ * Learn the histogram seed (learn_factorized_histogram_seed,
+ * include/cdet/mcmcmc_sampler.h:607-671) from histogram_learning_steps
+ * uniform envelopes drawn from one std::mt19937_64 stream, evaluated in
+ * batches through `eval`.  seed->tau_prob and seed->space_prob must hold
+ * tau_bins^2 and (N1*N2)^2 doubles; the scalar fields are set from config.
+ * total_pair_weight (may be NULL) receives the accumulated directed-pair weight.
+ */
+int cdet_histogram_learn(const cdet_mcmcmc_config* config, cdet_batch_eval_fn eval, void* eval_user,
+                         cdet_histogram_seed* seed, double* total_pair_weight);
+
+/* Write <prefix>_tau.csv, _space.csv, _replica.csv and _metadata.txt
+ * (export_factorized_histogram_seed, src/main.cpp:704-767). */
+int cdet_histogram_export(const cdet_histogram_seed* seed, double beta, const char* prefix);
+
+/* Conditional generation probabilities of all 2^E subsets of an envelope
+ * (compute_conditional_generation_probabilities, mcmcmc_sampler.h:252-330),
+ * double recursion; seed = NULL is the uniform seed (all ones). */
+int cdet_seed_conditional_probabilities(const cdet_histogram_seed* seed, const cdet_vertex* vertices,
+                                        int envelope_order, double* out);
+
+/* Draw out[i] = sample_child_from_parent(parents[i]) for i = 0..n-1 from one
+ * std::mt19937_64(rng_seed) stream (mcmcmc_seed.h:51-55, :106-128). */
+int cdet_seed_sample_children(const cdet_histogram_seed* seed, const cdet_vertex* parents, int64_t n,
+                              uint64_t rng_seed, cdet_vertex* out);
+
+/* Density q(child | parent) of the seed kernel (mcmcmc_seed.h:57-60, :130-150). */
+int cdet_seed_density(const cdet_histogram_seed* seed, const cdet_vertex* child, const cdet_vertex* parent,
+                      double* out);
 
 /* Message of the last failing driver call on this host thread. */
 const char* cdet_mcmcmc_last_error(void);

@@ -78,3 +78,31 @@ def test_driver_mfma64_decisions(gpu_available, tmp_path):
     # chains agree at least until the first round-off flip
     assert np.array_equal(t_exact[:, :100], t_mfma[:, :100])
     ctx.close()
+
+
+def test_driver_histogram_seed_gpu_matches_oracle(gpu_available, tmp_path):
+    # Histogram seed: learning over 256 uniform envelopes (two batches of the
+    # GPU path), then chains with the learned kernel.  Tables, chosen subsets
+    # and checkpoints equal the oracle-driven run.
+    ctx = gpu_ctx()
+    gfn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    ofn = ctypes.cast(O.lib().orc_eval_batch_host, ctypes.c_void_p)
+    orc = helpers.oracle_of("c1")
+
+    def cfg(d, **over):
+        return c1_config(d, seed_mode=1, N1=3, N2=3, histogram_learning_steps=256, histogram_tau_bins=8,
+                         lambda_warmup_steps=400, production_steps=600, checkpoint_interval_steps=250, **over)
+
+    print("histogram learn gpu / oracle", flush=True)
+    hg, wg = mcmcmc.learn_histogram(cfg(tmp_path), gfn, ctx.handle)
+    ho, wo = mcmcmc.learn_histogram(cfg(tmp_path), ofn, orc._h)
+    assert np.array_equal(hg.tau, ho.tau) and np.array_equal(hg.space, ho.space)
+    assert np.array_equal(hg.replica, ho.replica) and wg == wo
+    print("histogram driver gpu", flush=True)
+    _, ratg, tg = mcmcmc.run(cfg(tmp_path / "gpu"), gfn, ctx.handle, trace_steps=1000)
+    print("histogram driver oracle", flush=True)
+    _, rato, to = mcmcmc.run(cfg(tmp_path / "orc"), ofn, orc._h, trace_steps=1000)
+    assert np.array_equal(tg, to)
+    assert chk(tmp_path / "gpu") == chk(tmp_path / "orc")
+    assert np.array_equal(ratg[1:], rato[1:], equal_nan=True)
+    ctx.close()

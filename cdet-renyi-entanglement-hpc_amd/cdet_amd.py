@@ -56,7 +56,48 @@ class CdetCounters(ctypes.Structure):
 
 class CdetLattice(ctypes.Structure):
     _fields_ = [("pbc", ctypes.c_int32), ("mask_kind", ctypes.c_int32),
-                ("mask_param", ctypes.c_int32), ("node_chunk", ctypes.c_int32)]
+                ("mask_param", ctypes.c_int32), ("node_chunk", ctypes.c_int32),
+                ("ordinary_layout", ctypes.c_int32)]
+
+
+ORDINARY_RESIDENT, ORDINARY_CHUNKED = 0, 1
+
+
+class ChunkSpec(ctypes.Structure):
+    """cdet_chunk_spec (include/cdet_chunks.h)."""
+
+    _fields_ = [("N1", ctypes.c_int32), ("N2", ctypes.c_int32),
+                ("cheb_nu", ctypes.c_int32), ("cheb_nv", ctypes.c_int32),
+                ("node_chunk_size", ctypes.c_int64), ("sitepair_chunk_size", ctypes.c_int64),
+                ("beta", ctypes.c_double), ("mu", ctypes.c_double),
+                ("alpha_up", ctypes.c_double), ("alpha_down", ctypes.c_double),
+                ("pbc", ctypes.c_int32), ("mask_kind", ctypes.c_int32),
+                ("subregion_square_radius", ctypes.c_int32)]
+
+
+def lattice(pbc=True, mask="first_row", mask_param=0, node_chunk=0, ordinary_layout=ORDINARY_RESIDENT):
+    kind = MASK_KINDS[mask] if isinstance(mask, str) else int(mask)
+    return CdetLattice(pbc=int(pbc), mask_kind=kind, mask_param=mask_param, node_chunk=node_chunk,
+                       ordinary_layout=ordinary_layout)
+
+
+def chunk_spec(params, node_chunk_size, sitepair_chunk_size, pbc=True, mask="first_row", radius=0):
+    kind = MASK_KINDS[mask] if isinstance(mask, str) else int(mask)
+    return ChunkSpec(N1=params.N1, N2=params.N2, cheb_nu=params.cheb_nu, cheb_nv=params.cheb_nv,
+                     node_chunk_size=node_chunk_size, sitepair_chunk_size=sitepair_chunk_size,
+                     beta=params.beta, mu=params.mu, alpha_up=params.alpha_up, alpha_down=params.alpha_down,
+                     pbc=int(pbc), mask_kind=kind, subregion_square_radius=radius)
+
+
+def chunks_write_plan(directory, spec):
+    """Job plans and manifest of a chunk directory (host only)."""
+    _check(lib().cdet_chunks_write_plan(str(directory).encode(), ctypes.byref(spec)))
+
+
+def chunks_read_spec(directory):
+    spec = ChunkSpec()
+    _check(lib().cdet_chunks_read_spec(str(directory).encode(), ctypes.byref(spec)))
+    return spec
 
 
 # Exported symbols and their signatures, as declared in include/cdet_hip.h.
@@ -89,6 +130,13 @@ def _signatures():
         "cdet_ctx_set_batch_mode": (i, [vp, i]),
         "cdet_get_counters": (i, [vp, P(CdetCounters)]),
         "cdet_reset_counters": (i, [vp]),
+        # include/cdet_chunks.h
+        "cdet_chunks_write_plan": (i, [ctypes.c_char_p, P(ChunkSpec)]),
+        "cdet_chunks_read_spec": (i, [ctypes.c_char_p, P(ChunkSpec)]),
+        "cdet_chunks_load_tables": (i, [vp, ctypes.c_char_p, P(CdetLattice)]),
+        "cdet_chunks_transform_samples": (i, [vp, ctypes.c_char_p, P(CdetLattice)]),
+        "cdet_chunks_write_tables": (i, [vp, ctypes.c_char_p, P(ChunkSpec)]),
+        "cdet_chunks_write_samples": (i, [vp, ctypes.c_char_p, P(CdetLattice)]),
     }
 
 
@@ -179,10 +227,29 @@ class Context:
                 self.upload_table(kind, t.upper, t.lower, t.ucorner, t.lcorner)
                 seen[id(t)] = kind
 
-    def build_tables(self, pbc=True, mask="first_row", mask_param=0, node_chunk=0):
-        kind = MASK_KINDS[mask] if isinstance(mask, str) else int(mask)
-        lat = CdetLattice(pbc=int(pbc), mask_kind=kind, mask_param=mask_param, node_chunk=node_chunk)
+    def build_tables(self, pbc=True, mask="first_row", mask_param=0, node_chunk=0,
+                     ordinary_layout=ORDINARY_RESIDENT):
+        lat = lattice(pbc, mask, mask_param, node_chunk, ordinary_layout)
         _check(lib().cdet_tables_build(self._h, ctypes.byref(lat)))
+
+    # -- chunk directories (include/cdet_chunks.h) --------------------------
+    def load_chunks(self, directory, expect=None):
+        """load_chunks workflow: coeffs/ + corners/ streamed into HBM."""
+        _check(lib().cdet_chunks_load_tables(self._h, str(directory).encode(),
+                                             ctypes.byref(expect) if expect is not None else None))
+
+    def transform_chunk_samples(self, directory, expect=None):
+        """Transform stage on the GPU: samples/ + corners/ into HBM tables."""
+        _check(lib().cdet_chunks_transform_samples(self._h, str(directory).encode(),
+                                                   ctypes.byref(expect) if expect is not None else None))
+
+    def write_chunks(self, directory, spec):
+        """Device tables (chunked ordinary layout) to a chunk directory."""
+        _check(lib().cdet_chunks_write_tables(self._h, str(directory).encode(), ctypes.byref(spec)))
+
+    def write_chunk_samples(self, directory, lat):
+        """samples/ + corners/ of every node job, by the GPU producer."""
+        _check(lib().cdet_chunks_write_samples(self._h, str(directory).encode(), ctypes.byref(lat)))
 
     def download_table(self, kind):
         p = self.params

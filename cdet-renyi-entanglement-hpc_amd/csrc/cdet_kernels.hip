@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/cdet_chunks.h"
 #include "../../include/cdet_hip.h"
 #include "../../include/cdet_mcmcmc.h"
 #include "cdet_cdet.hpp"
@@ -828,6 +829,8 @@ struct cdet_ctx {
     size_t d_hout_cap = 0;
     void* d_hstatus = nullptr;
     size_t d_hstatus_cap = 0;
+    // layout of the ordinary kinds now resident (CDET_ORDINARY_*)
+    int ordinary_layout = CDET_ORDINARY_RESIDENT;
 };
 
 namespace {
@@ -1111,9 +1114,11 @@ static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles) {
 
 static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner);
 static int alloc_table(cdet_ctx* ctx, int kind);
+static void invalidate_split(DevTable& t);
 
 #include "cdet_producer.hpp"
 #include "cdet_plan.hpp"
+#include "cdet_chunks.hpp"
 
 // ---------------------------------------------------------------------------
 // C-ABI
@@ -1242,6 +1247,15 @@ static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner) {
     return 0;
 }
 
+// Drop the FP32 split copies derived from a table whose values change.
+static void invalidate_split(DevTable& t) {
+    for (auto& b : t.split)
+        for (float*& f : b) {
+            (void)hipFree(f);
+            f = nullptr;
+        }
+}
+
 static int alloc_table(cdet_ctx* ctx, int kind) {
     DevTable& t = ctx->tab[kind];
     if (t.alias >= 0) {
@@ -1268,11 +1282,8 @@ int cdet_tables_upload(cdet_ctx* ctx, int kind, const double* upper, const doubl
     size_t nc, nk;
     table_sizes(ctx, nc, nk);
     DevTable& t = ctx->tab[kind];
-    for (auto& b : t.split)
-        for (float*& f : b) {
-            (void)hipFree(f);
-            f = nullptr;
-        }
+    invalidate_split(t);
+    if (kind >= CDET_TABLE_ORDINARY_UP) ctx->ordinary_layout = CDET_ORDINARY_RESIDENT;
     HIP_TRY(hipMemcpy(t.upper, upper, nc * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.lower, lower, nc * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.ucorner, ucorner, nk * sizeof(double), hipMemcpyHostToDevice));

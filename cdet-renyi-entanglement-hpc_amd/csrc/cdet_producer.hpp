@@ -368,13 +368,60 @@ bool host_site_in_mask(int x, int y, int N1, int N2, int kind, int param) {
     }
 }
 
-// One table kind: twisted (mask as given) or ordinary (empty mask, replica
-// (0,0) block in site-major order).
-int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const cdet_lattice* lat) {
+// Device inputs of one table kind's producer: eigen-decomposition, sector
+// mixing matrix, Chebyshev nodes and cosine tables.  `masked` applies the
+// subsystem mask (twisted kinds); ordinary kinds use the empty mask.
+struct Producer {
+    int S = 0;
+    double *d_ev = nullptr, *d_V = nullptr, *d_mre = nullptr, *d_mim = nullptr;
+    double *d_nu = nullptr, *d_nv = nullptr, *d_cu = nullptr, *d_cv = nullptr;
+    int32_t* d_err = nullptr;
+    size_t lds = 0;
+    Producer() = default;
+    Producer(const Producer&) = delete;
+    Producer& operator=(const Producer&) = delete;
+    ~Producer() {
+        void* b[] = {d_ev, d_V, d_mre, d_mim, d_nu, d_nv, d_cu, d_cv, d_err};
+        for (void* x : b)
+            if (x) (void)hipFree(x);
+    }
+};
+
+// Chebyshev nodes cos((2i+1)pi/2n) and cosine tables cos(a theta_i), [a][i]
+// (build_chebyshev_node_tables, replica_chebyshev_tables.h:432-471), on the
+// device.  Outputs are set before any allocation can fail, so a caller that
+// frees non-null pointers cleans up after a partial failure.
+int upload_node_tables(const cdet_params& p, double** d_nu, double** d_nv, double** d_cu, double** d_cv) {
+    std::vector<double> nu_nodes(static_cast<size_t>(p.cheb_nu)), nv_nodes(static_cast<size_t>(p.cheb_nv));
+    std::vector<double> cu(static_cast<size_t>(p.cheb_nu) * p.cheb_nu), cv(static_cast<size_t>(p.cheb_nv) * p.cheb_nv);
+    const double pi = M_PI;
+    for (int i = 0; i < p.cheb_nu; ++i) {
+        const double th = (static_cast<double>(2 * i + 1) * pi) / static_cast<double>(2 * p.cheb_nu);
+        nu_nodes[static_cast<size_t>(i)] = std::cos(th);
+        for (int aa = 0; aa < p.cheb_nu; ++aa) cu[static_cast<size_t>(aa) * p.cheb_nu + i] = std::cos(static_cast<double>(aa) * th);
+    }
+    for (int j = 0; j < p.cheb_nv; ++j) {
+        const double th = (static_cast<double>(2 * j + 1) * pi) / static_cast<double>(2 * p.cheb_nv);
+        nv_nodes[static_cast<size_t>(j)] = std::cos(th);
+        for (int b = 0; b < p.cheb_nv; ++b) cv[static_cast<size_t>(b) * p.cheb_nv + j] = std::cos(static_cast<double>(b) * th);
+    }
+    HIP_TRY(hipMalloc(d_nu, sizeof(double) * nu_nodes.size()));
+    HIP_TRY(hipMalloc(d_nv, sizeof(double) * nv_nodes.size()));
+    HIP_TRY(hipMalloc(d_cu, sizeof(double) * cu.size()));
+    HIP_TRY(hipMalloc(d_cv, sizeof(double) * cv.size()));
+    HIP_TRY(hipMemcpy(*d_nu, nu_nodes.data(), sizeof(double) * nu_nodes.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*d_nv, nv_nodes.data(), sizeof(double) * nv_nodes.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*d_cu, cu.data(), sizeof(double) * cu.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*d_cv, cv.data(), sizeof(double) * cv.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int producer_setup(cdet_ctx* ctx, double mu_eff, bool masked, const cdet_lattice* lat, Producer& pr) {
     const cdet_params& p = ctx->p;
     const int N1 = p.N1, N2 = p.N2, S = N1 * N2;
     if (S > kProdMaxSites)
         return fail(CDET_ERR_CAPACITY, "cdet_tables_build: lattices above 64 sites are not supported by this build");
+    pr.S = S;
     // Square-lattice Hamiltonian (replica_free_greens.h:133-169).
     std::vector<double> H(static_cast<size_t>(S) * S, 0.0);
     for (int y = 0; y < N2; ++y)
@@ -397,7 +444,7 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
     // S_0 = V^T Omega_0^{-1} V (Omega_0^{-1} = -i on the subsystem, 1 elsewhere).
     std::vector<double> mre(static_cast<size_t>(S) * S, 0.0), mim(static_cast<size_t>(S) * S, 0.0);
     for (int site = 0; site < S; ++site) {
-        const bool in = !ordinary && host_site_in_mask(site % N1, site / N1, N1, N2, lat->mask_kind, lat->mask_param);
+        const bool in = masked && host_site_in_mask(site % N1, site / N1, N1, N2, lat->mask_kind, lat->mask_param);
         for (int a = 0; a < S; ++a)
             for (int b = 0; b < S; ++b) {
                 const double w = V[static_cast<size_t>(site) * S + a] * V[static_cast<size_t>(site) * S + b];
@@ -407,106 +454,127 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
                     mre[static_cast<size_t>(a) * S + b] += w;
             }
     }
-    std::vector<double> nu_nodes(static_cast<size_t>(p.cheb_nu)), nv_nodes(static_cast<size_t>(p.cheb_nv));
-    std::vector<double> cu(static_cast<size_t>(p.cheb_nu) * p.cheb_nu), cv(static_cast<size_t>(p.cheb_nv) * p.cheb_nv);
-    const double pi = M_PI;
-    for (int i = 0; i < p.cheb_nu; ++i) {
-        const double th = (static_cast<double>(2 * i + 1) * pi) / static_cast<double>(2 * p.cheb_nu);
-        nu_nodes[static_cast<size_t>(i)] = std::cos(th);
-        for (int aa = 0; aa < p.cheb_nu; ++aa) cu[static_cast<size_t>(aa) * p.cheb_nu + i] = std::cos(static_cast<double>(aa) * th);
-    }
-    for (int j = 0; j < p.cheb_nv; ++j) {
-        const double th = (static_cast<double>(2 * j + 1) * pi) / static_cast<double>(2 * p.cheb_nv);
-        nv_nodes[static_cast<size_t>(j)] = std::cos(th);
-        for (int b = 0; b < p.cheb_nv; ++b) cv[static_cast<size_t>(b) * p.cheb_nv + j] = std::cos(static_cast<double>(b) * th);
-    }
-    double *d_ev, *d_V, *d_mre, *d_mim, *d_nu, *d_nv, *d_cu, *d_cv;
-    int32_t* d_err;
-    HIP_TRY(hipMalloc(&d_ev, sizeof(double) * S));
-    HIP_TRY(hipMalloc(&d_V, sizeof(double) * S * S));
-    HIP_TRY(hipMalloc(&d_mre, sizeof(double) * S * S));
-    HIP_TRY(hipMalloc(&d_mim, sizeof(double) * S * S));
-    HIP_TRY(hipMalloc(&d_nu, sizeof(double) * nu_nodes.size()));
-    HIP_TRY(hipMalloc(&d_nv, sizeof(double) * nv_nodes.size()));
-    HIP_TRY(hipMalloc(&d_cu, sizeof(double) * cu.size()));
-    HIP_TRY(hipMalloc(&d_cv, sizeof(double) * cv.size()));
-    HIP_TRY(hipMalloc(&d_err, sizeof(int32_t)));
-    HIP_TRY(hipMemcpy(d_ev, ev.data(), sizeof(double) * S, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_V, V.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_mre, mre.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_mim, mim.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_nu, nu_nodes.data(), sizeof(double) * nu_nodes.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_nv, nv_nodes.data(), sizeof(double) * nv_nodes.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_cu, cu.data(), sizeof(double) * cu.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_cv, cv.data(), sizeof(double) * cv.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(d_err, 0, sizeof(int32_t)));
-
-    int rc = alloc_table(ctx, kind);
+    int rc = upload_node_tables(p, &pr.d_nu, &pr.d_nv, &pr.d_cu, &pr.d_cv);
     if (rc) return rc;
-    DevTable& t = ctx->tab[kind];
-    size_t ncoef, ncorner;
-    table_sizes(ctx, ncoef, ncorner);
-    // cleared on the producer stream so that the clears are ordered before
-    // the node kernels (the context stream does not synchronise with the
-    // null stream)
-    HIP_TRY(hipMemsetAsync(t.upper, 0, ncoef * sizeof(double), ctx->stream));
-    HIP_TRY(hipMemsetAsync(t.lower, 0, ncoef * sizeof(double), ctx->stream));
-    HIP_TRY(hipMemsetAsync(t.ucorner, 0, ncorner * sizeof(double), ctx->stream));
-    HIP_TRY(hipMemsetAsync(t.lcorner, 0, ncorner * sizeof(double), ctx->stream));
-
-    const size_t lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
-    if (lds > 64 * 1024)
+    HIP_TRY(hipMalloc(&pr.d_ev, sizeof(double) * S));
+    HIP_TRY(hipMalloc(&pr.d_V, sizeof(double) * S * S));
+    HIP_TRY(hipMalloc(&pr.d_mre, sizeof(double) * S * S));
+    HIP_TRY(hipMalloc(&pr.d_mim, sizeof(double) * S * S));
+    HIP_TRY(hipMalloc(&pr.d_err, sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(pr.d_ev, ev.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pr.d_V, V.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pr.d_mre, mre.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pr.d_mim, mim.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(pr.d_err, 0, sizeof(int32_t)));
+    pr.lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
+    if (pr.lds > 64 * 1024)
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    const int nodes = p.cheb_nu * p.cheb_nv;
-    const int chunk = lat->node_chunk > 0 ? lat->node_chunk : nodes;
-    const int64_t active = ordinary ? static_cast<int64_t>(S) * S : static_cast<int64_t>(4) * S * S;
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pr.lds)));
+    return 0;
+}
+
+// Samples of one branch at [sitepair][node] in dst (nodes [node_begin,
+// node_end)) and, when corner is not null, the corner values (tau = tau' = 0).
+// replica00: the resident ordinary layout (replica (0,0), site-major).
+int producer_samples(cdet_ctx* ctx, const Producer& pr, int upper, bool replica00, double* dst, double* corner,
+                     int node_begin, int node_end, int node_chunk, hipStream_t s) {
+    const cdet_params& p = ctx->p;
+    ProdArgs a{};
+    a.S = pr.S;
+    a.N1 = p.N1;
+    a.N2 = p.N2;
+    a.nu = p.cheb_nu;
+    a.nv = p.cheb_nv;
+    a.ordinary = replica00 ? 1 : 0;
+    a.upper = upper;
+    a.beta = p.beta;
+    a.evals = pr.d_ev;
+    a.V = pr.d_V;
+    a.mix_re = pr.d_mre;
+    a.mix_im = pr.d_mim;
+    a.node_u = pr.d_nu;
+    a.node_v = pr.d_nv;
+    a.dst = dst;
+    a.corner = corner;
+    a.err = pr.d_err;
+    if (corner) hipLaunchKernelGGL(k_node_green, dim3(1), dim3(256), pr.lds, s, a, 0, 1);
+    const int chunk = node_chunk > 0 ? node_chunk : node_end - node_begin;
+    for (int b0 = node_begin; b0 < node_end; b0 += chunk) {
+        const int cnt = std::min(chunk, node_end - b0);
+        hipLaunchKernelGGL(k_node_green, dim3(cnt), dim3(256), pr.lds, s, a, b0, 0);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// In-place cosine transform of `active` sitepair sample grids at dst.
+int table_dct(cdet_ctx* ctx, const double* d_cu, const double* d_cv, double* dst, int64_t active, hipStream_t s,
+              double** gtmp_keep) {
+    const cdet_params& p = ctx->p;
     const size_t blk = static_cast<size_t>(p.cheb_nu) * p.cheb_nv;
-    double* gtmp = nullptr;
     size_t dct_lds = blk * sizeof(double);
+    double* gtmp = nullptr;
     if (dct_lds > kMaxLds) {
         HIP_TRY(hipMalloc(&gtmp, blk * sizeof(double) * static_cast<size_t>(active)));
+        *gtmp_keep = gtmp;  // freed by the caller after the stream drains
         dct_lds = 0;
     } else if (dct_lds > 64 * 1024) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(dct_lds)));
     }
-    hipStream_t s = ctx->stream;
-    for (int br = 0; br < 2; ++br) {
-        ProdArgs a{};
-        a.S = S;
-        a.N1 = N1;
-        a.N2 = N2;
-        a.nu = p.cheb_nu;
-        a.nv = p.cheb_nv;
-        a.ordinary = ordinary;
-        a.upper = br == 0 ? 1 : 0;
-        a.beta = p.beta;
-        a.evals = d_ev;
-        a.V = d_V;
-        a.mix_re = d_mre;
-        a.mix_im = d_mim;
-        a.node_u = d_nu;
-        a.node_v = d_nv;
-        a.dst = br == 0 ? t.upper : t.lower;
-        a.corner = br == 0 ? t.ucorner : t.lcorner;
-        a.err = d_err;
-        hipLaunchKernelGGL(k_node_green, dim3(1), dim3(256), lds, s, a, 0, 1);
-        for (int b0 = 0; b0 < nodes; b0 += chunk) {
-            const int cnt = std::min(chunk, nodes - b0);
-            hipLaunchKernelGGL(k_node_green, dim3(cnt), dim3(256), lds, s, a, b0, 0);
-        }
-        hipLaunchKernelGGL(k_dct, dim3(static_cast<unsigned>(active)), dim3(256), dct_lds, s, a.dst, p.cheb_nu,
-                           p.cheb_nv, d_cu, d_cv, gtmp);
-        HIP_TRY(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_dct, dim3(static_cast<unsigned>(active)), dim3(256), dct_lds, s, dst, p.cheb_nu, p.cheb_nv,
+                       d_cu, d_cv, gtmp);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int producer_status(const Producer& pr, hipStream_t s) {
     int32_t err = 0;
-    HIP_TRY(hipMemcpyAsync(&err, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&err, pr.d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    void* tmp[] = {d_ev, d_V, d_mre, d_mim, d_nu, d_nv, d_cu, d_cv, d_err, gtmp};
-    for (void* b : tmp)
-        if (b) (void)hipFree(b);
     if (err) return fail(CDET_ERR_SOLVE, "Complex matrix solve is singular within the requested precision");
+    return 0;
+}
+
+// One table kind into the context's device table.  Twisted: mask as given,
+// all replica blocks.  Ordinary: empty mask, either the resident replica-(0,0)
+// site-major layout or the chunked full layout.
+int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const cdet_lattice* lat) {
+    const cdet_params& p = ctx->p;
+    Producer pr;
+    int rc = producer_setup(ctx, mu_eff, !ordinary, lat, pr);
+    if (rc) return rc;
+    rc = alloc_table(ctx, kind);
+    if (rc) return rc;
+    DevTable& t = ctx->tab[kind];
+    invalidate_split(t);
+    size_t ncoef, ncorner;
+    table_sizes(ctx, ncoef, ncorner);
+    hipStream_t s = ctx->stream;
+    // cleared on the producer stream so that the clears are ordered before
+    // the node kernels (the context stream does not synchronise with the
+    // null stream)
+    HIP_TRY(hipMemsetAsync(t.upper, 0, ncoef * sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(t.lower, 0, ncoef * sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(t.ucorner, 0, ncorner * sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(t.lcorner, 0, ncorner * sizeof(double), s));
+    const bool replica00 = ordinary && lat->ordinary_layout == CDET_ORDINARY_RESIDENT;
+    const int S = pr.S;
+    const int64_t active = replica00 ? static_cast<int64_t>(S) * S : static_cast<int64_t>(4) * S * S;
+    const int nodes = p.cheb_nu * p.cheb_nv;
+    double* gtmp[2] = {nullptr, nullptr};
+    for (int br = 0; br < 2 && !rc; ++br) {
+        double* dst = br == 0 ? t.upper : t.lower;
+        rc = producer_samples(ctx, pr, br == 0 ? 1 : 0, replica00, dst, br == 0 ? t.ucorner : t.lcorner, 0, nodes,
+                              lat->node_chunk, s);
+        if (!rc) rc = table_dct(ctx, pr.d_cu, pr.d_cv, dst, active, s, &gtmp[br]);
+    }
+    const int st = producer_status(pr, s);
+    for (double* g : gtmp)
+        if (g) (void)hipFree(g);
+    if (rc) return rc;
+    if (st) return st;
+    if (ordinary) ctx->ordinary_layout = lat->ordinary_layout;
     return 0;
 }
 
@@ -514,6 +582,8 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
 
 static int build_tables_gpu(cdet_ctx* ctx, const cdet_lattice* lattice) {
     if (!ctx || !lattice) return fail(CDET_ERR_ARGUMENT, "cdet_tables_build: bad argument");
+    if (lattice->ordinary_layout != CDET_ORDINARY_RESIDENT && lattice->ordinary_layout != CDET_ORDINARY_CHUNKED)
+        return fail(CDET_ERR_ARGUMENT, "cdet_tables_build: unknown ordinary_layout");
     const cdet_params& p = ctx->p;
     // Twisted tables use mu - alpha_sigma with the subsystem mask, ordinary
     // tables the same mu with the empty mask

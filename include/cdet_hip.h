@@ -32,6 +32,7 @@ extern "C" {
 #define CDET_ERR_DEVICE (-5)        /* HIP runtime error */
 #define CDET_ERR_CAPACITY (-6)      /* order or size beyond the compiled maximum */
 #define CDET_ERR_SOLVE (-7)         /* producer: singular / not-real entry */
+#define CDET_ERR_IO (-8)            /* chunk directory: missing file, bad size or manifest */
 
 /* Largest envelope order accepted by the integrand entry points. */
 #define CDET_MAX_ORDER 14
@@ -139,7 +140,18 @@ typedef struct cdet_lattice {
     int32_t mask_kind;
     int32_t mask_param;           /* origin_square radius / left_columns width */
     int32_t node_chunk;           /* Chebyshev nodes per producer launch; <= 0: all */
+    int32_t ordinary_layout;      /* CDET_ORDINARY_RESIDENT or CDET_ORDINARY_CHUNKED */
 } cdet_lattice;
+
+/* Ordinary-table layouts.  RESIDENT: the replica (0,0) block in site-major
+ * order, as the reference's resident load_data builds it
+ * (include/cdet/load_data.h:39-43, replica_chebyshev_tables.h:697-793).
+ * CHUNKED: every replica block in the consistent sitepair layout, as its
+ * chunked workflow builds it (include/cdet/chunked_load_data.h:594-604).
+ * Lookups read the same index in both; the two agree by the lattice's x<->y
+ * reflection symmetry only. */
+#define CDET_ORDINARY_RESIDENT 0
+#define CDET_ORDINARY_CHUNKED 1
 
 /*
  * Build the twisted and ordinary tables on the GPU in FP64 (the reference's

@@ -933,8 +933,10 @@ static void node_time(double cu, double cv, bool upper, double beta, double& tau
 
 // Twisted table (replica_chebyshev_tables.h:592-695), or the ordinary
 // replica-(0,0) table in site-major order (:697-793) when ordinary != 0.
+// transform = false leaves the node samples untransformed at [sitepair][node]
+// (the chunked workflow's sample stage, chunked_load_data.h:947-996).
 static void build_table(const Calc& c, int nu, int nv, bool ordinary, double* upper, double* lower, double* ucorner,
-                        double* lcorner) {
+                        double* lcorner, bool transform = true) {
     const int S = c.S;
     const size_t SP = static_cast<size_t>(4) * S * S;
     const size_t ncoef = static_cast<size_t>(nu) * nv;
@@ -985,7 +987,8 @@ static void build_table(const Calc& c, int nu, int nv, bool ordinary, double* up
                 fill_site(g0, g1, dst, ncoef, static_cast<size_t>(i * nv + j));
             }
         const size_t active = ordinary ? static_cast<size_t>(S) * S : SP;
-        for (size_t flat = 0; flat < active; ++flat) dct_block(dst + flat * ncoef, t, nu, nv);
+        if (transform)
+            for (size_t flat = 0; flat < active; ++flat) dct_block(dst + flat * ncoef, t, nu, nv);
     }
 }
 
@@ -1169,6 +1172,16 @@ int orc_build_table(void* c, int nu, int nv, int ordinary, double* upper, double
                     double* lcorner) {
     try {
         orc::build_table(*static_cast<orc::Calc*>(c), nu, nv, ordinary != 0, upper, lower, ucorner, lcorner);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+// Node samples [sitepair][node] of all replica blocks, before the transform.
+int orc_build_samples(void* c, int nu, int nv, double* upper, double* lower, double* ucorner, double* lcorner) {
+    try {
+        orc::build_table(*static_cast<orc::Calc*>(c), nu, nv, false, upper, lower, ucorner, lcorner, false);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

@@ -73,6 +73,7 @@ def lib():
         L.orc_calc_mixing.argtypes = [vp, i, dp]
         L.orc_sector_matrix.argtypes = [vp, i, d, d, i, dp]
         L.orc_build_table.argtypes = [vp, i, i, i, dp, dp, dp, dp]
+        L.orc_build_samples.argtypes = [vp, i, i, dp, dp, dp, dp]
         L.orc_chebyshev_probes.argtypes = [ctypes.c_uint64, i, i, dp]
         L.orc_uniform_stream.argtypes = [ctypes.c_uint64, d, d, ctypes.c_int64, dp]
         _lib = L
@@ -143,13 +144,25 @@ class Calc:
         _check(lib().orc_sector_matrix(self._h, m, tau, taup, int(upper), _ptr(out)))
         return out[..., 0] + 1j * out[..., 1]
 
-    def build_table(self, nc, ordinary=False):
+    def build_table(self, nc, ordinary=False, nv=None):
+        nv = nc if nv is None else nv
         SP = 4 * self.S * self.S
-        up = np.zeros(SP * nc * nc)
-        lo = np.zeros(SP * nc * nc)
+        up = np.zeros(SP * nc * nv)
+        lo = np.zeros(SP * nc * nv)
         uc = np.zeros(SP)
         lc = np.zeros(SP)
-        _check(lib().orc_build_table(self._h, nc, nc, int(ordinary), _ptr(up), _ptr(lo), _ptr(uc), _ptr(lc)))
+        _check(lib().orc_build_table(self._h, nc, nv, int(ordinary), _ptr(up), _ptr(lo), _ptr(uc), _ptr(lc)))
+        return Table(up, lo, uc, lc)
+
+    def build_samples(self, nu, nv):
+        """Untransformed node samples [sitepair][node] of all replica blocks
+        (the chunked workflow's sample stage) and the corner values."""
+        SP = 4 * self.S * self.S
+        up = np.zeros(SP * nu * nv)
+        lo = np.zeros(SP * nu * nv)
+        uc = np.zeros(SP)
+        lc = np.zeros(SP)
+        _check(lib().orc_build_samples(self._h, nu, nv, _ptr(up), _ptr(lo), _ptr(uc), _ptr(lc)))
         return Table(up, lo, uc, lc)
 
 

@@ -11,12 +11,18 @@ import cdet_amd
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+HIP_FROM_DRIVER_HEADER = ("cdet_eval_batch_host", "cdet_ctx_set_batch_mode")
+
+
+def functions_in(header):
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", header)).read(), flags=re.S)
+    return set(re.findall(r"\b(cdet_[a-z0-9_]+)\s*\(", text))
+
+
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "cdet_hip.h")).read()
-    text += "\n".join(l for l in open(os.path.join(ROOT, "include", "cdet_mcmcmc.h")).read().splitlines()
-                      if "cdet_eval_batch_host" in l or "cdet_ctx_set_batch_mode" in l)
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(cdet_[a-z0-9_]+)\s*\(", text)))
+    names = functions_in("cdet_hip.h") | functions_in("cdet_chunks.h")
+    names |= set(HIP_FROM_DRIVER_HEADER)
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -45,5 +51,33 @@ def test_params_derive_is_host_only():
 def test_driver_library_exports():
     import mcmcmc
     lib = ctypes.CDLL(mcmcmc.LIB_PATH)
-    for name in ("cdet_mcmcmc_run", "cdet_mcmcmc_last_error"):
-        assert hasattr(lib, name)
+    names = functions_in("cdet_mcmcmc.h") - set(HIP_FROM_DRIVER_HEADER) - {"cdet_batch_eval_fn"}
+    assert len(names) >= 9
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_headers(tmp_path):
+    """sizeof/offsetof of every ctypes mirror against the C compiler's."""
+    import subprocess
+    import mcmcmc
+    mirrors = {"cdet_params": cdet_amd.CdetParams, "cdet_lattice": cdet_amd.CdetLattice,
+               "cdet_chunk_spec": cdet_amd.ChunkSpec, "cdet_mcmcmc_config": mcmcmc.MCMCMCConfig,
+               "cdet_mcmcmc_chain_result": mcmcmc.ChainResult, "cdet_histogram_seed": mcmcmc.HistogramSeed}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cdet_chunks.h"', '#include "cdet_mcmcmc.h"',
+             "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for cname, cls in mirrors.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)

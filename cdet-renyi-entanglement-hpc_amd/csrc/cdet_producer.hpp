@@ -263,6 +263,262 @@ __global__ __launch_bounds__(256) void k_node_green(ProdArgs a, int node_begin, 
     }
 }
 
+// Lattices of 65..100 sites: the S x S complex matrix alone fills the LDS, so
+// the solve is an in-place Gauss-Jordan inversion with partial pivoting
+// (row swaps recorded, columns unscrambled in reverse at the end), and the two
+// basis rotations stage their S x S products through a per-workgroup slot of
+// global scratch.  One workgroup per CU; each loops over nodes.
+//   K = diag(left) M^{-1} diag(e),  T = V K  (to scratch, then back to LDS),
+//   G = T V^T (to scratch), then the replica entries as in k_node_green.
+constexpr int kProdBigMaxSites = 100;
+constexpr int kProdBigThreads = 1024;
+
+__device__ inline void big_pivot_reduce(double& best, int& brow, double* red_v, int* red_i) {
+    // wave maximum, larger value first, then smaller row
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off);
+        const int orow = __shfl_xor(brow, off);
+        if (ov > best || (ov == best && orow < brow)) {
+            best = ov;
+            brow = orow;
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red_v[w] = best;
+        red_i[w] = brow;
+    }
+}
+
+__global__ __launch_bounds__(kProdBigThreads) void k_node_green_big(ProdArgs a, int node_begin, int node_end,
+                                                                    int corner_mode, double* __restrict__ tscr) {
+    extern __shared__ double sh[];
+    const int S = a.S;
+    const int S2 = S * S;
+    double* Ar = sh;
+    double* Ai = Ar + S2;
+    double* colr = Ai + S2;  // pivot column snapshot; later left[] and e[]
+    double* coli = colr + S;
+    double* red_v = coli + S;                                  // one per wave
+    int* red_i = reinterpret_cast<int*>(red_v + kProdBigThreads / 64);
+    int* perm = red_i + kProdBigThreads / 64;
+    int* s_piv = perm + S;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int nw = nt >> 6;
+    const bool upper = a.upper != 0;
+    double* tr_g = tscr + static_cast<int64_t>(blockIdx.x) * 2 * S2;
+    double* ti_g = tr_g + S2;
+    const int last = corner_mode ? 1 : node_end;
+    for (int node = corner_mode ? 0 : node_begin + blockIdx.x; node < last; node += gridDim.x) {
+        double tau = 0.0, taup = 0.0;
+        if (!corner_mode) {
+            const int i = node / a.nv;
+            const int j = node - i * a.nv;
+            const double cu = a.node_u[i];
+            const double cv = a.node_v[j];
+            if (upper) {
+                tau = a.beta * (1.0 + cu) / 2.0;
+                taup = tau * (1.0 + cv) / 2.0;
+            } else {
+                taup = a.beta * (1.0 + cu) / 2.0;
+                tau = taup * (1.0 + cv) / 2.0;
+            }
+        }
+        const double tf = tau / a.beta;
+        const double tpf = taup / a.beta;
+        const double theta = upper ? (tf + tpf) / 2.0 : 1.0 - (tf + tpf) / 2.0;
+        // middle (replica_free_greens.h:445-476)
+        for (int e = tid; e < S2; e += nt) {
+            const int r = e / S;
+            const int c = e - r * S;
+            const double xr = a.beta * a.evals[r];
+            const double xc = a.beta * a.evals[c];
+            const double sxr = upper ? xr : -xr;
+            const double sxc = upper ? xc : -xc;
+            const double cr = fmax(0.0, sxr);
+            const double cc = fmax(0.0, sxc);
+            const double rs = exp(-theta * cr);
+            const double cs = exp(-(1.0 - theta) * cc);
+            double mr, mi;
+            if (upper) {
+                mr = a.mix_re[r * S + c];
+                mi = a.mix_im[r * S + c];
+            } else {
+                mr = a.mix_re[c * S + r];
+                mi = -a.mix_im[c * S + r];
+            }
+            Ar[e] = (rs * mr) * cs;
+            Ai[e] = (rs * mi) * cs;
+            if (r == c) Ar[e] += exp(sxr - cr);
+        }
+        __syncthreads();
+        // in-place inversion
+        for (int col = 0; col < S; ++col) {
+            double best = -1.0;
+            int brow = S;
+            for (int r = col + tid; r < S; r += nt) {
+                const double v = sqrt(Ar[r * S + col] * Ar[r * S + col] + Ai[r * S + col] * Ai[r * S + col]);
+                if (v > best) {
+                    best = v;
+                    brow = r;
+                }
+            }
+            big_pivot_reduce(best, brow, red_v, red_i);
+            __syncthreads();
+            if (tid == 0) {
+                double pb = red_v[0];
+                int pr = red_i[0];
+                for (int w = 1; w < nw; ++w)
+                    if (red_v[w] > pb || (red_v[w] == pb && red_i[w] < pr)) {
+                        pb = red_v[w];
+                        pr = red_i[w];
+                    }
+                if (pb <= 1e-12 || pr >= S) {
+                    atomicOr(a.err, 1);
+                    pr = col;
+                }
+                *s_piv = pr;
+                perm[col] = pr;
+            }
+            __syncthreads();
+            const int prow = *s_piv;
+            if (prow != col) {
+                for (int j = tid; j < S; j += nt) {
+                    double t;
+                    t = Ar[col * S + j]; Ar[col * S + j] = Ar[prow * S + j]; Ar[prow * S + j] = t;
+                    t = Ai[col * S + j]; Ai[col * S + j] = Ai[prow * S + j]; Ai[prow * S + j] = t;
+                }
+                __syncthreads();
+            }
+            const double pr_ = Ar[col * S + col];
+            const double pi_ = Ai[col * S + col];
+            __syncthreads();
+            // pivot row / pivot, with the pivot entry replaced by 1 first
+            for (int j = tid; j < S; j += nt) {
+                const double xr = j == col ? 1.0 : Ar[col * S + j];
+                const double xi = j == col ? 0.0 : Ai[col * S + j];
+                double r, i;
+                cdiv_(xr, xi, pr_, pi_, r, i);
+                Ar[col * S + j] = r;
+                Ai[col * S + j] = i;
+            }
+            for (int r = tid; r < S; r += nt) {
+                colr[r] = Ar[r * S + col];
+                coli[r] = Ai[r * S + col];
+            }
+            __syncthreads();
+            for (int e = tid; e < S2; e += nt) {
+                const int r = e / S;
+                if (r == col) continue;
+                const int j = e - r * S;
+                const double fr = colr[r], fi = coli[r];
+                double pr2, pi2;
+                cmul_(fr, fi, Ar[col * S + j], Ai[col * S + j], pr2, pi2);
+                Ar[e] = (j == col ? 0.0 : Ar[e]) - pr2;
+                Ai[e] = (j == col ? 0.0 : Ai[e]) - pi2;
+            }
+            __syncthreads();
+        }
+        for (int k = S - 1; k >= 0; --k) {
+            const int q = perm[k];
+            if (q != k) {
+                for (int r = tid; r < S; r += nt) {
+                    double t;
+                    t = Ar[r * S + k]; Ar[r * S + k] = Ar[r * S + q]; Ar[r * S + q] = t;
+                    t = Ai[r * S + k]; Ai[r * S + k] = Ai[r * S + q]; Ai[r * S + q] = t;
+                }
+                __syncthreads();
+            }
+        }
+        // left factor and diagonal right-hand side (replica_free_greens.h:477-496)
+        for (int r = tid; r < S; r += nt) {
+            const double x = a.beta * a.evals[r];
+            const double sx = upper ? x : -x;
+            const double c = fmax(0.0, sx);
+            double le, re;
+            if (upper) {
+                le = (1.0 - tf) * x - (1.0 - theta) * c;
+                re = tpf * x - theta * c;
+            } else {
+                le = tf * sx - (1.0 - theta) * c;
+                re = (1.0 - tpf) * sx - theta * c;
+            }
+            colr[r] = upper ? exp(le) : -exp(le);
+            coli[r] = exp(re);
+        }
+        __syncthreads();
+        // T = V K with K[p][q] = (M^{-1}[p][q] e[q]) left[p], to scratch
+        for (int e = tid; e < S2; e += nt) {
+            const int i = e / S;
+            const int q = e - i * S;
+            const double eq = coli[q];
+            double tr = 0.0, ti = 0.0;
+            for (int p = 0; p < S; ++p) {
+                const double v = a.V[i * S + p];
+                const double kr = (Ar[p * S + q] * eq) * colr[p];
+                const double ki = (Ai[p * S + q] * eq) * colr[p];
+                tr = __builtin_fma(v, kr, tr);
+                ti = __builtin_fma(v, ki, ti);
+            }
+            tr_g[e] = tr;
+            ti_g[e] = ti;
+        }
+        __syncthreads();
+        for (int e = tid; e < S2; e += nt) {
+            Ar[e] = tr_g[e];
+            Ai[e] = ti_g[e];
+        }
+        __syncthreads();
+        // G = T V^T, to scratch
+        for (int e = tid; e < S2; e += nt) {
+            const int i = e / S;
+            const int jx = e - i * S;
+            double gr = 0.0, gi = 0.0;
+            for (int q = 0; q < S; ++q) {
+                const double v = a.V[jx * S + q];
+                gr = __builtin_fma(Ar[i * S + q], v, gr);
+                gi = __builtin_fma(Ai[i * S + q], v, gi);
+            }
+            tr_g[e] = gr;
+            ti_g[e] = gi;
+        }
+        __syncthreads();
+        const int64_t stride = corner_mode ? 1 : static_cast<int64_t>(a.nu) * a.nv;
+        double* out = corner_mode ? a.corner : a.dst + node;
+        if (a.ordinary) {
+            for (int e = tid; e < S2; e += nt) out[static_cast<int64_t>(e) * stride] = tr_g[e];
+        } else {
+            const int SP = 4 * S2;
+            for (int f = tid; f < SP; f += nt) {
+                int g = f;
+                const int y2 = g % a.N2;
+                g /= a.N2;
+                const int x2 = g % a.N1;
+                g /= a.N1;
+                const int y1 = g % a.N2;
+                g /= a.N2;
+                const int x1 = g % a.N1;
+                g /= a.N1;
+                const int rb = g & 1;
+                const int ra = g >> 1;
+                const int s1 = y1 * a.N1 + x1;
+                const int s2 = y2 * a.N1 + x2;
+                const double re = tr_g[s1 * S + s2];
+                const double im = ti_g[s1 * S + s2];
+                out[static_cast<int64_t>(f) * stride] = (ra == rb) ? re : (ra == 0 ? -im : im);
+            }
+        }
+        // the scratch slot and the LDS are reused by the next node
+        __syncthreads();
+    }
+}
+
+size_t big_lds_bytes(int S) {
+    return sizeof(double) * (2 * static_cast<size_t>(S) * S + 2 * S + kProdBigThreads / 64) +
+           sizeof(int) * (kProdBigThreads / 64 + S + 1);
+}
+
 // In-place separable cosine transform of one sitepair's nu x nv sample grid.
 __global__ __launch_bounds__(256) void k_dct(double* __restrict__ tab, int nu, int nv, const double* __restrict__ cos_u,
                                              const double* __restrict__ cos_v, double* __restrict__ gtmp) {
@@ -377,11 +633,14 @@ struct Producer {
     double *d_nu = nullptr, *d_nv = nullptr, *d_cu = nullptr, *d_cv = nullptr;
     int32_t* d_err = nullptr;
     size_t lds = 0;
+    bool big = false;          // k_node_green_big (S > 64)
+    int grid = 0;              // its persistent grid
+    double* d_tscr = nullptr;  // its scratch: grid slots of 2 S^2 doubles
     Producer() = default;
     Producer(const Producer&) = delete;
     Producer& operator=(const Producer&) = delete;
     ~Producer() {
-        void* b[] = {d_ev, d_V, d_mre, d_mim, d_nu, d_nv, d_cu, d_cv, d_err};
+        void* b[] = {d_ev, d_V, d_mre, d_mim, d_nu, d_nv, d_cu, d_cv, d_err, d_tscr};
         for (void* x : b)
             if (x) (void)hipFree(x);
     }
@@ -419,8 +678,8 @@ int upload_node_tables(const cdet_params& p, double** d_nu, double** d_nv, doubl
 int producer_setup(cdet_ctx* ctx, double mu_eff, bool masked, const cdet_lattice* lat, Producer& pr) {
     const cdet_params& p = ctx->p;
     const int N1 = p.N1, N2 = p.N2, S = N1 * N2;
-    if (S > kProdMaxSites)
-        return fail(CDET_ERR_CAPACITY, "cdet_tables_build: lattices above 64 sites are not supported by this build");
+    if (S > kProdBigMaxSites)
+        return fail(CDET_ERR_CAPACITY, "cdet_tables_build: lattices above 100 sites are not supported by this build");
     pr.S = S;
     // Square-lattice Hamiltonian (replica_free_greens.h:133-169).
     std::vector<double> H(static_cast<size_t>(S) * S, 0.0);
@@ -466,10 +725,21 @@ int producer_setup(cdet_ctx* ctx, double mu_eff, bool masked, const cdet_lattice
     HIP_TRY(hipMemcpy(pr.d_mre, mre.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(pr.d_mim, mim.data(), sizeof(double) * S * S, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(pr.d_err, 0, sizeof(int32_t)));
-    pr.lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
-    if (pr.lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green),
+    if (S <= kProdMaxSites) {
+        pr.lds = sizeof(double) * (4 * static_cast<size_t>(S) * S + 3 * S);
+        if (pr.lds > 64 * 1024)
+            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pr.lds)));
+    } else {
+        pr.big = true;
+        pr.lds = big_lds_bytes(S);
+        if (pr.lds > static_cast<size_t>(160 * 1024))
+            return fail(CDET_ERR_CAPACITY, "cdet_tables_build: producer LDS exceeds 160 KiB");
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_node_green_big),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pr.lds)));
+        pr.grid = std::max(1, ctx->ncu);
+        HIP_TRY(hipMalloc(&pr.d_tscr, sizeof(double) * 2 * static_cast<size_t>(S) * S * pr.grid));
+    }
     return 0;
 }
 
@@ -497,11 +767,21 @@ int producer_samples(cdet_ctx* ctx, const Producer& pr, int upper, bool replica0
     a.dst = dst;
     a.corner = corner;
     a.err = pr.d_err;
-    if (corner) hipLaunchKernelGGL(k_node_green, dim3(1), dim3(256), pr.lds, s, a, 0, 1);
     const int chunk = node_chunk > 0 ? node_chunk : node_end - node_begin;
-    for (int b0 = node_begin; b0 < node_end; b0 += chunk) {
-        const int cnt = std::min(chunk, node_end - b0);
-        hipLaunchKernelGGL(k_node_green, dim3(cnt), dim3(256), pr.lds, s, a, b0, 0);
+    if (pr.big) {
+        if (corner)
+            hipLaunchKernelGGL(k_node_green_big, dim3(1), dim3(kProdBigThreads), pr.lds, s, a, 0, 1, 1, pr.d_tscr);
+        for (int b0 = node_begin; b0 < node_end; b0 += chunk) {
+            const int cnt = std::min(chunk, node_end - b0);
+            hipLaunchKernelGGL(k_node_green_big, dim3(std::min(cnt, pr.grid)), dim3(kProdBigThreads), pr.lds, s, a,
+                               b0, b0 + cnt, 0, pr.d_tscr);
+        }
+    } else {
+        if (corner) hipLaunchKernelGGL(k_node_green, dim3(1), dim3(256), pr.lds, s, a, 0, 1);
+        for (int b0 = node_begin; b0 < node_end; b0 += chunk) {
+            const int cnt = std::min(chunk, node_end - b0);
+            hipLaunchKernelGGL(k_node_green, dim3(cnt), dim3(256), pr.lds, s, a, b0, 0);
+        }
     }
     HIP_TRY(hipGetLastError());
     return 0;

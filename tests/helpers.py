@@ -24,6 +24,13 @@ SETS = {
                  mask="origin_square", mask_param=2),
     "c2": dict(N1=4, N2=4, beta=6.0, U=2.0, mu=1.0, alpha_up=1.0, alpha_down=1.0, nc=96,
                mask="origin_square", mask_param=1),
+    # lattices above 64 sites (the inversion producer), table-level coverage
+    # at a small coefficient count: 72 sites with distinct spin shifts, and the
+    # 10x10 C5 lattice with its x < 5 half
+    "p9x8": dict(N1=9, N2=8, beta=5.0, U=2.0, mu=0.7, alpha_up=1.1, alpha_down=0.9, nc=4,
+                 mask="left_columns", mask_param=4),
+    "p10x10": dict(N1=10, N2=10, beta=10.0, U=2.0, mu=1.0, alpha_up=1.0, alpha_down=1.0, nc=4,
+                   mask="left_columns", mask_param=5),
 }
 
 

@@ -11,7 +11,7 @@ import helpers
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["ref2x2", "ref2x2_eq", "c1", "p8x8"])
+@pytest.mark.parametrize("name", ["ref2x2", "ref2x2_eq", "c1", "p8x8", "p9x8", "p10x10"])
 def test_gpu_tables_match_oracle(gpu_available, name):
     s = helpers.SETS[name]
     p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"])
@@ -30,7 +30,7 @@ def test_gpu_tables_match_oracle(gpu_available, name):
     # integrand on producer tables agrees with the oracle on oracle tables
     # (the 64-site lattice is checked at the table level only: its 16-term
     # expansion is a coverage case for the producer, not an integrand workload)
-    if name != "p8x8":
+    if not name.startswith("p"):
         orc = helpers.oracle_of(name)
         for v in helpers.seeded_configs(99, [3, 5, 6]):
             got = ctx.evaluate_all_subset_integrands(v)
@@ -40,7 +40,7 @@ def test_gpu_tables_match_oracle(gpu_available, name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["c1", "c1_neq"])
+@pytest.mark.parametrize("name", ["c1", "c1_neq", "p9x8"])
 def test_gpu_tables_bitwise_reproducible(gpu_available, name):
     # Two builds of the same tables on one device are bit-identical (the node
     # kernels and the transform have no order-dependent arithmetic).

@@ -56,9 +56,10 @@ def parse():
                     help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: order-10 integrand batch (headline); c3: 2^20 table lookups on the 8x8 lattice; "
-                         "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice")
+                         "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice; "
+                         "c5: 1024 configurations at orders 2..14 on the 10x10 lattice")
     ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
     return ap.parse_args()
 
@@ -331,8 +332,94 @@ def main_c4(args):
     ctx.close()
 
 
+C5 = dict(N1=10, N2=10, beta=10.0, U=2.0, mu=1.0, alpha=1.0, nc=192, mask="left_columns", width=5,
+          configs=1024, order_min=2, order_max=14, base_seed=2026101401, tag=0xC5)
+
+
+def main_c5(args):
+    """Largest case (configuration C5): tables built on the GPU in uneven node
+    chunks (7 Nc + 3 nodes per launch) for the 10x10 lattice, then one batch of
+    1024 seeded configurations at orders 2 + floor(13 u^2) (2..14) evaluated as
+    all-subset integrands, vertices resident in HBM, graph replays; FP64
+    matrix-core and bit-identical FP64 contraction timed side by side."""
+    import torch
+    import cdet_amd
+    import rng
+    w = C5
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    params = cdet_amd.make_params(w["N1"], w["N2"], w["nc"], w["beta"], w["U"], w["mu"], w["alpha"], w["alpha"])
+    ctx = cdet_amd.Context(params)
+    print("c5: building tables", file=sys.stderr, flush=True)
+    t0 = time.time()
+    ctx.build_tables(pbc=True, mask=w["mask"], mask_param=w["width"], node_chunk=7 * w["nc"] + 3)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    print(f"c5: tables built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    orders, verts = rng.synthetic_configs(w["base_seed"], w["tag"], w["configs"], w["order_min"], w["order_max"],
+                                          squared=True)
+    d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device=dev)
+    sptr = torch.cuda.current_stream().cuda_stream
+    res = {}
+    outs = {}
+    for mode in ("exact", "mfma64"):
+        plan = ctx.plan(orders, mode=mode)
+        d_o = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
+        plan.run(d_v, d_o, None, use_graph=False, stream=sptr)
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        stage = np.array(plan.profile(d_v, d_o, None, stream=sptr))
+        res[mode] = dict(ms=ms, stage_ms=[round(float(x), 4) for x in stage], lookups=int(plan.query_count))
+        outs[mode] = d_o.cpu().numpy()
+        print(f"c5: {mode} {ms:.3f} ms per batch, stages {stage}", file=sys.stderr, flush=True)
+        plan.close()
+    # FP64 matrix-core values against the bit-identical form, per order
+    _, ob = cdet_amd.packed_offsets(orders)
+    per_order = {}
+    for n in range(w["order_min"], w["order_max"] + 1):
+        rel = []
+        for i in np.nonzero(orders == n)[0]:
+            ref = outs["exact"][ob[i]:ob[i + 1]]
+            got = outs["mfma64"][ob[i]:ob[i + 1]]
+            big = np.abs(ref) >= 1e-6 * np.abs(ref).max()
+            if np.any(big):
+                rel.append(float(np.max(np.abs(got - ref)[big] / np.abs(ref)[big])))
+        if rel:
+            per_order[str(n)] = {"configs": len(rel), "max_rel_err": max(rel)}
+    # one lookup reads one nu x nv coefficient block per table (tables are
+    # shared by the two spins here); upper bound on the contraction's bytes
+    block = w["nc"] * w["nc"] * 8
+    cms = res["mfma64"]["stage_ms"][1]
+    line = {
+        "metric": "C5 CDet all-subset integrand evals/s (10x10, beta=10, Nc=192, 1024 configs, orders 2..14)",
+        "value": round(w["configs"] / (res["mfma64"]["ms"] * 1e-3), 1), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["mfma64"]["ms"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "10x10 Hubbard, beta=10, x<5 half subsystem, 192x192 Chebyshev; 1024 seeded "
+                               "configurations, orders 2 + floor(13 u^2), all-subset integrands",
+                   "table_build_s": round(build_s, 2), "node_chunk": 7 * w["nc"] + 3},
+        "exact_fp64_evals_per_s": round(w["configs"] / (res["exact"]["ms"] * 1e-3), 1),
+        "stages_ms": {m: res[m]["stage_ms"] for m in res},
+        "lookups_per_batch": res["mfma64"]["lookups"],
+        "contraction_block_read_upper_bound_gbs": round(res["mfma64"]["lookups"] * block / (cms * 1e-3) / 1e9, 1),
+        "orders": {str(n): int(np.count_nonzero(orders == n)) for n in range(w["order_min"], w["order_max"] + 1)},
+        "mfma64_vs_exact": per_order,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
+    if args.gpus == 1 and args.workload == "c5":
+        return main_c5(args)
     if args.gpus == 1 and args.workload == "c3":
         return main_c3(args)
     if args.gpus == 1 and args.workload == "c4":

@@ -74,15 +74,17 @@ class MT19937_64:
         return u * (b - a) + a
 
 
-def synthetic_configs(base_seed, tag, count, order_min=1, order_max=10):
+def synthetic_configs(base_seed, tag, count, order_min=1, order_max=10, squared=False):
     """Seeded configurations: configuration i draws from
-    mt19937_64(derive_mcmcmc_seed(base_seed, tag, i)) its order
-    order_min + floor((order_max - order_min + 1) * U[0,1)) and then, per
-    vertex, (tau, x, y, replica) ~ U[0,1)^4."""
+    mt19937_64(derive_mcmcmc_seed(base_seed, tag, i)) a uniform u, its order
+    order_min + floor((order_max - order_min + 1) * u) (u^2 in place of u when
+    `squared`, which makes high orders rare), and then, per vertex,
+    (tau, x, y, replica) ~ U[0,1)^4."""
     seeds = [derive_mcmcmc_seed(base_seed, tag, i) for i in range(count)]
     eng = MT19937_64(seeds)
     span = order_max - order_min + 1
-    orders = order_min + np.floor(span * eng.uniform()).astype(np.int64)
+    u = eng.uniform()
+    orders = order_min + np.floor(span * (u * u if squared else u)).astype(np.int64)
     orders = np.minimum(orders, order_max)
     draws = np.stack([eng.uniform() for _ in range(4 * order_max)], axis=1)
     verts = [draws[i, :4 * orders[i]].reshape(orders[i], 4) for i in range(count)]

@@ -61,6 +61,7 @@ def parse():
                          "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice; "
                          "c5: 1024 configurations at orders 2..14 on the 10x10 lattice")
     ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
+    ap.add_argument("--c5-sweeps", type=int, default=200, help="c5: lockstep sweeps of the 1024 chains (0: none)")
     return ap.parse_args()
 
 
@@ -260,12 +261,38 @@ C4 = dict(N1=6, N2=6, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="left_col
           chains=256, envelope=12, base_seed=2026101401)
 
 
+def run_sweeps(ctx, w, envelopes, sweeps, order_max):
+    """Lockstep MCMCMC sweeps of len(envelopes) chains through the GPU batch
+    callback, uniform seed.  Chain c runs at envelope max(envelopes[c], 3) and
+    measures orders 2..envelope-1, normalised to order 2: at mu - alpha = 0 the
+    order-1 integrand vanishes identically, so an envelope-2 chain would have
+    no nonzero weight (the sampler stops with "MCMCMC local normalization is
+    zero or non-finite") and an order-1 normalisation would divide by zero.
+    Returns (seconds, ratios, warmup sweeps)."""
+    import ctypes
+    import cdet_amd
+    import mcmcmc
+    envelopes = np.maximum(np.asarray(envelopes), 3)
+    c = mcmcmc.MCMCMCConfig()
+    c.order_min, c.order_max, c.envelope_order, c.normalization_order = 2, order_max, order_max + 1, 2
+    for o in range(2, order_max + 1):
+        c.lambdas[o] = 1.0
+    warm = max(1, sweeps // 10)
+    c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = warm, max(1, warm // 5), 0.25
+    c.production_steps, c.chain_count, c.base_seed = sweeps - warm, len(envelopes), w["base_seed"]
+    c.seed_mode, c.checkpoint_enabled, c.resume_mode = 0, 0, 0
+    fn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    t = time.perf_counter()
+    _, ratios, _ = mcmcmc.run_envelopes(c, envelopes, fn, ctx.handle)
+    return time.perf_counter() - t, ratios, warm
+
+
 def main_c4(args):
     """Batched MCMCMC sweeps (configuration C4): 256 chains advanced in lockstep
-    by the C++ driver, every chain's all-subset envelope evaluated in one GPU
-    batch per sweep (host buffers through cdet_eval_batch_host), the heat-bath
-    steps on the host.  All chains use envelope order 12 (orders 1..11), the
-    top of C4's per-chain range."""
+    by the C++ driver, chain c at envelope E_c = 2 + floor(11 u^2) (u from
+    mt19937_64(derive_mcmcmc_seed(2026101401, 0xC4, c)), orders 1..E_c-1), every
+    chain's all-subset envelope evaluated in one GPU batch per sweep (host
+    buffers through cdet_eval_batch_host), the heat-bath steps on host threads."""
     import ctypes
     import torch
     import cdet_amd
@@ -284,20 +311,9 @@ def main_c4(args):
     mode = "mfma64" if args.mode == "split32" else args.mode
     cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[mode])
     E = w["envelope"]
-    c = mcmcmc.MCMCMCConfig()
-    c.order_min, c.order_max, c.envelope_order, c.normalization_order = 1, E - 1, E, 1
-    for o in range(1, E):
-        c.lambdas[o] = 1.0
-    warm = max(1, args.sweeps // 10)
-    c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = warm, max(1, warm // 5), 0.25
-    c.production_steps, c.chain_count, c.base_seed = args.sweeps - warm, w["chains"], w["base_seed"]
-    c.seed_mode, c.checkpoint_enabled, c.resume_mode = 0, 0, 0
-    c.checkpoint_dir = b"/tmp/cdet_c4_unused"
-    fn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
     # GPU batch alone: one envelope batch of all chains, graph replays
     import rng
-    orders = [E] * w["chains"]
-    _, verts = rng.synthetic_configs(w["base_seed"], 0xC4, w["chains"], E, E)
+    orders, verts = rng.synthetic_configs(w["base_seed"], 0xC4, w["chains"], 2, E, squared=True)
     plan = ctx.plan(orders, mode=mode)
     d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device=dev)
     d_o = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
@@ -311,22 +327,24 @@ def main_c4(args):
     torch.cuda.synchronize()
     batch_ms = (time.perf_counter() - t) / 20 * 1e3
     plan.close()
-    print(f"c4: GPU batch of {w['chains']} order-{E} envelopes {batch_ms:.3f} ms", file=sys.stderr, flush=True)
-    t = time.perf_counter()
-    results, ratios, _ = mcmcmc.run(c, fn, ctx.handle)
-    dt = time.perf_counter() - t
+    print(f"c4: GPU batch of {w['chains']} envelopes (orders 2..{E}) {batch_ms:.3f} ms", file=sys.stderr,
+          flush=True)
+    dt, ratios, warm = run_sweeps(ctx, w, orders, args.sweeps, E - 1)
     sweeps = args.sweeps
     line = {
-        "metric": "C4 MCMCMC sweeps/s (6x6, beta=8, Nc=128, 256 chains, envelope 12, lockstep batches)",
+        "metric": "C4 MCMCMC sweeps/s (6x6, beta=8, Nc=128, 256 chains, envelopes 2..12, lockstep batches)",
         "value": round(sweeps / dt, 2), "unit": "sweeps/s", "n_gpus": 1, "steps": sweeps, "warmup": warm,
         "ms_per_step": round(dt / sweeps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "6x6 Hubbard, beta=8, x<3 half-strip subsystem, 128x128 Chebyshev; 256 chains, "
-                               "orders 1..11 in envelope 12, uniform seed, one GPU batch per sweep",
+                               "envelope max(3, 2 + floor(11 u^2)) per chain, orders 2..envelope-1, "
+                               "uniform seed, one GPU batch per sweep",
+                   "envelopes": {str(n): int(np.count_nonzero(orders == n)) for n in range(2, E + 1)},
                    "mode": mode, "table_build_s": round(build_s, 2)},
         "envelope_evals_per_s": round(sweeps * w["chains"] / dt, 1),
         "gpu_batch_ms": round(batch_ms, 4),
-        "ratio_order2_over_order1": (float(ratios[2]) if len(ratios) > 2 and np.isfinite(ratios[2]) else None),
+        # odd orders vanish at half filling; order 4 is the first nonzero ratio
+        "ratio_order4_over_order2": (float(ratios[4]) if len(ratios) > 4 and np.isfinite(ratios[4]) else None),
     }
     print(json.dumps(line), flush=True)
     ctx.close()
@@ -393,6 +411,13 @@ def main_c5(args):
                 rel.append(float(np.max(np.abs(got - ref)[big] / np.abs(ref)[big])))
         if rel:
             per_order[str(n)] = {"configs": len(rel), "max_rel_err": max(rel)}
+    # lockstep sweeps of 1024 chains at envelopes E_c (orders 1..E_c-1)
+    ctx_mode = "mfma64"
+    cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[ctx_mode])
+    sweep_s, sweep_ratios = None, None
+    if args.c5_sweeps > 0:
+        print(f"c5: {args.c5_sweeps} sweeps of {w['configs']} chains", file=sys.stderr, flush=True)
+        sweep_s, sweep_ratios, _ = run_sweeps(ctx, w, orders, args.c5_sweeps, w["order_max"] - 1)
     # one lookup reads one nu x nv coefficient block per table (tables are
     # shared by the two spins here); upper bound on the contraction's bytes
     block = w["nc"] * w["nc"] * 8
@@ -411,6 +436,9 @@ def main_c5(args):
         "contraction_block_read_upper_bound_gbs": round(res["mfma64"]["lookups"] * block / (cms * 1e-3) / 1e9, 1),
         "orders": {str(n): int(np.count_nonzero(orders == n)) for n in range(w["order_min"], w["order_max"] + 1)},
         "mfma64_vs_exact": per_order,
+        "sweeps_per_s": round(args.c5_sweeps / sweep_s, 2) if sweep_s else None, "sweeps": args.c5_sweeps,
+        "sweep_ratio_order4_over_order2": (float(sweep_ratios[4]) if sweep_ratios is not None
+                                           and np.isfinite(sweep_ratios[4]) else None),
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -30,6 +30,7 @@
 #include <functional>
 #include <iomanip>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <random>
 #include <sstream>
@@ -529,6 +530,8 @@ struct StepWork {  // MCMCMCStepWorkspace (mcmcmc_sampler.h:114-122)
 };
 
 struct Chain {
+    cdet_mcmcmc_config cfg{};        // this chain's settings (its envelope order)
+    const MaskPlan* plan = nullptr;  // mask plan of that envelope order
     StepWork work;
     std::string error;
     size_t index = 0;
@@ -834,21 +837,41 @@ int host_thread_count(const cdet_mcmcmc_config& c) {
 // step work (heat-bath choice, regeneration, statistics, checkpoint) spread
 // over host threads.  A chain's work touches only its own state, files and
 // trace row, so the thread schedule cannot change any result.
-void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_eval_fn eval, void* eval_user,
-                cdet_mcmcmc_chain_result* results, double* ratios, uint64_t* trace, int64_t trace_stride) {
-    if (c.envelope_order <= 1)
-        throw std::runtime_error("The all-subset replica integrand is defined only for envelope order >= 2");
-    const MaskPlan plan = make_plan(c);
-    const int E = c.envelope_order;
-    const size_t sc = size_t{1} << E;
+//
+// envelopes (may be null): chain k's envelope order.  Its sampler then runs
+// with envelope_order = envelopes[k] and order_max = min(order_max,
+// envelopes[k] - 1), and its checkpoint records those values.  Without it every
+// chain uses the configuration as given, as the reference does.
+void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const int32_t* envelopes,
+                cdet_batch_eval_fn eval, void* eval_user, cdet_mcmcmc_chain_result* results, double* ratios,
+                uint64_t* trace, int64_t trace_stride) {
     std::vector<Chain> chains(static_cast<size_t>(c.chain_count));
+    std::map<int, MaskPlan> plans;  // by envelope order
+    int max_env = 0;
+    for (int k = 0; k < c.chain_count; ++k) {
+        Chain& ch = chains[static_cast<size_t>(k)];
+        ch.cfg = c;
+        if (envelopes) {
+            const int e = envelopes[k];
+            ch.cfg.envelope_order = e;
+            ch.cfg.order_max = std::min(c.order_max, e - 1);
+        }
+        validate(ch.cfg);
+        if (ch.cfg.envelope_order <= 1)
+            throw std::runtime_error("The all-subset replica integrand is defined only for envelope order >= 2");
+        if (!plans.count(ch.cfg.envelope_order)) plans.emplace(ch.cfg.envelope_order, make_plan(ch.cfg));
+        ch.plan = &plans.at(ch.cfg.envelope_order);
+        max_env = std::max(max_env, ch.cfg.envelope_order);
+    }
     constexpr uint64_t kChainTag = 0x4D434D434D435348ULL;  // :1103
     for (int k = 0; k < c.chain_count; ++k) {
         Chain& ch = chains[static_cast<size_t>(k)];
+        const cdet_mcmcmc_config& cc = ch.cfg;
+        const int E = cc.envelope_order;
         const uint64_t s = derive_seed(c.base_seed, kChainTag, static_cast<uint64_t>(k));
         bool resumed = false;
         if (c.checkpoint_enabled && c.resume_mode != CDET_RESUME_OFF) {
-            resumed = load_checkpoint(c, static_cast<size_t>(k), s, ch);
+            resumed = load_checkpoint(cc, static_cast<size_t>(k), s, ch);
             if (!resumed && c.resume_mode == CDET_RESUME_REQUIRE)
                 throw std::runtime_error("MCMCMC_RESUME=require but checkpoint is missing for chain " +
                                          std::to_string(k));
@@ -861,7 +884,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
             for (int i = 0; i < E; ++i) ch.vertices.push_back(uniform_vertex(ch.rng));
             ch.lambdas.assign(c.lambdas, c.lambdas + E + 1);
             ch.warmup_block_counts.assign(static_cast<size_t>(E + 1), 0);
-            init_result(ch.result, c);
+            init_result(ch.result, cc);
         }
         if (c.checkpoint_enabled) {
             const uint64_t done = completed_steps(ch);
@@ -874,7 +897,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
         if (c.lambda_warmup_steps % c.lambda_adapt_interval != 0 &&
             std::any_of(ch.warmup_block_counts.begin(), ch.warmup_block_counts.end(),
                         [](uint64_t x) { return x != 0; })) {
-            adapt_lambdas(ch.lambdas, c, ch.warmup_block_counts);
+            adapt_lambdas(ch.lambdas, ch.cfg, ch.warmup_block_counts);
             std::fill(ch.warmup_block_counts.begin(), ch.warmup_block_counts.end(), 0);
         }
         ch.entered_production = true;
@@ -886,8 +909,10 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
     };
     // One chain's step on its integrand values (:1030-1085).
     auto advance = [&](Chain& ch, const double* values) {
+        const cdet_mcmcmc_config& cc = ch.cfg;
+        const int E = cc.envelope_order;
         const bool warm = ch.result.warmup_proposed < static_cast<uint64_t>(c.lambda_warmup_steps);
-        const StepOut s = heat_bath_step(c, plan, seed, ch, values);
+        const StepOut s = heat_bath_step(cc, *ch.plan, seed, ch, values);
         ch.vertices.swap(ch.next);
         if (trace && static_cast<int64_t>(ch.steps_this_run) < trace_stride)
             trace[static_cast<int64_t>(ch.index) * trace_stride + static_cast<int64_t>(ch.steps_this_run)] = s.chosen_mask;
@@ -902,7 +927,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
             r.warmup_retained.add(static_cast<long double>(s.chosen_order) / static_cast<long double>(E));
             ++ch.warmup_block_counts[static_cast<size_t>(s.chosen_order)];
             if (next_step % static_cast<uint64_t>(c.lambda_adapt_interval) == 0) {
-                adapt_lambdas(ch.lambdas, c, ch.warmup_block_counts);
+                adapt_lambdas(ch.lambdas, cc, ch.warmup_block_counts);
                 std::fill(ch.warmup_block_counts.begin(), ch.warmup_block_counts.end(), 0);
             }
         } else {  // :1064-1083
@@ -910,22 +935,25 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
             ++r.production_accepted;
             ++r.production_chosen[static_cast<size_t>(s.chosen_order)];
             r.production_retained.add(static_cast<long double>(s.chosen_order) / static_cast<long double>(E));
-            for (int o = c.order_min; o <= c.order_max; ++o)
+            for (int o = cc.order_min; o <= cc.order_max; ++o)
                 r.order_signal[static_cast<size_t>(o)].add(order_signal[static_cast<size_t>(o)] /
                                                              ch.lambdas[static_cast<size_t>(o)]);
         }
-        checkpoint_if_due(c, ch, false);
+        checkpoint_if_due(cc, ch, false);
         if (c.stop_after_steps && ch.steps_this_run >= c.stop_after_steps) ch.stopped = true;
     };
     // Threads pay off once a step's host work is a few microseconds per chain:
     // the histogram seed's subset recursion (2^E * E) or a large mask plan.
-    const size_t per_chain = plan.masks.size() + (seed.uniform ? 0 : sc * static_cast<size_t>(E));
-    const int threads = per_chain * chains.size() >= 32768 ? std::min(host_thread_count(c), c.chain_count) : 1;
+    size_t host_work = 0;
+    for (const Chain& ch : chains)
+        host_work += ch.plan->masks.size() +
+                     (seed.uniform ? 0 : (size_t{1} << ch.cfg.envelope_order) * static_cast<size_t>(ch.cfg.envelope_order));
+    const int threads = host_work >= 32768 ? std::min(host_thread_count(c), c.chain_count) : 1;
     WorkPool pool(threads);
     std::vector<cdet_vertex> hv;
     std::vector<int32_t> ho;
     std::vector<double> hout;
-    std::vector<size_t> batch;
+    std::vector<size_t> batch, hoff;
     for (;;) {
         batch.clear();
         for (auto& ch : chains) {
@@ -935,15 +963,21 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
         }
         if (batch.empty()) break;
         hv.clear();
-        ho.assign(batch.size(), E);
-        for (size_t b : batch) hv.insert(hv.end(), chains[b].vertices.begin(), chains[b].vertices.end());
-        hout.assign(batch.size() * sc, 0.0);
+        ho.clear();
+        hoff.assign(1, 0);
+        for (size_t b : batch) {
+            const Chain& ch = chains[b];
+            hv.insert(hv.end(), ch.vertices.begin(), ch.vertices.end());
+            ho.push_back(ch.cfg.envelope_order);
+            hoff.push_back(hoff.back() + (size_t{1} << ch.cfg.envelope_order));
+        }
+        hout.assign(hoff.back(), 0.0);
         const int rc = eval(eval_user, hv.data(), ho.data(), static_cast<int64_t>(batch.size()), hout.data());
         if (rc != 0) throw std::runtime_error("integrand evaluation failed (status " + std::to_string(rc) + ")");
         pool.run(batch.size(), [&](size_t bi) {
             Chain& ch = chains[batch[bi]];
             try {
-                advance(ch, hout.data() + bi * sc);
+                advance(ch, hout.data() + hoff[bi]);
             } catch (const std::exception& e) {
                 ch.error = e.what();
                 ch.stopped = true;
@@ -955,25 +989,47 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, cdet_batch_
     for (auto& ch : chains) {
         if (!ch.stopped) {
             if (ch.result.warmup_proposed >= static_cast<uint64_t>(c.lambda_warmup_steps)) enter_production(ch);
-            checkpoint_if_due(c, ch, true);  // forced final save (:1085)
+            checkpoint_if_due(ch.cfg, ch, true);  // forced final save (:1085)
         }
     }
-    // Cross-chain statistics (:1117-1142).
-    std::vector<Stats> cross(static_cast<size_t>(E + 1));
-    for (auto& ch : chains)
-        for (int o = c.order_min; o <= c.order_max; ++o) {
-            const Stats& s = ch.result.order_signal[static_cast<size_t>(o)];
-            if (s.count > 0) cross[static_cast<size_t>(o)].add(s.mean);
-        }
     if (ratios) {
-        const long double ref = cross[static_cast<size_t>(c.normalization_order)].mean;
-        for (int o = 0; o <= E; ++o) ratios[o] = std::numeric_limits<double>::quiet_NaN();
-        for (int o = c.order_min; o <= c.order_max; ++o)
-            ratios[o] = ref == 0.0L ? std::numeric_limits<double>::quiet_NaN()
-                                    : static_cast<double>(cross[static_cast<size_t>(o)].mean / ref);
+        for (int o = 0; o <= max_env; ++o) ratios[o] = std::numeric_limits<double>::quiet_NaN();
+        const size_t norm = static_cast<size_t>(c.normalization_order);
+        if (!envelopes) {
+            // Cross-chain statistics (:1117-1142): ratio of the cross-chain
+            // means of the order signals.
+            std::vector<Stats> cross(static_cast<size_t>(max_env + 1));
+            for (auto& ch : chains)
+                for (int o = c.order_min; o <= c.order_max; ++o) {
+                    const Stats& s = ch.result.order_signal[static_cast<size_t>(o)];
+                    if (s.count > 0) cross[static_cast<size_t>(o)].add(s.mean);
+                }
+            const long double ref = cross[norm].mean;
+            for (int o = c.order_min; o <= c.order_max; ++o)
+                ratios[o] = ref == 0.0L ? std::numeric_limits<double>::quiet_NaN()
+                                        : static_cast<double>(cross[static_cast<size_t>(o)].mean / ref);
+        } else {
+            // Chains of different envelopes normalise their signals by
+            // different partition sums, so each chain's ratio to its own
+            // normalization-order signal is formed first and those ratios
+            // are averaged over the chains that measure the order.
+            std::vector<Stats> cross(static_cast<size_t>(max_env + 1));
+            for (auto& ch : chains) {
+                const long double ref = ch.result.order_signal[norm].mean;
+                if (ch.result.order_signal[norm].count == 0 || ref == 0.0L) continue;
+                for (int o = ch.cfg.order_min; o <= ch.cfg.order_max; ++o) {
+                    const Stats& s = ch.result.order_signal[static_cast<size_t>(o)];
+                    if (s.count > 0) cross[static_cast<size_t>(o)].add(s.mean / ref);
+                }
+            }
+            for (int o = c.order_min; o <= std::min(c.order_max, max_env - 1); ++o)
+                if (cross[static_cast<size_t>(o)].count > 0)
+                    ratios[o] = static_cast<double>(cross[static_cast<size_t>(o)].mean);
+        }
     }
     if (results) {
         for (auto& ch : chains) {
+            const int E = ch.cfg.envelope_order;
             cdet_mcmcmc_chain_result& o = results[ch.index];
             std::memset(&o, 0, sizeof(o));
             o.seed = ch.seed;
@@ -1034,7 +1090,23 @@ int cdet_mcmcmc_run_seeded(const cdet_mcmcmc_config* cfg, const cdet_histogram_s
         if (!cfg || !eval) throw std::runtime_error("cdet_mcmcmc_run: null argument");
         validate(*cfg);
         const SeedKernel kernel = SeedKernel::from(seed);
-        run_chains(*cfg, kernel, eval, eval_user, results, ratios, trace, trace_stride);
+        run_chains(*cfg, kernel, nullptr, eval, eval_user, results, ratios, trace, trace_stride);
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int cdet_mcmcmc_run_envelopes(const cdet_mcmcmc_config* cfg, const cdet_histogram_seed* seed,
+                              const int32_t* envelope_orders, cdet_batch_eval_fn eval, void* eval_user,
+                              cdet_mcmcmc_chain_result* results, double* ratios, uint64_t* trace,
+                              int64_t trace_stride) {
+    try {
+        if (!cfg || !eval || !envelope_orders) throw std::runtime_error("cdet_mcmcmc_run_envelopes: null argument");
+        validate(*cfg);
+        const SeedKernel kernel = SeedKernel::from(seed);
+        run_chains(*cfg, kernel, envelope_orders, eval, eval_user, results, ratios, trace, trace_stride);
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();
@@ -1075,7 +1147,7 @@ int cdet_mcmcmc_run(const cdet_mcmcmc_config* cfg, cdet_batch_eval_fn eval, void
         const cdet_mcmcmc_config& c = *cfg;
         validate(c);
         if (c.seed_mode != CDET_SEED_HISTOGRAM) {
-            run_chains(c, SeedKernel{}, eval, eval_user, results, ratios, trace, trace_stride);
+            run_chains(c, SeedKernel{}, nullptr, eval, eval_user, results, ratios, trace, trace_stride);
             return 0;
         }
         // src/main.cpp:1071-1102: learn, export, optionally stop, run.
@@ -1088,7 +1160,7 @@ int cdet_mcmcmc_run(const cdet_mcmcmc_config* cfg, cdet_batch_eval_fn eval, void
         learn_histogram(c, eval, eval_user, h, nullptr);
         export_histogram(h, c.beta, std::string(c.histogram_output_prefix));
         if (c.histogram_learning_only) return 0;
-        run_chains(c, SeedKernel::from(&h), eval, eval_user, results, ratios, trace, trace_stride);
+        run_chains(c, SeedKernel::from(&h), nullptr, eval, eval_user, results, ratios, trace, trace_stride);
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();

@@ -125,6 +125,11 @@ def lib():
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ChainResult),
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.c_int64]
+        L.cdet_mcmcmc_run_envelopes.restype = ctypes.c_int
+        L.cdet_mcmcmc_run_envelopes.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.POINTER(HistogramSeed),
+                                                ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.POINTER(ChainResult), ctypes.POINTER(ctypes.c_double),
+                                                ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
         L.cdet_histogram_learn.restype = ctypes.c_int
         L.cdet_histogram_learn.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(HistogramSeed), ctypes.POINTER(ctypes.c_double)]
@@ -262,6 +267,22 @@ def run_seeded(config, seed, eval_fn, eval_user, trace_steps=0):
         ctypes.byref(config), ctypes.byref(seed) if seed is not None else None, fn, eval_user, results, ratios,
         trace.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if trace_steps else None, trace_steps))
     return list(results), np.array(ratios[:config.envelope_order + 1]), (trace if trace_steps else None)
+
+
+def run_envelopes(config, envelopes, eval_fn, eval_user, seed=None, trace_steps=0):
+    """Chains with their own envelope orders (cdet_mcmcmc_run_envelopes)."""
+    n = config.chain_count
+    env = np.ascontiguousarray(envelopes, dtype=np.int32)
+    assert env.shape == (n,)
+    results = (ChainResult * n)()
+    ratios = (ctypes.c_double * (MAXE + 1))()
+    trace = np.zeros((n, max(trace_steps, 1)), dtype=np.uint64)
+    _check(lib().cdet_mcmcmc_run_envelopes(
+        ctypes.byref(config), ctypes.byref(seed) if seed is not None else None,
+        env.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.cast(eval_fn, ctypes.c_void_p), eval_user,
+        results, ratios, trace.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if trace_steps else None,
+        trace_steps))
+    return list(results), np.array(ratios[:int(env.max()) + 1]), (trace if trace_steps else None)
 
 
 def learn_histogram(config, eval_fn, eval_user):

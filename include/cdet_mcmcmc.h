@@ -144,6 +144,23 @@ int cdet_mcmcmc_run_seeded(const cdet_mcmcmc_config* config, const cdet_histogra
                            double* ratios, uint64_t* trace, int64_t trace_stride);
 
 /*
+ * Chains with their own envelope orders (no counterpart in the reference,
+ * whose chains share MCMCMC_ENVELOPE_ORDER): chain k runs the reference's
+ * sampler with envelope_order = envelope_orders[k] and order_max =
+ * min(config->order_max, envelope_orders[k] - 1); its checkpoint records those
+ * two values.  Each chain's settings must pass the reference's validation
+ * (so normalization_order < every envelope order).  ratios (max envelope
+ * order + 1 entries): for each order, the mean over the chains that measure it
+ * of the chain's order-signal mean divided by its normalization-order signal
+ * mean (chains of different envelopes normalise by different partition sums,
+ * so their raw signals are not averaged).  seed as in cdet_mcmcmc_run_seeded.
+ */
+int cdet_mcmcmc_run_envelopes(const cdet_mcmcmc_config* config, const cdet_histogram_seed* seed,
+                              const int32_t* envelope_orders, cdet_batch_eval_fn eval, void* eval_user,
+                              cdet_mcmcmc_chain_result* results, double* ratios, uint64_t* trace,
+                              int64_t trace_stride);
+
+/*
  * Learn the histogram seed (learn_factorized_histogram_seed,
  * include/cdet/mcmcmc_sampler.h:607-671) from histogram_learning_steps
  * uniform envelopes drawn from one std::mt19937_64 stream, evaluated in

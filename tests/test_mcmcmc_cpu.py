@@ -109,3 +109,49 @@ def test_params_file_parsing(tmp_path):
     assert (c.order_min, c.order_max, c.envelope_order, c.normalization_order) == (1, 6, 7, 1)
     assert list(c.lambdas[1:7]) == [1.0] * 6 and c.resume_mode == 2 and c.chain_count == 2
     assert mcmcmc.integrand_scalars_from_params(params) == (72.0 ** 0.5, 0.5, 0.5)
+
+
+def test_per_chain_envelopes_match_single_envelope_runs(tmp_path):
+    # chain k with envelope E_k is the reference sampler at envelope_order =
+    # E_k and order_max = min(order_max, E_k - 1), seeded by its index
+    fn, orc = oracle_eval()
+    env = [4, 3, 5]
+    c = base_config(tmp_path / "mix", chain_count=3, checkpoint_enabled=0)
+    c.envelope_order = 5
+    _, _, t_mix = mcmcmc.run_envelopes(c, env, fn, orc._h, trace_steps=500)
+    for k, e in enumerate(env):
+        one = base_config(tmp_path / f"one{k}", chain_count=k + 1, checkpoint_enabled=0)
+        one.envelope_order, one.order_max = e, min(3, e - 1)
+        _, _, t_one = mcmcmc.run(one, fn, orc._h, trace_steps=500)
+        assert np.array_equal(t_mix[k], t_one[k]), k
+    # equal envelopes reproduce the shared-envelope run
+    same = base_config(tmp_path / "s", chain_count=2, checkpoint_enabled=0)
+    _, _, t_same = mcmcmc.run_envelopes(same, [4, 4], fn, orc._h, trace_steps=500)
+    _, _, t_ref = mcmcmc.run(base_config(tmp_path / "r", chain_count=2, checkpoint_enabled=0), fn, orc._h,
+                             trace_steps=500)
+    assert np.array_equal(t_same, t_ref)
+
+
+def test_per_chain_envelopes_resume_and_ratios(tmp_path):
+    fn, orc = oracle_eval()
+    env = [5, 3, 4, 2]
+    full, cut = tmp_path / "full", tmp_path / "cut"
+    mk = lambda d, **o: base_config(d, chain_count=4, **o)
+    mcmcmc.run_envelopes(mk(full), env, fn, orc._h)
+    mcmcmc.run_envelopes(mk(cut, stop_after_steps=333), env, fn, orc._h)
+    mcmcmc.run_envelopes(mk(cut, resume_mode=2), env, fn, orc._h)
+    a, b = chk_bytes(full), chk_bytes(cut)
+    assert a == b and len(a) == 4
+    assert b"envelope_order 2\norder_min 1\n" not in a["chain_000003.chk"]
+    assert b"order_max 1\nenvelope_order 2\n" in a["chain_000003.chk"]
+    # synthetic product integrand: every order integrates to one
+    s = mcmcmc.MCMCMCConfig()
+    s.order_min, s.order_max, s.envelope_order, s.normalization_order = 1, 4, 5, 1
+    for o in (1, 2, 3, 4):
+        s.lambdas[o] = 1.0
+    s.lambda_warmup_steps, s.lambda_adapt_interval, s.lambda_adapt_rate = 20000, 1000, 0.25
+    s.production_steps, s.chain_count, s.base_seed = 200000, 4, 0x7812BC91
+    res, ratios, _ = mcmcmc.run_envelopes(s, [5, 5, 4, 3], ctypes.cast(O.lib().orc_synthetic_batch_eval,
+                                                                     ctypes.c_void_p), None)
+    assert all(abs(ratios[o] - 1.0) < 0.05 for o in (2, 3, 4)), ratios
+    assert res[3].order_signal_count[3] == 0 and res[0].order_signal_count[4] == 200000

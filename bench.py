@@ -440,6 +440,11 @@ def main_c5(args):
         "sweep_ratio_order4_over_order2": (float(sweep_ratios[4]) if sweep_ratios is not None
                                            and np.isfinite(sweep_ratios[4]) else None),
     }
+    if args.copy_probe:  # 256 MiB device copy: calibration of the HBM counters
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     ctx.close()
 

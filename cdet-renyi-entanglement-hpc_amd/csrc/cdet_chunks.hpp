@@ -634,10 +634,7 @@ void transform_samples(cdet_ctx* ctx, const fs::path& dir, const cdet_lattice* e
                     hip_check(hipGetLastError(), "k_place_nodes");
                 }
             }
-            double* gtmp = nullptr;
-            const int rc = table_dct(ctx, d_cu, d_cv, dst, static_cast<int64_t>(g.sitepairs), s, &gtmp);
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-            if (gtmp) (void)hipFree(gtmp);
+            const int rc = table_dct(ctx, d_cu, d_cv, dst, static_cast<int64_t>(g.sitepairs), s);
             if (rc) throw Error(rc, g_last_error);
             const fs::path cp = corner_path(dir, kind, br);
             require_size(cp, g.sitepairs * sizeof(double));

@@ -546,6 +546,94 @@ __global__ __launch_bounds__(256) void k_dct(double* __restrict__ tab, int nu, i
     }
 }
 
+// The separable cosine transform as two FP64 matrix-core GEMMs (Nc a
+// multiple of 16).  For every sitepair block X (nu x nv):
+//   pass 1:  Y^T = (X Wv^T)^T          Wv[b][j] = (b ? 2 : 1)/nv cos(b theta_j)
+//   pass 2:  C   = (Y^T Wu^T)^T = Wu X Wv^T
+// Each pass is a row GEMM  out(r, n) = sum_k X[r][k] W[n][k]  over all rows r
+// of the blocks, stored block-transposed: row r = blk * s + i goes to
+// s * blk * N + n * blk + i.  A workgroup owns 64 rows; wave w computes the
+// 16 x N tile out^T[n][r] = W X^T with W as the MFMA A operand and the rows as
+// B, so that each accumulator register holds 16 consecutive rows of one n and
+// the transposed store is contiguous.  K is staged through LDS 16 at a time.
+typedef double pd4 __attribute__((ext_vector_type(4)));
+constexpr int kDctRows = 64;
+constexpr int kDctK = 16;
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_dct_pass(const double* __restrict__ X, int64_t R, int K,
+                                                  const double* __restrict__ W, int blk, double* __restrict__ out) {
+    constexpr int N = NT * 16;
+    __shared__ double xs[kDctRows][kDctK + 1];
+    __shared__ double ws[N][kDctK + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    for (int64_t r0 = static_cast<int64_t>(blockIdx.x) * kDctRows; r0 < R;
+         r0 += static_cast<int64_t>(gridDim.x) * kDctRows) {
+        pd4 acc[NT];
+        for (int t = 0; t < NT; ++t) acc[t] = pd4{0.0, 0.0, 0.0, 0.0};
+        for (int kc = 0; kc < K; kc += kDctK) {
+            for (int e = tid; e < kDctRows * kDctK; e += 256) {
+                const int row = e / kDctK;
+                const int kk = e - row * kDctK;
+                const int64_t r = r0 + row;
+                xs[row][kk] = r < R ? X[r * K + kc + kk] : 0.0;
+            }
+            for (int e = tid; e < N * kDctK; e += 256) {
+                const int n = e / kDctK;
+                const int kk = e - n * kDctK;
+                ws[n][kk] = W[static_cast<int64_t>(n) * K + kc + kk];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < kDctK; ks += 4) {
+                const double b = xs[wave * 16 + j][ks + g];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double a = ws[t * 16 + j][ks + g];
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+        const int64_t r = r0 + wave * 16 + j;
+        if (r < R) {
+            const int64_t blkno = r / blk;
+            const int64_t base = blkno * blk * N + (r - blkno * blk);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) out[base + static_cast<int64_t>(t * 16 + 4 * q + g) * blk] = acc[t][q];
+        }
+    }
+}
+
+template <int NT>
+void launch_dct_pass_nt(const double* X, int64_t R, int K, const double* W, int blk, double* out, int grid,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_dct_pass<NT>, dim3(grid), dim3(256), 0, s, X, R, K, W, blk, out);
+}
+
+int launch_dct_pass(int N, const double* X, int64_t R, int K, const double* W, int blk, double* out, int ncu,
+                    hipStream_t s) {
+    const int64_t panels = (R + kDctRows - 1) / kDctRows;
+    const int grid = static_cast<int>(std::min<int64_t>(panels, static_cast<int64_t>(ncu) * 8));
+    switch (N / 16) {
+#define CDET_DCT_NT(n) \
+    case n: launch_dct_pass_nt<n>(X, R, K, W, blk, out, grid, s); break;
+        CDET_DCT_NT(1) CDET_DCT_NT(2) CDET_DCT_NT(3) CDET_DCT_NT(4) CDET_DCT_NT(5) CDET_DCT_NT(6)
+        CDET_DCT_NT(7) CDET_DCT_NT(8) CDET_DCT_NT(9) CDET_DCT_NT(10) CDET_DCT_NT(11) CDET_DCT_NT(12)
+        CDET_DCT_NT(13) CDET_DCT_NT(14) CDET_DCT_NT(15) CDET_DCT_NT(16)
+#undef CDET_DCT_NT
+        default: return fail(CDET_ERR_CAPACITY, "cosine transform: Chebyshev count outside 16..256");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 // Cyclic Jacobi eigendecomposition of a real symmetric matrix (row-major),
 // eigenvalues ascending, eigenvectors in columns of V.
 void host_jacobi(std::vector<double> A, int n, std::vector<double>& ev, std::vector<double>& V) {
@@ -787,24 +875,73 @@ int producer_samples(cdet_ctx* ctx, const Producer& pr, int upper, bool replica0
     return 0;
 }
 
-// In-place cosine transform of `active` sitepair sample grids at dst.
-int table_dct(cdet_ctx* ctx, const double* d_cu, const double* d_cv, double* dst, int64_t active, hipStream_t s,
-              double** gtmp_keep) {
+// In-place cosine transform of `active` sitepair sample grids at dst.  With
+// nu and nv multiples of 16 (up to 256) it runs as the two matrix-core passes
+// above, a bounded group of sitepairs at a time through a temporary of the
+// same layout; otherwise the per-sitepair k_dct.  Returns with the stream
+// drained.
+int table_dct(cdet_ctx* ctx, const double* d_cu, const double* d_cv, double* dst, int64_t active, hipStream_t s) {
     const cdet_params& p = ctx->p;
-    const size_t blk = static_cast<size_t>(p.cheb_nu) * p.cheb_nv;
+    const int nu = p.cheb_nu, nv = p.cheb_nv;
+    const size_t blk = static_cast<size_t>(nu) * nv;
+    if (nu % 16 == 0 && nv % 16 == 0 && nu <= 256 && nv <= 256) {
+        // scaled cosine tables W[a][i] = (a ? 2 : 1)/n cos(a theta_i)
+        std::vector<double> wu(static_cast<size_t>(nu) * nu), wv(static_cast<size_t>(nv) * nv);
+        const double pi = M_PI;
+        for (int a = 0; a < nu; ++a)
+            for (int i = 0; i < nu; ++i) {
+                const double th = (static_cast<double>(2 * i + 1) * pi) / static_cast<double>(2 * nu);
+                wu[static_cast<size_t>(a) * nu + i] =
+                    (a == 0 ? 1.0 : 2.0) / static_cast<double>(nu) * std::cos(static_cast<double>(a) * th);
+            }
+        for (int b = 0; b < nv; ++b)
+            for (int j = 0; j < nv; ++j) {
+                const double th = (static_cast<double>(2 * j + 1) * pi) / static_cast<double>(2 * nv);
+                wv[static_cast<size_t>(b) * nv + j] =
+                    (b == 0 ? 1.0 : 2.0) / static_cast<double>(nv) * std::cos(static_cast<double>(b) * th);
+            }
+        const int64_t group = std::max<int64_t>(1, std::min<int64_t>(active, (int64_t{512} << 20) / (blk * 8)));
+        double *d_w = nullptr, *tmp = nullptr;
+        int rc = 0;
+        if (hipMalloc(&d_w, sizeof(double) * (wu.size() + wv.size())) != hipSuccess ||
+            hipMalloc(&tmp, sizeof(double) * blk * static_cast<size_t>(group)) != hipSuccess) {
+            rc = fail(CDET_ERR_DEVICE, "cosine transform: device allocation failed");
+        } else {
+            double* d_wu = d_w;
+            double* d_wv = d_w + wu.size();
+            if (hipMemcpyAsync(d_wu, wu.data(), sizeof(double) * wu.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(d_wv, wv.data(), sizeof(double) * wv.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+                rc = fail(CDET_ERR_DEVICE, "cosine transform: table upload failed");
+            for (int64_t sp0 = 0; sp0 < active && !rc; sp0 += group) {
+                const int64_t cnt = std::min(group, active - sp0);
+                double* x = dst + static_cast<size_t>(sp0) * blk;
+                rc = launch_dct_pass(nv, x, cnt * nu, nv, d_wv, nu, tmp, ctx->ncu, s);
+                if (!rc) rc = launch_dct_pass(nu, tmp, cnt * nv, nu, d_wu, nv, x, ctx->ncu, s);
+            }
+        }
+        const hipError_t se = hipStreamSynchronize(s);
+        if (d_w) (void)hipFree(d_w);
+        if (tmp) (void)hipFree(tmp);
+        if (rc) return rc;
+        if (se != hipSuccess) return fail(CDET_ERR_DEVICE, std::string("cosine transform: ") + hipGetErrorString(se));
+        return 0;
+    }
     size_t dct_lds = blk * sizeof(double);
     double* gtmp = nullptr;
     if (dct_lds > kMaxLds) {
         HIP_TRY(hipMalloc(&gtmp, blk * sizeof(double) * static_cast<size_t>(active)));
-        *gtmp_keep = gtmp;  // freed by the caller after the stream drains
         dct_lds = 0;
     } else if (dct_lds > 64 * 1024) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_dct), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(dct_lds)));
     }
-    hipLaunchKernelGGL(k_dct, dim3(static_cast<unsigned>(active)), dim3(256), dct_lds, s, dst, p.cheb_nu, p.cheb_nv,
-                       d_cu, d_cv, gtmp);
-    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_dct, dim3(static_cast<unsigned>(active)), dim3(256), dct_lds, s, dst, nu, nv, d_cu, d_cv,
+                       gtmp);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(s);
+    if (gtmp) (void)hipFree(gtmp);
+    if (le != hipSuccess) return fail(CDET_ERR_DEVICE, std::string("k_dct: ") + hipGetErrorString(le));
+    if (se != hipSuccess) return fail(CDET_ERR_DEVICE, std::string("k_dct: ") + hipGetErrorString(se));
     return 0;
 }
 
@@ -842,16 +979,13 @@ int build_one_table(cdet_ctx* ctx, int kind, double mu_eff, int ordinary, const 
     const int S = pr.S;
     const int64_t active = replica00 ? static_cast<int64_t>(S) * S : static_cast<int64_t>(4) * S * S;
     const int nodes = p.cheb_nu * p.cheb_nv;
-    double* gtmp[2] = {nullptr, nullptr};
     for (int br = 0; br < 2 && !rc; ++br) {
         double* dst = br == 0 ? t.upper : t.lower;
         rc = producer_samples(ctx, pr, br == 0 ? 1 : 0, replica00, dst, br == 0 ? t.ucorner : t.lcorner, 0, nodes,
                               lat->node_chunk, s);
-        if (!rc) rc = table_dct(ctx, pr.d_cu, pr.d_cv, dst, active, s, &gtmp[br]);
+        if (!rc) rc = table_dct(ctx, pr.d_cu, pr.d_cv, dst, active, s);
     }
     const int st = producer_status(pr, s);
-    for (double* g : gtmp)
-        if (g) (void)hipFree(g);
     if (rc) return rc;
     if (st) return st;
     if (ordinary) ctx->ordinary_layout = lat->ordinary_layout;

@@ -17,6 +17,8 @@ Printed JSON line fields beyond the driver contract:
   cpu_baseline  the oracle (CPU restatement of the reference) on the host
                 cores, bounded sample of the same order-10 configurations
   plain_fp64    one-configuration-per-launch FP64 GPU form, evals/s
+  per_order     evals/s of every order bucket (1..10) of the seeded set, as
+                all-subset and as single-order integrands
   errors        against the bit-identical FP64 form (error_metrics), beside
                 the same figures for that FP64 form on tables moved by one
                 ulp (the spread any FP64 evaluation order shows)
@@ -54,6 +56,7 @@ def parse():
     ap.add_argument("--plain-evals", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
+    ap.add_argument("--no-per-order", action="store_true", help="skip the per-order evals/s table")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
@@ -92,6 +95,32 @@ def error_metrics(got, ref, cond):
         "well_conditioned_fraction": round(float(np.count_nonzero(well) / max(1, np.count_nonzero(use))), 4),
         "max_err_over_conditioning": float(np.max(np.abs(got - ref)[use] / cond[use])) if np.any(use) else 0.0,
     }
+
+
+def per_order_rates(ctx, orders, verts, mode, dev, sptr, reps=20):
+    """evals/s of every order bucket of the 8192-configuration set, as the
+    all-subset integrand (a2, orders >= 2) and the single-order integrand (a3),
+    each bucket one batch with its vertices resident in HBM, graph replays."""
+    import torch
+    out = {"all_subset": {}, "single_order": {}}
+    for n in range(int(orders.min()), int(orders.max()) + 1):
+        idx = np.nonzero(orders == n)[0]
+        d_v = torch.tensor(np.concatenate([verts[i] for i in idx]), dtype=torch.float64, device=dev)
+        for kind, single in (("all_subset", False), ("single_order", True)):
+            if not single and n < 2:
+                continue
+            plan = ctx.plan([n] * len(idx), mode=mode, single=single)
+            d_o = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
+            for _ in range(3):
+                plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
+            torch.cuda.synchronize()
+            out[kind][str(n)] = round(len(idx) * reps / (time.perf_counter() - t), 1)
+            plan.close()
+    return out
 
 
 def ulp_reference(ctx, params, d_verts, count, order, ref, cond, dev, sptr):
@@ -537,6 +566,7 @@ def main():
                              "fpm_cdet": round(stage[2], 4)}}
 
     ulp = None if args.no_ulp else ulp_reference(ctx, params, d_verts, count, w["order"], ref, cond, dev, sptr)
+    per_order = None if args.no_per_order else per_order_rates(ctx, orders, verts, args.mode, dev, sptr)
 
     # plain one-configuration-per-launch FP64 form
     nplain = min(args.plain_evals, count)
@@ -569,6 +599,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_fp64_evals_per_s": round(plain, 1),
+        "per_order_evals_per_s": per_order,
         "speedup_vs_plain_fp64": round(value / plain, 2),
         "errors_vs_fp64_exact": errors,
         "errors_fp64_one_ulp_tables": ulp,

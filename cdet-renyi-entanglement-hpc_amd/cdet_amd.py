@@ -128,6 +128,7 @@ def _signatures():
         "cdet_green_pair_eval": (i, [vp, vp, vp, i64, vp, i, vp]),
         "cdet_eval_batch_host": (i, [vp, vp, P(ctypes.c_int32), i64, dp]),
         "cdet_ctx_set_batch_mode": (i, [vp, i]),
+        "cdet_seed_cond_batch_host": (i, [vp, vp, vp, P(ctypes.c_int32), i64, dp]),
         "cdet_get_counters": (i, [vp, P(CdetCounters)]),
         "cdet_reset_counters": (i, [vp]),
         # include/cdet_chunks.h

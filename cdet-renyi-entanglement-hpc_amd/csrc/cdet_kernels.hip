@@ -831,6 +831,14 @@ struct cdet_ctx {
     size_t d_hstatus_cap = 0;
     // layout of the ordinary kinds now resident (CDET_ORDINARY_*)
     int ordinary_layout = CDET_ORDINARY_RESIDENT;
+    // histogram-seed conditional probabilities (cdet_seed_cond_batch_host)
+    void* d_seed_tab = nullptr;
+    size_t seed_tab_cap = 0;
+    uint64_t seed_hash = 0;
+    void* d_seed_in = nullptr;
+    size_t seed_in_cap = 0;
+    void* d_seed_out = nullptr;
+    size_t seed_out_cap = 0;
 };
 
 namespace {
@@ -1119,6 +1127,7 @@ static void invalidate_split(DevTable& t);
 #include "cdet_producer.hpp"
 #include "cdet_plan.hpp"
 #include "cdet_chunks.hpp"
+#include "cdet_seed.hpp"
 
 // ---------------------------------------------------------------------------
 // C-ABI
@@ -1233,6 +1242,9 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_hv);
     (void)hipFree(ctx->d_hout);
     (void)hipFree(ctx->d_hstatus);
+    (void)hipFree(ctx->d_seed_tab);
+    (void)hipFree(ctx->d_seed_in);
+    (void)hipFree(ctx->d_seed_out);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;

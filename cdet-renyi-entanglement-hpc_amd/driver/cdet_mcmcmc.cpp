@@ -761,7 +761,7 @@ struct StepOut {
 };
 
 StepOut heat_bath_step(const cdet_mcmcmc_config& c, const MaskPlan& plan, const SeedKernel& seed, Chain& ch,
-                       const double* values) {
+                       const double* values, const double* cond_values) {
     StepWork& ws = ch.work;
     std::vector<double>& fluxes = ws.fluxes;
     std::vector<double>& weights = ws.weights;
@@ -770,15 +770,20 @@ StepOut heat_bath_step(const cdet_mcmcmc_config& c, const MaskPlan& plan, const 
     fluxes.resize(nc);
     weights.resize(nc);
     order_signal.assign(static_cast<size_t>(c.envelope_order + 1), 0.0);
-    if (!seed.uniform)
+    // conditional generation probabilities: given (the batch callback) or
+    // computed here
+    const double* condp = cond_values;
+    if (!seed.uniform && !condp) {
         conditional_probabilities(seed, ch.vertices.data(), c.envelope_order, ws.cond, ws.dens, ws.comp);
+        condp = ws.cond.data();
+    }
     double norm = 0.0;
     for (size_t k = 0; k < nc; ++k) {
         const uint64_t mask = plan.masks[k];
         const int order = plan.orders[k];
         const double value = values[mask];
         // Probability of regenerating the dropped vertices from this subset.
-        const double cond = seed.uniform ? 1.0 : ws.cond[static_cast<size_t>(plan.full_mask ^ mask)];
+        const double cond = seed.uniform ? 1.0 : condp[plan.full_mask ^ mask];
         const double flux = ch.lambdas[static_cast<size_t>(order)] * value * cond;
         const double w = std::abs(flux) * plan.inv_binom[static_cast<size_t>(order)];
         fluxes[k] = flux;
@@ -842,9 +847,11 @@ int host_thread_count(const cdet_mcmcmc_config& c) {
 // with envelope_order = envelopes[k] and order_max = min(order_max,
 // envelopes[k] - 1), and its checkpoint records those values.  Without it every
 // chain uses the configuration as given, as the reference does.
-void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const int32_t* envelopes,
-                cdet_batch_eval_fn eval, void* eval_user, cdet_mcmcmc_chain_result* results, double* ratios,
-                uint64_t* trace, int64_t trace_stride) {
+void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_histogram_seed* hseed,
+                const int32_t* envelopes, const cdet_mcmcmc_backend& be, cdet_mcmcmc_chain_result* results,
+                double* ratios, uint64_t* trace, int64_t trace_stride) {
+    if (!be.eval) throw std::runtime_error("cdet_mcmcmc_run: null evaluation callback");
+    const bool gpu_cond = !seed.uniform && be.cond != nullptr;
     std::vector<Chain> chains(static_cast<size_t>(c.chain_count));
     std::map<int, MaskPlan> plans;  // by envelope order
     int max_env = 0;
@@ -908,11 +915,11 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const int32
                ch.result.production_proposed < static_cast<uint64_t>(c.production_steps);
     };
     // One chain's step on its integrand values (:1030-1085).
-    auto advance = [&](Chain& ch, const double* values) {
+    auto advance = [&](Chain& ch, const double* values, const double* cond_values) {
         const cdet_mcmcmc_config& cc = ch.cfg;
         const int E = cc.envelope_order;
         const bool warm = ch.result.warmup_proposed < static_cast<uint64_t>(c.lambda_warmup_steps);
-        const StepOut s = heat_bath_step(cc, *ch.plan, seed, ch, values);
+        const StepOut s = heat_bath_step(cc, *ch.plan, seed, ch, values, cond_values);
         ch.vertices.swap(ch.next);
         if (trace && static_cast<int64_t>(ch.steps_this_run) < trace_stride)
             trace[static_cast<int64_t>(ch.index) * trace_stride + static_cast<int64_t>(ch.steps_this_run)] = s.chosen_mask;
@@ -946,13 +953,14 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const int32
     // the histogram seed's subset recursion (2^E * E) or a large mask plan.
     size_t host_work = 0;
     for (const Chain& ch : chains)
-        host_work += ch.plan->masks.size() +
-                     (seed.uniform ? 0 : (size_t{1} << ch.cfg.envelope_order) * static_cast<size_t>(ch.cfg.envelope_order));
+        host_work += ch.plan->masks.size() + ((seed.uniform || gpu_cond) ? 0
+                                                 : (size_t{1} << ch.cfg.envelope_order) *
+                                                       static_cast<size_t>(ch.cfg.envelope_order));
     const int threads = host_work >= 32768 ? std::min(host_thread_count(c), c.chain_count) : 1;
     WorkPool pool(threads);
     std::vector<cdet_vertex> hv;
     std::vector<int32_t> ho;
-    std::vector<double> hout;
+    std::vector<double> hout, hcond;
     std::vector<size_t> batch, hoff;
     for (;;) {
         batch.clear();
@@ -972,12 +980,20 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const int32
             hoff.push_back(hoff.back() + (size_t{1} << ch.cfg.envelope_order));
         }
         hout.assign(hoff.back(), 0.0);
-        const int rc = eval(eval_user, hv.data(), ho.data(), static_cast<int64_t>(batch.size()), hout.data());
+        const int rc = be.eval(be.eval_user, hv.data(), ho.data(), static_cast<int64_t>(batch.size()), hout.data());
         if (rc != 0) throw std::runtime_error("integrand evaluation failed (status " + std::to_string(rc) + ")");
+        if (gpu_cond) {
+            hcond.assign(hoff.back(), 0.0);
+            const int rcc =
+                be.cond(be.cond_user, hseed, hv.data(), ho.data(), static_cast<int64_t>(batch.size()), hcond.data());
+            if (rcc != 0)
+                throw std::runtime_error("conditional generation probabilities failed (status " + std::to_string(rcc) +
+                                         ")");
+        }
         pool.run(batch.size(), [&](size_t bi) {
             Chain& ch = chains[batch[bi]];
             try {
-                advance(ch, hout.data() + hoff[bi]);
+                advance(ch, hout.data() + hoff[bi], gpu_cond ? hcond.data() + hoff[bi] : nullptr);
             } catch (const std::exception& e) {
                 ch.error = e.what();
                 ch.stopped = true;
@@ -1090,7 +1106,8 @@ int cdet_mcmcmc_run_seeded(const cdet_mcmcmc_config* cfg, const cdet_histogram_s
         if (!cfg || !eval) throw std::runtime_error("cdet_mcmcmc_run: null argument");
         validate(*cfg);
         const SeedKernel kernel = SeedKernel::from(seed);
-        run_chains(*cfg, kernel, nullptr, eval, eval_user, results, ratios, trace, trace_stride);
+        run_chains(*cfg, kernel, seed, nullptr, cdet_mcmcmc_backend{eval, eval_user, nullptr, nullptr}, results,
+                   ratios, trace, trace_stride);
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();
@@ -1106,7 +1123,24 @@ int cdet_mcmcmc_run_envelopes(const cdet_mcmcmc_config* cfg, const cdet_histogra
         if (!cfg || !eval || !envelope_orders) throw std::runtime_error("cdet_mcmcmc_run_envelopes: null argument");
         validate(*cfg);
         const SeedKernel kernel = SeedKernel::from(seed);
-        run_chains(*cfg, kernel, envelope_orders, eval, eval_user, results, ratios, trace, trace_stride);
+        run_chains(*cfg, kernel, seed, envelope_orders, cdet_mcmcmc_backend{eval, eval_user, nullptr, nullptr},
+                   results, ratios, trace, trace_stride);
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int cdet_mcmcmc_run_backend(const cdet_mcmcmc_config* cfg, const cdet_histogram_seed* seed,
+                            const int32_t* envelope_orders, const cdet_mcmcmc_backend* backend,
+                            cdet_mcmcmc_chain_result* results, double* ratios, uint64_t* trace,
+                            int64_t trace_stride) {
+    try {
+        if (!cfg || !backend) throw std::runtime_error("cdet_mcmcmc_run_backend: null argument");
+        validate(*cfg);
+        const SeedKernel kernel = SeedKernel::from(seed);
+        run_chains(*cfg, kernel, seed, envelope_orders, *backend, results, ratios, trace, trace_stride);
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();
@@ -1147,7 +1181,8 @@ int cdet_mcmcmc_run(const cdet_mcmcmc_config* cfg, cdet_batch_eval_fn eval, void
         const cdet_mcmcmc_config& c = *cfg;
         validate(c);
         if (c.seed_mode != CDET_SEED_HISTOGRAM) {
-            run_chains(c, SeedKernel{}, nullptr, eval, eval_user, results, ratios, trace, trace_stride);
+            run_chains(c, SeedKernel{}, nullptr, nullptr, cdet_mcmcmc_backend{eval, eval_user, nullptr, nullptr},
+                       results, ratios, trace, trace_stride);
             return 0;
         }
         // src/main.cpp:1071-1102: learn, export, optionally stop, run.
@@ -1160,7 +1195,8 @@ int cdet_mcmcmc_run(const cdet_mcmcmc_config* cfg, cdet_batch_eval_fn eval, void
         learn_histogram(c, eval, eval_user, h, nullptr);
         export_histogram(h, c.beta, std::string(c.histogram_output_prefix));
         if (c.histogram_learning_only) return 0;
-        run_chains(c, SeedKernel::from(&h), nullptr, eval, eval_user, results, ratios, trace, trace_stride);
+        run_chains(c, SeedKernel::from(&h), &h, nullptr, cdet_mcmcmc_backend{eval, eval_user, nullptr, nullptr},
+                   results, ratios, trace, trace_stride);
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();

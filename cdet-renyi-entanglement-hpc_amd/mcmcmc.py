@@ -93,6 +93,17 @@ class HistogramSeed(ctypes.Structure):
         return np.array(self.replica_prob[:]).reshape(2, 2)
 
 
+COND_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, ctypes.POINTER(ctypes.c_double))
+
+
+class Backend(ctypes.Structure):
+    """cdet_mcmcmc_backend (include/cdet_mcmcmc.h)."""
+
+    _fields_ = [("eval", ctypes.c_void_p), ("eval_user", ctypes.c_void_p),
+                ("cond", ctypes.c_void_p), ("cond_user", ctypes.c_void_p)]
+
+
 class ChainResult(ctypes.Structure):
     _fields_ = [
         ("seed", ctypes.c_uint64), ("warmup_proposed_steps", ctypes.c_uint64),
@@ -130,6 +141,11 @@ def lib():
                                                 ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.POINTER(ChainResult), ctypes.POINTER(ctypes.c_double),
                                                 ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
+        L.cdet_mcmcmc_run_backend.restype = ctypes.c_int
+        L.cdet_mcmcmc_run_backend.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.POINTER(HistogramSeed),
+                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Backend),
+                                              ctypes.POINTER(ChainResult), ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
         L.cdet_histogram_learn.restype = ctypes.c_int
         L.cdet_histogram_learn.argtypes = [ctypes.POINTER(MCMCMCConfig), ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(HistogramSeed), ctypes.POINTER(ctypes.c_double)]
@@ -283,6 +299,27 @@ def run_envelopes(config, envelopes, eval_fn, eval_user, seed=None, trace_steps=
         results, ratios, trace.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if trace_steps else None,
         trace_steps))
     return list(results), np.array(ratios[:int(env.max()) + 1]), (trace if trace_steps else None)
+
+
+def run_backend(config, eval_fn, eval_user, cond_fn=None, cond_user=None, seed=None, envelopes=None,
+                trace_steps=0):
+    """Chains through a backend (cdet_mcmcmc_run_backend): integrand callback
+    and, optionally, the conditional-generation callback of the histogram
+    seed.  envelopes: per-chain envelope orders or None."""
+    n = config.chain_count
+    env = None if envelopes is None else np.ascontiguousarray(envelopes, dtype=np.int32)
+    be = Backend(ctypes.cast(eval_fn, ctypes.c_void_p), eval_user,
+                 ctypes.cast(cond_fn, ctypes.c_void_p) if cond_fn is not None else None, cond_user)
+    results = (ChainResult * n)()
+    ratios = (ctypes.c_double * (MAXE + 1))()
+    trace = np.zeros((n, max(trace_steps, 1)), dtype=np.uint64)
+    _check(lib().cdet_mcmcmc_run_backend(
+        ctypes.byref(config), ctypes.byref(seed) if seed is not None else None,
+        env.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if env is not None else None, ctypes.byref(be),
+        results, ratios, trace.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if trace_steps else None,
+        trace_steps))
+    emax = config.envelope_order if env is None else int(env.max())
+    return list(results), np.array(ratios[:emax + 1]), (trace if trace_steps else None)
 
 
 def learn_histogram(config, eval_fn, eval_user):

@@ -194,6 +194,33 @@ int cdet_seed_density(const cdet_histogram_seed* seed, const cdet_vertex* child,
 const char* cdet_mcmcmc_last_error(void);
 
 /*
+ * Conditional-generation callback: for `count` envelopes packed as for
+ * cdet_batch_eval_fn, write the histogram seed's conditional generation
+ * probabilities of every subset, packed 2^order per envelope, bit-identical
+ * to cdet_seed_conditional_probabilities.  Returns 0 on success.
+ * cdet_seed_cond_batch_host (below) provides it on the GPU.
+ */
+typedef int (*cdet_batch_cond_fn)(void* user, const cdet_histogram_seed* seed, const cdet_vertex* h_vertices,
+                                  const int32_t* h_orders, int64_t count, double* h_out);
+
+/* Where a run evaluates: the integrand batch callback, and optionally the
+ * conditional-generation callback of the histogram seed (NULL: the driver's
+ * own host recursion, on host threads).  Either choice gives the same chains. */
+typedef struct cdet_mcmcmc_backend {
+    cdet_batch_eval_fn eval;
+    void* eval_user;
+    cdet_batch_cond_fn cond;
+    void* cond_user;
+} cdet_mcmcmc_backend;
+
+/* cdet_mcmcmc_run_envelopes with a backend; envelope_orders may be NULL (every
+ * chain at config->envelope_order, the reference's setting). */
+int cdet_mcmcmc_run_backend(const cdet_mcmcmc_config* config, const cdet_histogram_seed* seed,
+                            const int32_t* envelope_orders, const cdet_mcmcmc_backend* backend,
+                            cdet_mcmcmc_chain_result* results, double* ratios, uint64_t* trace,
+                            int64_t trace_stride);
+
+/*
  * GPU implementation of cdet_batch_eval_fn over a context (user = cdet_ctx*):
  * host buffers in and out, one batched plan per distinct order list (cached),
  * mode from cdet_ctx_set_batch_mode (default CDET_MODE_EXACT, bit-identical
@@ -202,6 +229,13 @@ const char* cdet_mcmcmc_last_error(void);
 int cdet_eval_batch_host(void* ctx, const cdet_vertex* h_vertices, const int32_t* h_orders, int64_t count,
                          double* h_out);
 int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode);
+
+/* GPU implementation of cdet_batch_cond_fn over a context (user = cdet_ctx*):
+ * one workgroup per envelope walks the popcount layers of the subset
+ * recursion with the probabilities in LDS (envelopes up to CDET_MAX_ORDER).
+ * The seed tables are uploaded when they change.  Synchronous. */
+int cdet_seed_cond_batch_host(void* ctx, const cdet_histogram_seed* seed, const cdet_vertex* h_vertices,
+                              const int32_t* h_orders, int64_t count, double* h_out);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ import cdet_amd
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-HIP_FROM_DRIVER_HEADER = ("cdet_eval_batch_host", "cdet_ctx_set_batch_mode")
+HIP_FROM_DRIVER_HEADER = ("cdet_eval_batch_host", "cdet_ctx_set_batch_mode", "cdet_seed_cond_batch_host")
 
 
 def functions_in(header):
@@ -51,7 +51,7 @@ def test_params_derive_is_host_only():
 def test_driver_library_exports():
     import mcmcmc
     lib = ctypes.CDLL(mcmcmc.LIB_PATH)
-    names = functions_in("cdet_mcmcmc.h") - set(HIP_FROM_DRIVER_HEADER) - {"cdet_batch_eval_fn"}
+    names = functions_in("cdet_mcmcmc.h") - set(HIP_FROM_DRIVER_HEADER) - {"cdet_batch_eval_fn", "cdet_batch_cond_fn"}
     assert len(names) >= 9
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
@@ -63,7 +63,8 @@ def test_struct_layouts_match_headers(tmp_path):
     import mcmcmc
     mirrors = {"cdet_params": cdet_amd.CdetParams, "cdet_lattice": cdet_amd.CdetLattice,
                "cdet_chunk_spec": cdet_amd.ChunkSpec, "cdet_mcmcmc_config": mcmcmc.MCMCMCConfig,
-               "cdet_mcmcmc_chain_result": mcmcmc.ChainResult, "cdet_histogram_seed": mcmcmc.HistogramSeed}
+               "cdet_mcmcmc_chain_result": mcmcmc.ChainResult, "cdet_histogram_seed": mcmcmc.HistogramSeed,
+               "cdet_mcmcmc_backend": mcmcmc.Backend}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cdet_chunks.h"', '#include "cdet_mcmcmc.h"',
              "int main(void) {"]
     for cname, cls in mirrors.items():

@@ -106,3 +106,68 @@ def test_driver_histogram_seed_gpu_matches_oracle(gpu_available, tmp_path):
     assert chk(tmp_path / "gpu") == chk(tmp_path / "orc")
     assert np.array_equal(ratg[1:], rato[1:], equal_nan=True)
     ctx.close()
+
+
+def gpu_cond(ctx, seed, env_list):
+    """cdet_seed_cond_batch_host over envelopes (list of (E, 4) arrays)."""
+    orders = np.array([len(v) for v in env_list], dtype=np.int32)
+    verts = np.ascontiguousarray(np.concatenate(env_list), dtype=np.float64)
+    out = np.zeros(int(np.sum(1 << orders.astype(np.int64))))
+    rc = cdet_amd.lib().cdet_seed_cond_batch_host(
+        ctx.handle, ctypes.byref(seed), verts.ctypes.data, orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        len(orders), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc != 0:
+        raise RuntimeError(cdet_amd.lib().cdet_last_error().decode())
+    return orders, out
+
+
+def test_gpu_conditional_probabilities_bit_identical(gpu_available):
+    # the GPU recursion equals the driver's host recursion to the last bit,
+    # envelopes 2..14, on learned-like random tables
+    ctx = gpu_ctx()
+    r = np.random.default_rng(5)
+    bins, n1, n2 = 8, 3, 3
+
+    def rows(k):
+        m = r.uniform(0.02, 1.0, (k, k))
+        return m / m.sum(axis=1, keepdims=True)
+
+    seed = mcmcmc.HistogramSeed.make(bins, n1, n2, rows(bins), rows(n1 * n2), rows(2))
+    envs = [r.uniform(0.0, 1.0, (E, 4)) for E in list(range(2, 15)) + [14, 9, 3]]
+    print("gpu conditional probabilities", flush=True)
+    orders, out = gpu_cond(ctx, seed, envs)
+    off = 0
+    for E, v in zip(orders, envs):
+        want = mcmcmc.conditional_probabilities(seed, v)
+        got = out[off:off + (1 << E)]
+        assert np.array_equal(got, want), E
+        off += 1 << E
+    # a negative table entry is reported with the reference's message
+    bad = mcmcmc.HistogramSeed.make(bins, n1, n2, -rows(bins), rows(n1 * n2), rows(2))
+    with pytest.raises(RuntimeError, match="MCMCMC seed density is negative or non-finite"):
+        gpu_cond(ctx, bad, envs[:2])
+    ctx.close()
+
+
+def test_driver_histogram_gpu_conditional_backend(gpu_available, tmp_path):
+    # chains with the GPU conditional-generation callback equal those with the
+    # driver's host recursion (per-chain envelopes, checkpoints included)
+    ctx = gpu_ctx()
+    gfn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    cfn = ctypes.cast(cdet_amd.lib().cdet_seed_cond_batch_host, ctypes.c_void_p)
+
+    def cfg(d):
+        return c1_config(d, seed_mode=1, N1=3, N2=3, histogram_learning_steps=128, histogram_tau_bins=8,
+                         lambda_warmup_steps=300, production_steps=400, checkpoint_interval_steps=200,
+                         chain_count=4)
+
+    h, _ = mcmcmc.learn_histogram(cfg(tmp_path), gfn, ctx.handle)
+    env = [7, 5, 10, 3]
+    print("histogram driver, host recursion", flush=True)
+    _, _, t_host = mcmcmc.run_backend(cfg(tmp_path / "h"), gfn, ctx.handle, seed=h, envelopes=env, trace_steps=700)
+    print("histogram driver, GPU recursion", flush=True)
+    _, _, t_gpu = mcmcmc.run_backend(cfg(tmp_path / "g"), gfn, ctx.handle, cond_fn=cfn, cond_user=ctx.handle,
+                                     seed=h, envelopes=env, trace_steps=700)
+    assert np.array_equal(t_host, t_gpu)
+    assert chk(tmp_path / "h") == chk(tmp_path / "g")
+    ctx.close()

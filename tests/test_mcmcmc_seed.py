@@ -331,3 +331,39 @@ def test_params_histogram_keys(tmp_path):
     assert (c.histogram_learning_steps, c.histogram_tau_bins, c.histogram_uniform_mix,
             c.histogram_pseudo_count, c.histogram_learning_only) == (77, 8, 0.3, 1.5, 1)
     assert c.histogram_output_prefix == b"hist/run" and mcmcmc.compare_uniform_seed(params)
+
+
+def test_conditional_callback_matches_host_recursion(tmp_path):
+    # a conditional-generation callback (here the host recursion through the
+    # C-ABI, as the GPU backend provides it) gives the same chains as the
+    # driver's own recursion
+    fn, orc = oracle_eval()
+    calls = []
+
+    @mcmcmc.COND_FN
+    def cond(user, seedp, verts, orders, count, out):
+        h = ctypes.cast(seedp, ctypes.POINTER(mcmcmc.HistogramSeed)).contents
+        vb = 0
+        ob = 0
+        for i in range(count):
+            E = orders[i]
+            v = np.ctypeslib.as_array(ctypes.cast(verts, ctypes.POINTER(ctypes.c_double)),
+                                      shape=((vb + E) * 4,))[vb * 4:].reshape(E, 4)
+            p = mcmcmc.conditional_probabilities(h, v)
+            for k in range(1 << E):
+                out[ob + k] = p[k]
+            vb += E
+            ob += 1 << E
+        calls.append(count)
+        return 0
+
+    c = hist_config(tmp_path / "a", checkpoint_enabled=0, production_steps=120, lambda_warmup_steps=60)
+    h, _ = mcmcmc.learn_histogram(c, fn, orc._h)
+    _, _, t_host = mcmcmc.run_backend(c, fn, orc._h, seed=h, envelopes=[4, 3], trace_steps=200)
+    _, _, t_cb = mcmcmc.run_backend(c, fn, orc._h, cond_fn=cond, seed=h, envelopes=[4, 3], trace_steps=200)
+    assert np.array_equal(t_host, t_cb) and len(calls) == 180
+    # no callback for the uniform seed
+    calls.clear()
+    _, _, t_u = mcmcmc.run_backend(c, fn, orc._h, cond_fn=cond, trace_steps=200)
+    _, _, t_u2 = mcmcmc.run_seeded(c, None, fn, orc._h, trace_steps=200)
+    assert np.array_equal(t_u, t_u2) and not calls

@@ -250,6 +250,83 @@ __device__ inline double det_minor_brute(const double* A, int n, uint32_t mask, 
     return sign < 0 ? -det : det;
 }
 
+// det_minor_brute with the k x k working matrix in registers, for n <= NM:
+// every loop runs to the compile-time bound with the rows and columns past k
+// predicated off, and the pivot row swap is a select over all rows, so no
+// register array is indexed at run time.  The arithmetic, pivot choice and
+// early zero are those of det_minor_brute, value for value.
+template <int NM>
+__device__ inline double det_minor_regs(const double* A, int n, uint32_t mask) {
+    const int k = __builtin_popcount(mask);
+    if (k == 0) return 1.0;
+    int idx[NM];
+    {
+        uint32_t mm = mask;
+#pragma unroll
+        for (int r = 0; r < NM; ++r) {
+            idx[r] = mm ? __builtin_ctz(mm) : 0;
+            mm &= mm - 1u;
+        }
+    }
+    double W[NM][NM];
+#pragma unroll
+    for (int r = 0; r < NM; ++r)
+#pragma unroll
+        for (int c = 0; c < NM; ++c) W[r][c] = (r < k && c < k) ? A[idx[r] * n + idx[c]] : 0.0;
+    double det = 1.0;
+    int sign = 1;
+    bool zero = false;
+#pragma unroll
+    for (int col = 0; col < NM; ++col) {
+        if (col < k && !zero) {
+            int prow = col;
+            double pabs = fabs(W[col][col]);
+#pragma unroll
+            for (int rr = col + 1; rr < NM; ++rr) {
+                const double cand = fabs(W[rr][col]);
+                if (rr < k && cand > pabs) {
+                    pabs = cand;
+                    prow = rr;
+                }
+            }
+            if (pabs == 0.0) {
+                zero = true;
+            } else {
+                if (prow != col) {
+#pragma unroll
+                    for (int rr = col + 1; rr < NM; ++rr) {
+                        if (rr == prow) {
+#pragma unroll
+                            for (int c = col; c < NM; ++c) {
+                                const double t = W[col][c];
+                                W[col][c] = W[rr][c];
+                                W[rr][c] = t;
+                            }
+                        }
+                    }
+                    sign = -sign;
+                }
+                const double piv = W[col][col];
+                det *= piv;
+#pragma unroll
+                for (int rr = col + 1; rr < NM; ++rr) {
+                    if (rr < k) {
+                        const double f = W[rr][col] / piv;
+                        if (f != 0.0) {
+                            W[rr][col] = 0.0;
+#pragma unroll
+                            for (int c = col + 1; c < NM; ++c)
+                                if (c < k) W[rr][c] = __builtin_fma(-f, W[col][c], W[rr][c]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (zero) return 0.0;
+    return sign < 0 ? -det : det;
+}
+
 // Maximum over the workgroup: butterfly within each wave, then across the
 // waves through red[] (exact for any order).
 __device__ inline double block_max(double v, double* red) {

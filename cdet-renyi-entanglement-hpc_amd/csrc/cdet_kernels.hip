@@ -611,6 +611,8 @@ __global__ __launch_bounds__(256) void k_fpm(KScalars ks, int n, int items, cons
 // in LDS.  Counts the fallbacks (include/cdet/runtime_counters.h:27) and
 // flags the configuration's status.
 constexpr int kDetThreads = 64;
+constexpr int kDetRegMax = 10;  // orders whose fallback determinants run in registers
+template <bool REG>
 __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
     KScalars ks, int n, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
     const double* __restrict__ g_up, const double* __restrict__ g_dn, const int32_t* __restrict__ fb_list,
@@ -652,7 +654,13 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
         __syncthreads();
         const uint32_t m = static_cast<uint32_t>(chunk) * kDetThreads + tid;
         if (m < sc) {
-            const double det = m == 0 ? 1.0 : det_minor_brute(A, n, m, ws + tid, kDetThreads);
+            double det = 1.0;
+            if (m != 0) {
+                if constexpr (REG)
+                    det = det_minor_regs<kDetRegMax>(A, n, m);
+                else
+                    det = det_minor_brute(A, n, m, ws + tid, kDetThreads);
+            }
             pm[static_cast<size_t>(4 * li + 2 * cls + spin) * sc + m] = det;
         }
     }
@@ -1027,9 +1035,15 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
             hipLaunchKernelGGL(k_fpm<false>, dim3(items), dim3(256), FpmLds::bytes(n), stream, ctx->ks, n, items,
                                cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
         }
-        const size_t det_lds = sizeof(double) * (kDetThreads + 1) * static_cast<size_t>(n) * n;
-        hipLaunchKernelGGL(k_fallback_dets, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n, cfg_list,
-                           pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+        if (n <= kDetRegMax) {
+            const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
+            hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
+                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+        } else {
+            const size_t det_lds = sizeof(double) * (kDetThreads + 1) * static_cast<size_t>(n) * n;
+            hipLaunchKernelGGL(k_fallback_dets<false>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
+                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+        }
     }
     const int items = 2 * count;
     if (cdet_gscr(n, mode)) {
@@ -1204,7 +1218,7 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     refresh_scalars(ctx);
     *out_ctx = ctx;

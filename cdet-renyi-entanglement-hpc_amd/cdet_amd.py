@@ -199,7 +199,11 @@ class Context:
             self._h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @property
     def handle(self):
@@ -313,7 +317,11 @@ class BatchPlan:
             self._h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @staticmethod
     def _ptr(x):

@@ -185,13 +185,30 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 if (rec && lane == 0) {
+                    // lane 0 folds the 64 terms of each step in descending lane
+                    // order, reading them as pairs (q - 1, q) sixteen bytes at a
+                    // time, the next eight terms in flight while eight fold
 #pragma unroll
                     for (int u = 0; u < kFoldSteps; ++u) {
+                        const double2* src = reinterpret_cast<const double2*>(wtb + u * nt);
+                        double2 cur[4], nxt[4];
 #pragma unroll
-                        for (int q = 63; q >= 0; --q) {
-                            const double v = wtb[u * nt + q];
-                            acc = acc - v;
-                            rs += fabs(v);
+                        for (int i = 0; i < 4; ++i) cur[i] = src[31 - i];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            if (c + 1 < 8) {
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) nxt[i] = src[31 - 4 * (c + 1) - i];
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                acc = acc - cur[i].y;
+                                rs += fabs(cur[i].y);
+                                acc = acc - cur[i].x;
+                                rs += fabs(cur[i].x);
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
                         }
                     }
                 }

@@ -678,11 +678,13 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
                                               const int* __restrict__ pop_off, double* __restrict__ gscratch,
                                               size_t gscratch_stride, int32_t* __restrict__ fb_count) {
     extern __shared__ double lds_dyn[];
-    __shared__ double tb[kFoldSteps * 256];
+    __shared__ __attribute__((aligned(16))) double tb[kFoldSteps * 256];
+    __shared__ int s_popoff[kMaxOrder + 2];  // popcount-layer offsets, read every layer
     // The bucket's fallback queue has been consumed (k_fallback_dets ran
     // before); clear its length for the next run.
     if (fb_count && blockIdx.x == 0 && threadIdx.x == 0) *fb_count = 0;  // wave-wide chain terms of the subset recursion
     const int tid = threadIdx.x;
+    for (int k = tid; k < kMaxOrder + 2; k += blockDim.x) s_popoff[k] = pop_off[k];
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
     const int n2 = n * n;
@@ -723,7 +725,7 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
         if (mode == 0 || mode == 2) {
             // mode 2 writes the magnitude scale of each value instead of the value:
             // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
-            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off, mode == 2 ? rhs : nullptr, false, tb);
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, false, tb);
             // Assembly (:833-877).
             double* dst = (cls == 0 ? out : out_ord) + cfg_obase[c];
             bool bad = false;
@@ -744,7 +746,7 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
             }
             if (bad && status) atomicOr(&status[c], 1);
         } else {
-            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, pop_off, nullptr, true, tb);
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, nullptr, true, tb);
             const double cres = dp[sc - 1];
             if (tid == 0) {
                 const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});

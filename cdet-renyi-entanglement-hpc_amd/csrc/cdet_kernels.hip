@@ -58,6 +58,7 @@ struct DevTable {
     double* ucorner = nullptr;
     double* lcorner = nullptr;
     float* split[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [branch][hi, lo]
+    double* steps[2] = {nullptr, nullptr};  // [branch] coefficients in matrix-core step order
     int alias = -1;  // kind whose storage this kind shares
 };
 
@@ -789,6 +790,24 @@ __global__ void k_split_f32(const double* __restrict__ c, int64_t n, float* __re
     lo[i] = static_cast<float>(x - static_cast<double>(h));
 }
 
+// Coefficients in the order the FP64 matrix-core contraction reads them.  A
+// block (nu x nv, row-major) becomes (nu / 16) (nv / 16) steps of 256 values,
+// step m (nv / 16) + t holding rows 16 m.., columns 16 t..; within a step,
+// element (row i, column 4 r + g) sits at 64 r + 16 g + i, so the A operand
+// of MFMA step r is one contiguous 512-byte read by lane 16 g + i.
+__global__ void k_step_order(const double* __restrict__ c, int64_t n, int nu, int nv, double* __restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int64_t per = static_cast<int64_t>(nu) * nv;
+    const int64_t b = e / per;
+    const int w = static_cast<int>(e - b * per);
+    const int row = w / nv;
+    const int col = w - row * nv;
+    const int step = (row >> 4) * (nv >> 4) + (col >> 4);
+    const int c16 = col & 15;
+    out[b * per + step * 256 + 64 * (c16 >> 2) + 16 * (c16 & 3) + (row & 15)] = c[e];
+}
+
 // Fill kernel (graph-capture safe replacement of hipMemsetAsync).
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -810,6 +829,7 @@ struct cdet_ctx {
     cdet_params p{};
     int device = 0;
     int ncu = 256;  // compute units of the device
+    uint64_t tables_gen = 0;  // bumped whenever the kernels' table pointers change
     hipStream_t stream = nullptr;
     DevTable tab[4];
     KScalars ks{};
@@ -906,6 +926,7 @@ HDD hdd_div(HDD x, HDD y) {
 }
 
 void refresh_scalars(cdet_ctx* ctx) {
+    ++ctx->tables_gen;
     KScalars& k = ctx->ks;
     const cdet_params& p = ctx->p;
     k.N1 = p.N1;
@@ -950,6 +971,8 @@ void refresh_scalars(cdet_ctx* ctx) {
         k.coeff[kind][1] = t.lower;
         k.corner[kind][0] = t.ucorner;
         k.corner[kind][1] = t.lcorner;
+        k.steps[kind][0] = t.steps[0];
+        k.steps[kind][1] = t.steps[1];
         k.split_hi[kind][0] = t.split[0][0];
         k.split_lo[kind][0] = t.split[0][1];
         k.split_hi[kind][1] = t.split[1][0];
@@ -1082,6 +1105,35 @@ static int mfma_tile_queries(const KScalars& ks, int mode) {
     return (mode == CDET_MODE_MFMA64 && use_f64_v2(ks)) ? kF64Tile : kMfmaTile;
 }
 
+static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner);
+
+// Step-order copies of every resident table for the FP64 matrix-core
+// contraction (nu, nv multiples of 16), built once per table.
+static int ensure_step_tables(cdet_ctx* ctx) {
+    if (!use_f64_v2(ctx->ks)) return 0;
+    size_t nc, nk;
+    table_sizes(ctx, nc, nk);
+    bool made = false;
+    for (int kind = 0; kind < 4; ++kind) {
+        DevTable& t = ctx->tab[kind];
+        if (t.alias >= 0 || !t.upper) continue;
+        for (int br = 0; br < 2; ++br) {
+            if (t.steps[br]) continue;
+            HIP_TRY(hipMalloc(&t.steps[br], nc * sizeof(double)));
+            hipLaunchKernelGGL(k_step_order, dim3(static_cast<unsigned>((nc + 255) / 256)), dim3(256), 0, ctx->stream,
+                               br == 0 ? t.upper : t.lower, static_cast<int64_t>(nc), ctx->p.cheb_nu, ctx->p.cheb_nv,
+                               t.steps[br]);
+            HIP_TRY(hipGetLastError());
+            made = true;
+        }
+    }
+    if (made) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        refresh_scalars(ctx);
+    }
+    return 0;
+}
+
 // FP32 hi/lo copies of every resident table (split-FP32 mode), built once.
 static int ensure_split_tables(cdet_ctx* ctx) {
     size_t nc, nk;
@@ -1136,7 +1188,6 @@ static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles) {
     return static_cast<int>(g);
 }
 
-static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner);
 static int alloc_table(cdet_ctx* ctx, int kind);
 static void invalidate_split(DevTable& t);
 
@@ -1236,8 +1287,7 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
             (void)hipFree(t.lower);
             (void)hipFree(t.ucorner);
             (void)hipFree(t.lcorner);
-            for (auto& b : t.split)
-                for (float* f : b) (void)hipFree(f);
+            invalidate_split(t);
         }
     }
     (void)hipFree(ctx->d_masks);
@@ -1275,13 +1325,18 @@ static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner) {
     return 0;
 }
 
-// Drop the FP32 split copies derived from a table whose values change.
+// Drop the copies (FP32 split, step order) derived from a table whose values
+// change.
 static void invalidate_split(DevTable& t) {
     for (auto& b : t.split)
         for (float*& f : b) {
             (void)hipFree(f);
             f = nullptr;
         }
+    for (double*& d : t.steps) {
+        (void)hipFree(d);
+        d = nullptr;
+    }
 }
 
 static int alloc_table(cdet_ctx* ctx, int kind) {
@@ -1331,8 +1386,7 @@ int cdet_tables_alias(cdet_ctx* ctx, int dst, int src) {
         (void)hipFree(t.lower);
         (void)hipFree(t.ucorner);
         (void)hipFree(t.lcorner);
-        for (auto& b : t.split)
-            for (float* f : b) (void)hipFree(f);
+        invalidate_split(t);
     }
     t = DevTable{};
     t.alias = src;

@@ -265,11 +265,12 @@ constexpr int kF64Tile = 64;  // queries per wave in k_contract_f64
 // One tile of k_contract_f64 with NT live 16-wide N tiles (16 (NT-1) < len <=
 // 16 NT).  K is visited in groups of 16; MFMA step r of group t uses
 // k = 16 t + 4 r + g on both operands, so each lane's K basis is one stride-4
-// Chebyshev sequence (BasisK) and its four coefficient loads of a group read
-// C[row][16 t + 4 r + g], r = 0..3 (32 contiguous bytes per row and step).
-// Accumulator register r holds row 16 m + 4 r + g, again a stride-4 sequence
-// (BasisK over u).  The next group's coefficients are loaded before the
-// current group's MFMAs; every N tile reuses each A fragment.
+// Chebyshev sequence (BasisK).  The coefficients are read from the
+// step-order copy of the table (k_step_order): the four A values of a step
+// are four contiguous 512-byte wave reads.  Accumulator register r holds row
+// 16 m + 4 r + g, again a stride-4 sequence (BasisK over u).  The next group's
+// coefficients are loaded before the current group's MFMAs; every N tile
+// reuses each A fragment.
 template <bool SAME, int NT>
 __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const double* __restrict__ cu,
                                          const double* __restrict__ cd, int32_t start, int32_t len,
@@ -291,25 +292,21 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
     double G[NT], Gd[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) G[nt] = Gd[nt] = 0.0;
-    // The (slab, group) pairs form one stream of mslabs * groups steps; the
-    // coefficients of later steps are loaded (rotating registers, fenced from
-    // the scheduler) while step k runs, across slab boundaries too.
+    // The (slab, group) pairs form one stream of mslabs * groups steps, 256
+    // values each in step order; the coefficients of later steps are loaded
+    // (rotating registers, fenced from the scheduler) while step k runs.
     const int total = mslabs * groups;
-    int64_t loff = static_cast<int64_t>(j) * nv + g;  // next load: row 16 m + j, column 16 t + g
-    int lk = 0, lt = 0;
+    int64_t loff = 16 * g + j;  // next load: step lk, value 64 r + lane
+    int lk = 0;
     auto load_next = [&](double* x, double* y) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            x[r] = cu[loff + 4 * r];
-            if (!SAME) y[r] = cd[loff + 4 * r];
+            x[r] = cu[loff + 64 * r];
+            if (!SAME) y[r] = cd[loff + 64 * r];
         }
         if (lk + 1 < total) {  // past the end the last step is read again
             ++lk;
-            loff += 16;
-            if (++lt == groups) {
-                lt = 0;
-                loff += 15 * static_cast<int64_t>(nv);
-            }
+            loff += 256;
         }
     };
     d4 acc[NT], accd[NT];
@@ -407,7 +404,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SAME ? 4 : 
         const double* cu;
         const double* cd;
         bool same;
-        ks.key_tables(key, cu, cd, same);
+        ks.key_steps(key, cu, cd, same);
         const int nts = (len + 15) >> 4;
         if (nts == 4)
             contract_f64_tile<SAME, 4>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);

@@ -47,11 +47,20 @@ struct cdet_batch_plan {
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    uint64_t g_tables = 0;  // table generation the graph was captured with
     const void* g_verts = nullptr;
     void* g_out = nullptr;
     void* g_status = nullptr;
     hipStream_t g_stream = nullptr;
 };
+
+// The derived table copies the plan's contraction reads (split FP32, step
+// order), made before a run when the tables changed since.
+static int plan_tables(cdet_batch_plan* p) {
+    if (p->contract_mode == CDET_MODE_SPLIT32) return ensure_split_tables(p->ctx);
+    if (p->contract_mode == CDET_MODE_MFMA64) return ensure_step_tables(p->ctx);
+    return 0;
+}
 
 static void plan_free(cdet_batch_plan* p) {
     if (!p) return;
@@ -245,12 +254,10 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
         plan_free(p);
         return rc;
     }
-    if (mode == CDET_MODE_SPLIT32) {
-        rc = ensure_split_tables(ctx);
-        if (rc) {
-            plan_free(p);
-            return rc;
-        }
+    rc = plan_tables(p);
+    if (rc) {
+        plan_free(p);
+        return rc;
     }
     *out = p;
     return 0;
@@ -269,13 +276,16 @@ int64_t cdet_batch_query_count(const cdet_batch_plan* plan) { return plan ? plan
 int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
                    int use_graph, void* stream) {
     if (!p || !d_vertices || !d_out) return fail(CDET_ERR_ARGUMENT, "cdet_batch_run: bad argument");
+    int trc = plan_tables(p);
+    if (trc) return trc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
     if (!use_graph) {
         int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
         if (!rc) p->ctx->calls += static_cast<uint64_t>(p->count);
         return rc;
     }
-    if (!p->graph_exec || p->g_verts != d_vertices || p->g_out != d_out || p->g_status != d_status || p->g_stream != s) {
+    if (!p->graph_exec || p->g_verts != d_vertices || p->g_out != d_out || p->g_status != d_status || p->g_stream != s ||
+        p->g_tables != p->ctx->tables_gen) {
         if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
         if (p->graph) (void)hipGraphDestroy(p->graph);
         p->graph_exec = nullptr;
@@ -295,6 +305,7 @@ int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_
         p->g_out = d_out;
         p->g_status = d_status;
         p->g_stream = s;
+        p->g_tables = p->ctx->tables_gen;
     }
     HIP_TRY(hipGraphLaunch(p->graph_exec, s));
     p->ctx->calls += static_cast<uint64_t>(p->count);
@@ -304,6 +315,8 @@ int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_
 int cdet_batch_profile(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
                        void* stream, float* stage_ms) {
     if (!p || !d_vertices || !d_out || !stage_ms) return fail(CDET_ERR_ARGUMENT, "cdet_batch_profile: bad argument");
+    int trc = plan_tables(p);
+    if (trc) return trc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
     hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
@@ -365,8 +378,8 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
         return fail(CDET_ERR_ARGUMENT, "cdet_green_pair_eval: bad argument");
     if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
     if (count >= (int64_t(1) << 31)) return fail(CDET_ERR_CAPACITY, "too many lookups for one call");
-    if (mode == CDET_MODE_SPLIT32) {
-        int rc = ensure_split_tables(ctx);
+    if (mode == CDET_MODE_SPLIT32 || mode == CDET_MODE_MFMA64) {
+        int rc = mode == CDET_MODE_SPLIT32 ? ensure_split_tables(ctx) : ensure_step_tables(ctx);
         if (rc) return rc;
     }
     QueryScratch& q = query_scratch(ctx);

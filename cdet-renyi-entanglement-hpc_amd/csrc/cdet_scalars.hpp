@@ -18,6 +18,7 @@ struct KScalars {
     int same_spin[2];                  // class up/down tables and shifts identical
     const double* coeff[4][2];         // [kind][branch 0 = upper, 1 = lower]
     const double* corner[4][2];
+    const double* steps[4][2];         // coeff in matrix-core step order (FP64 MFMA mode)
     const float* split_hi[4][2];       // FP32 hi/lo split of coeff (split-FP32 mode)
     const float* split_lo[4][2];
 
@@ -31,6 +32,18 @@ struct KScalars {
         const int64_t base = sp * static_cast<int64_t>(nu) * nv;
         cu = coeff[ku][br] + base;
         cd = coeff[ku + 1][br] + base;
+        same = same_spin[cls] != 0;
+    }
+    // The same block in matrix-core step order.
+    __device__ inline void key_steps(int32_t key, const double*& cu, const double*& cd, bool& same) const {
+        const int64_t sp = key % SP;
+        const int rest = static_cast<int>(key / SP);
+        const int br = rest & 1;
+        const int cls = rest >> 1;
+        const int ku = cls == 0 ? CDET_TABLE_TWISTED_UP : CDET_TABLE_ORDINARY_UP;
+        const int64_t base = sp * static_cast<int64_t>(nu) * nv;
+        cu = steps[ku][br] + base;
+        cd = steps[ku + 1][br] + base;
         same = same_spin[cls] != 0;
     }
     __device__ inline void key_split(int32_t key, const float*& hu, const float*& lu, const float*& hd,

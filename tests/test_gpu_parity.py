@@ -167,6 +167,66 @@ def test_batched_mfma64_within_tolerance(gpu_available, name):
     assert worst < 1e-9
 
 
+def test_mfma64_plan_follows_table_changes(gpu_available):
+    # The FP64 matrix-core contraction reads step-order copies of the tables;
+    # tables uploaded after the plan (and its captured graph) exist must be
+    # the ones a later run reads, eager and graph-replayed alike.
+    import torch
+    from oracle import oracle as O
+    name = "c1"
+    s = helpers.SETS[name]
+    op = helpers.params_of(name)
+    ctx = cdet_amd.Context(cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"],
+                                                s["alpha_up"], s["alpha_down"],
+                                                derived=(op.scale, op.diag_up_shift, op.diag_down_shift)))
+    base = helpers.tables_of(name)
+    ctx.upload_tables(base)
+    orders = [3, 5, 6, 8]
+    configs = helpers.seeded_configs(6100, orders)
+    plan = ctx.plan(orders, mode="mfma64")
+    verts = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    _, ob = cdet_amd.packed_offsets(orders, False)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def check(tables, graph):
+        orc = O.Oracle(op, tables)
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run(verts, out, None, use_graph=graph, stream=stream)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        for k, v in enumerate(configs):
+            want = orc.all_subsets(v)
+            scale = np.maximum(np.abs(want), 1e-6 * np.max(np.abs(want)))
+            assert np.max(np.abs(got[ob[k]:ob[k + 1]] - want) / scale) < 1e-9, (graph, k)
+
+    say("mfma64 plan, first tables")
+    check(base, True)
+    made = {}  # keep the kinds that share a table sharing it (uploads alias them)
+    for t in base:
+        if id(t) not in made:
+            made[id(t)] = O.Table(t.upper * (1.0 + 1e-3), t.lower * (1.0 - 1e-3), t.ucorner, t.lcorner)
+    bent = [made[id(t)] for t in base]
+    # a kind another kind aliases cannot be uploaded: give the aliasing kinds
+    # their own copy first, upload the sources, then alias again
+    src = {k: next(i for i in range(4) if base[i] is base[k]) for k in range(4)}
+    for k in range(4):
+        if src[k] != k:
+            ctx.upload_table(k, bent[k].upper, bent[k].lower, bent[k].ucorner, bent[k].lcorner)
+    for k in range(4):
+        if src[k] == k:
+            ctx.upload_table(k, bent[k].upper, bent[k].lower, bent[k].ucorner, bent[k].lcorner)
+    for k in range(4):
+        if src[k] != k:
+            ctx.alias_table(k, src[k])
+    say("mfma64 plan, tables uploaded after the plan")
+    check(bent, True)
+    check(bent, False)
+    plan.close()
+    ctx.close()
+
+
 @pytest.mark.parametrize("name", ["c1", "c1_neq"])
 def test_conditioning_scale_bounds_values_and_mfma_error(gpu_available, name):
     # The conditioning scale of a value is the sum of |a[mask]| and |terms| of

@@ -794,18 +794,23 @@ __global__ void k_split_f32(const double* __restrict__ c, int64_t n, float* __re
 // block (nu x nv, row-major) becomes (nu / 16) (nv / 16) steps of 256 values,
 // step m (nv / 16) + t holding rows 16 m.., columns 16 t..; within a step,
 // element (row i, column 4 r + g) sits at 64 r + 16 g + i, so the A operand
-// of MFMA step r is one contiguous 512-byte read by lane 16 g + i.
+// of MFMA step r is one contiguous 512-byte read by lane 16 g + i.  One
+// thread per output value (contiguous writes); the reads of a wave cover 16
+// rows x 64 bytes.
 __global__ void k_step_order(const double* __restrict__ c, int64_t n, int nu, int nv, double* __restrict__ out) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const int64_t per = static_cast<int64_t>(nu) * nv;
     const int64_t b = e / per;
     const int w = static_cast<int>(e - b * per);
-    const int row = w / nv;
-    const int col = w - row * nv;
-    const int step = (row >> 4) * (nv >> 4) + (col >> 4);
-    const int c16 = col & 15;
-    out[b * per + step * 256 + 64 * (c16 >> 2) + 16 * (c16 & 3) + (row & 15)] = c[e];
+    const int step = w >> 8;
+    const int k = w & 255;
+    const int groups = nv >> 4;
+    const int m = step / groups;
+    const int t = step - m * groups;
+    const int row = 16 * m + (k & 15);
+    const int col = 16 * t + 4 * (k >> 6) + ((k >> 4) & 3);
+    out[e] = c[b * per + static_cast<int64_t>(row) * nv + col];
 }
 
 // Fill kernel (graph-capture safe replacement of hipMemsetAsync).

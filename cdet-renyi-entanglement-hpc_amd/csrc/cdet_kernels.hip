@@ -1106,8 +1106,16 @@ static int launch_combine(int mode, int64_t count, int64_t out_total, const int6
 // The 64-query FP64 kernel needs both coefficient counts to be multiples of
 // 16; other shapes use the 32-query kernel.
 static bool use_f64_v2(const KScalars& ks) { return ks.nu % 16 == 0 && ks.nv % 16 == 0; }
-static int mfma_tile_queries(const KScalars& ks, int mode) {
-    return (mode == CDET_MODE_MFMA64 && use_f64_v2(ks)) ? kF64Tile : kMfmaTile;
+// Lookups per tile.  The FP64 matrix-core contraction takes tiles of up to
+// kF64Tile lookups when a batch has enough of them to give every resident
+// wave (4 per SIMD) about three tiles; smaller batches take tiles of 16
+// lookups, so that more, smaller tiles spread the matrix-core work evenly
+// over the SIMDs (each extra tile re-reads its coefficient block, which the
+// large batches cannot afford).
+static int mfma_tile_queries(const KScalars& ks, int mode, int64_t lookups, int ncu) {
+    if (mode != CDET_MODE_MFMA64 || !use_f64_v2(ks)) return kMfmaTile;
+    const int64_t resident_waves = 16 * static_cast<int64_t>(ncu);
+    return lookups / kF64Tile >= 3 * resident_waves ? kF64Tile : 16;
 }
 
 static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner);

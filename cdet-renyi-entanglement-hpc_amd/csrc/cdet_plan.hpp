@@ -160,7 +160,6 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     }
     p->eval_kind = eval_kind;
     p->count = count;
-    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks, mode);
     std::vector<int64_t> vb(static_cast<size_t>(count)), pb(static_cast<size_t>(count)), ob(static_cast<size_t>(count));
     p->bucket_cnt.assign(kMaxOrder + 1, 0);
     p->bucket_off.assign(kMaxOrder + 2, 0);
@@ -196,6 +195,7 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
         std::vector<int> fillpos(p->bucket_off.begin(), p->bucket_off.end());
         for (int64_t c = 0; c < count; ++c) lists[static_cast<size_t>(fillpos[static_cast<size_t>(orders[c])]++)] = static_cast<int32_t>(c);
     }
+    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks, mode, p->nq, ctx->ncu);
     p->nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
     const int64_t nonempty = std::min<int64_t>(p->nkeys, p->nq);
     p->max_tiles = p->nq / p->tile + nonempty + 1;
@@ -384,7 +384,7 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     }
     QueryScratch& q = query_scratch(ctx);
     const int32_t nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
-    const int tile = mode == CDET_MODE_EXACT ? 64 : mfma_tile_queries(ctx->ks, mode);
+    const int tile = mode == CDET_MODE_EXACT ? 64 : mfma_tile_queries(ctx->ks, mode, count, ctx->ncu);
     const int64_t max_tiles = count / tile + std::min<int64_t>(nkeys, count) + 1;
     if (q.cap < count || q.tiles_cap < max_tiles || !q.hist) {
         void* bufs[] = {q.gup, q.gdn, q.qu, q.qv, q.su, q.sv, q.qkey, q.qrank, q.sdest, q.hist, q.keyoff,

@@ -356,3 +356,35 @@ def test_green_pair_eval(gpu_available, mode, name):
         assert bits_equal(got, want)
     else:
         assert np.max(np.abs(got - want)) < 1e-13
+
+
+def test_green_pair_eval_large_batch_tiles(gpu_available):
+    # 800,000 lookups: enough that the FP64 matrix-core contraction takes tiles
+    # of up to 64 lookups (1-4 N tiles per wave) instead of 16; checked against
+    # the bit-identical FP64 path (itself pinned to the oracle above) at the
+    # table-level bound of the MFMA form.
+    import ctypes
+    import torch
+    ctx = gpu_ctx("c1")
+    rng = np.random.default_rng(11)
+    n = 800_000
+    qi = np.zeros((n, 7), dtype=np.int32)
+    qi[:, 0] = rng.integers(0, 2, n)
+    qi[:, 1:3] = rng.integers(0, 2, (n, 2))
+    qi[qi[:, 0] == 1, 1:3] = 0
+    qi[:, 3:7] = rng.integers(0, 3, (n, 4))
+    qt = rng.uniform(0, 4.0, (n, 2))
+    d_qi = torch.tensor(qi, device="cuda")
+    d_qt = torch.tensor(qt, device="cuda")
+    res = {}
+    for mode in ("exact", "mfma64"):
+        say(f"green pair eval, {n} lookups, {mode}")
+        out = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+        rc = cdet_amd.lib().cdet_green_pair_eval(ctx.handle, ctypes.c_void_p(d_qi.data_ptr()),
+                                                 ctypes.c_void_p(d_qt.data_ptr()), n,
+                                                 ctypes.c_void_p(out.data_ptr()), cdet_amd.MODES[mode], None)
+        assert rc == 0, cdet_amd._err()
+        torch.cuda.synchronize()
+        res[mode] = out.cpu().numpy()
+    assert np.max(np.abs(res["mfma64"] - res["exact"])) < 1e-13
+

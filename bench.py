@@ -66,6 +66,8 @@ def parse():
                          "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice; "
                          "c5: 1024 configurations at orders 2..14 on the 10x10 lattice")
     ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
+    ap.add_argument("--sweep-groups", type=int, default=2, choices=[1, 2],
+                    help="c4/c5 sweeps: chain groups in flight (2: two GPU streams, submit/wait; 1: one batch)")
     ap.add_argument("--c5-sweeps", type=int, default=200, help="c5: lockstep sweeps of the 1024 chains (0: none)")
     return ap.parse_args()
 
@@ -292,9 +294,13 @@ C4 = dict(N1=6, N2=6, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="left_col
           chains=256, envelope=12, base_seed=2026101401)
 
 
-def run_sweeps(ctx, w, envelopes, sweeps, order_max):
+def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=2):
     """Lockstep MCMCMC sweeps of len(envelopes) chains through the GPU batch
-    callback, uniform seed.  Chain c runs at envelope max(envelopes[c], 3) and
+    callback, uniform seed.  groups=2: the chains advance as two groups whose
+    batches alternate on two GPU streams (cdet_eval_batch_submit / wait), so
+    one group's heat-bath steps overlap the other's evaluation; groups=1: one
+    batch of all chains per sweep (cdet_eval_batch_host).  The chains' steps
+    are the same either way.  Chain c runs at envelope max(envelopes[c], 3) and
     measures orders 2..envelope-1, normalised to order 2: at mu - alpha = 0 the
     order-1 integrand vanishes identically, so an envelope-2 chain would have
     no nonzero weight (the sampler stops with "MCMCMC local normalization is
@@ -312,9 +318,12 @@ def run_sweeps(ctx, w, envelopes, sweeps, order_max):
     c.lambda_warmup_steps, c.lambda_adapt_interval, c.lambda_adapt_rate = warm, max(1, warm // 5), 0.25
     c.production_steps, c.chain_count, c.base_seed = sweeps - warm, len(envelopes), w["base_seed"]
     c.seed_mode, c.checkpoint_enabled, c.resume_mode = 0, 0, 0
-    fn = ctypes.cast(cdet_amd.lib().cdet_eval_batch_host, ctypes.c_void_p)
+    L = cdet_amd.lib()
+    fn = ctypes.cast(L.cdet_eval_batch_host, ctypes.c_void_p)
+    sub = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p) if groups == 2 else None
+    wt = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p) if groups == 2 else None
     t = time.perf_counter()
-    _, ratios, _ = mcmcmc.run_envelopes(c, envelopes, fn, ctx.handle)
+    _, ratios, _ = mcmcmc.run_backend(c, fn, ctx.handle, envelopes=envelopes, submit_fn=sub, wait_fn=wt)
     return time.perf_counter() - t, ratios, warm
 
 
@@ -360,7 +369,7 @@ def main_c4(args):
     plan.close()
     print(f"c4: GPU batch of {w['chains']} envelopes (orders 2..{E}) {batch_ms:.3f} ms", file=sys.stderr,
           flush=True)
-    dt, ratios, warm = run_sweeps(ctx, w, orders, args.sweeps, E - 1)
+    dt, ratios, warm = run_sweeps(ctx, w, orders, args.sweeps, E - 1, args.sweep_groups)
     sweeps = args.sweeps
     line = {
         "metric": "C4 MCMCMC sweeps/s (6x6, beta=8, Nc=128, 256 chains, envelopes 2..12, lockstep batches)",
@@ -369,7 +378,9 @@ def main_c4(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "6x6 Hubbard, beta=8, x<3 half-strip subsystem, 128x128 Chebyshev; 256 chains, "
                                "envelope max(3, 2 + floor(11 u^2)) per chain, orders 2..envelope-1, "
-                               "uniform seed, one GPU batch per sweep",
+                               "uniform seed, " + ("two chain groups in flight (a GPU batch per group per sweep)"
+                                                   if args.sweep_groups == 2 else "one GPU batch per sweep"),
+                   "sweep_groups": args.sweep_groups,
                    "envelopes": {str(n): int(np.count_nonzero(orders == n)) for n in range(2, E + 1)},
                    "mode": mode, "table_build_s": round(build_s, 2)},
         "envelope_evals_per_s": round(sweeps * w["chains"] / dt, 1),
@@ -448,7 +459,8 @@ def main_c5(args):
     sweep_s, sweep_ratios = None, None
     if args.c5_sweeps > 0:
         print(f"c5: {args.c5_sweeps} sweeps of {w['configs']} chains", file=sys.stderr, flush=True)
-        sweep_s, sweep_ratios, _ = run_sweeps(ctx, w, orders, args.c5_sweeps, w["order_max"] - 1)
+        sweep_s, sweep_ratios, _ = run_sweeps(ctx, w, orders, args.c5_sweeps, w["order_max"] - 1,
+                                                 args.sweep_groups)
     # one lookup reads one nu x nv coefficient block per table (tables are
     # shared by the two spins here); upper bound on the contraction's bytes
     block = w["nc"] * w["nc"] * 8
@@ -468,6 +480,7 @@ def main_c5(args):
         "orders": {str(n): int(np.count_nonzero(orders == n)) for n in range(w["order_min"], w["order_max"] + 1)},
         "mfma64_vs_exact": per_order,
         "sweeps_per_s": round(args.c5_sweeps / sweep_s, 2) if sweep_s else None, "sweeps": args.c5_sweeps,
+        "sweep_groups": args.sweep_groups,
         "sweep_ratio_order4_over_order2": (float(sweep_ratios[4]) if sweep_ratios is not None
                                            and np.isfinite(sweep_ratios[4]) else None),
     }

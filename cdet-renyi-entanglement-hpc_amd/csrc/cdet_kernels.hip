@@ -858,6 +858,28 @@ struct cdet_ctx {
     int batch_mode = CDET_MODE_EXACT;
     cdet_batch_plan* host_plan = nullptr;
     std::vector<int32_t> host_plan_orders;
+    // cdet_eval_batch_submit / wait: two slots, each with its own plan cache,
+    // stream, device buffers and pinned host staging
+    struct HostSlot {
+        cdet_batch_plan* plan = nullptr;
+        std::vector<int32_t> orders;
+        hipStream_t stream = nullptr;
+        void* d_v = nullptr;
+        size_t d_v_cap = 0;
+        void* d_out = nullptr;
+        size_t d_out_cap = 0;
+        void* d_status = nullptr;
+        size_t d_status_cap = 0;
+        void* h_v = nullptr;  // pinned
+        size_t h_v_cap = 0;
+        void* h_out = nullptr;  // pinned
+        size_t h_out_cap = 0;
+        void* h_status = nullptr;  // pinned
+        size_t h_status_cap = 0;
+        int64_t count = 0;
+        size_t out_bytes = 0;
+        bool pending = false;
+    } slots[2];
     void* d_hv = nullptr;
     size_t d_hv_cap = 0;
     void* d_hout = nullptr;
@@ -1318,6 +1340,17 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_zero64);
     (void)hipFree(ctx->plain_scratch.ptr);
     if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
+    for (auto& sl : ctx->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.plan) cdet_batch_plan_destroy(sl.plan);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        (void)hipFree(sl.d_v);
+        (void)hipFree(sl.d_out);
+        (void)hipFree(sl.d_status);
+        (void)hipHostFree(sl.h_v);
+        (void)hipHostFree(sl.h_out);
+        (void)hipHostFree(sl.h_status);
+    }
     (void)hipFree(ctx->d_hv);
     (void)hipFree(ctx->d_hout);
     (void)hipFree(ctx->d_hstatus);

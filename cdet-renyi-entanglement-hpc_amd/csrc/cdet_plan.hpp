@@ -453,6 +453,14 @@ extern "C" int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode) {
         ctx->host_plan = nullptr;
         ctx->host_plan_orders.clear();
     }
+    if (ctx->batch_mode != mode) {
+        for (auto& sl : ctx->slots) {
+            if (sl.pending) return fail(CDET_ERR_ARGUMENT, "cdet_ctx_set_batch_mode: a submitted batch is pending");
+            if (sl.plan) cdet_batch_plan_destroy(sl.plan);
+            sl.plan = nullptr;
+            sl.orders.clear();
+        }
+    }
     ctx->batch_mode = mode;
     return 0;
 }
@@ -492,3 +500,74 @@ extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, c
         if (x & 1) return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
     return 0;
 }
+
+// Pinned host staging that only grows.
+static int grow_pinned(void** p, size_t& cap, size_t need) {
+    if (need <= cap) return 0;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    cap = 0;
+    HIP_TRY(hipHostMalloc(p, need, hipHostMallocDefault));
+    cap = need;
+    return 0;
+}
+
+// Asynchronous form of cdet_eval_batch_host over two slots: the vertices are
+// copied into the slot's pinned staging, the plan runs (graph) on the slot's
+// stream, and the values come back to pinned memory; cdet_eval_batch_wait
+// synchronizes the slot and copies them out.
+extern "C" int cdet_eval_batch_submit(void* user, int slot, const cdet_vertex* h_vertices, const int32_t* h_orders,
+                                      int64_t count) {
+    cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
+    if (!ctx || slot < 0 || slot > 1 || !h_vertices || !h_orders || count <= 0)
+        return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_submit: bad argument");
+    auto& sl = ctx->slots[slot];
+    if (sl.pending) return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_submit: the slot's previous batch was not waited for");
+    if (!sl.stream) HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+    const bool same = sl.plan && static_cast<int64_t>(sl.orders.size()) == count &&
+                      std::equal(sl.orders.begin(), sl.orders.end(), h_orders);
+    if (!same) {
+        if (sl.plan) cdet_batch_plan_destroy(sl.plan);
+        sl.plan = nullptr;
+        int rc = cdet_batch_plan_create(ctx, h_orders, count, ctx->batch_mode, 0, &sl.plan);
+        if (rc) return rc;
+        sl.orders.assign(h_orders, h_orders + count);
+    }
+    cdet_batch_plan* p = sl.plan;
+    const size_t vbytes = sizeof(cdet_vertex) * static_cast<size_t>(p->vert_total);
+    const size_t obytes = sizeof(double) * static_cast<size_t>(p->out_total);
+    const size_t sbytes = sizeof(int32_t) * static_cast<size_t>(count);
+    int rc = grow(&sl.d_v, sl.d_v_cap, vbytes);
+    if (!rc) rc = grow(&sl.d_out, sl.d_out_cap, obytes);
+    if (!rc) rc = grow(&sl.d_status, sl.d_status_cap, sbytes);
+    if (!rc) rc = grow_pinned(&sl.h_v, sl.h_v_cap, vbytes);
+    if (!rc) rc = grow_pinned(&sl.h_out, sl.h_out_cap, obytes);
+    if (!rc) rc = grow_pinned(&sl.h_status, sl.h_status_cap, sbytes);
+    if (rc) return rc;
+    std::memcpy(sl.h_v, h_vertices, vbytes);
+    HIP_TRY(hipMemcpyAsync(sl.d_v, sl.h_v, vbytes, hipMemcpyHostToDevice, sl.stream));
+    rc = cdet_batch_run(p, static_cast<const cdet_vertex*>(sl.d_v), static_cast<double*>(sl.d_out),
+                        static_cast<int32_t*>(sl.d_status), 1, sl.stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(sl.h_out, sl.d_out, obytes, hipMemcpyDeviceToHost, sl.stream));
+    HIP_TRY(hipMemcpyAsync(sl.h_status, sl.d_status, sbytes, hipMemcpyDeviceToHost, sl.stream));
+    sl.count = count;
+    sl.out_bytes = obytes;
+    sl.pending = true;
+    return 0;
+}
+
+extern "C" int cdet_eval_batch_wait(void* user, int slot, double* h_out) {
+    cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
+    if (!ctx || slot < 0 || slot > 1 || !h_out) return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_wait: bad argument");
+    auto& sl = ctx->slots[slot];
+    if (!sl.pending) return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_wait: no batch was submitted to the slot");
+    sl.pending = false;
+    HIP_TRY(hipStreamSynchronize(sl.stream));
+    const int32_t* st = static_cast<const int32_t*>(sl.h_status);
+    for (int64_t i = 0; i < sl.count; ++i)
+        if (st[i] & 1) return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
+    std::memcpy(h_out, sl.h_out, sl.out_bytes);
+    return 0;
+}
+

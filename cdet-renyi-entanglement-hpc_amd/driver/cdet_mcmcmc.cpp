@@ -958,49 +958,113 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
                                                        static_cast<size_t>(ch.cfg.envelope_order));
     const int threads = host_work >= 32768 ? std::min(host_thread_count(c), c.chain_count) : 1;
     WorkPool pool(threads);
-    std::vector<cdet_vertex> hv;
-    std::vector<int32_t> ho;
-    std::vector<double> hout, hcond;
-    std::vector<size_t> batch, hoff;
-    for (;;) {
-        batch.clear();
-        for (auto& ch : chains) {
+    // Sweeps.  With an asynchronous backend (submit + wait) the chains form
+    // two groups (the halves of the chain list), each with its own batch slot:
+    // while one group's batch is evaluated, the other group's chains take
+    // their heat-bath steps and its next batch is submitted.  Every chain
+    // takes the same steps as in the lockstep form, since a chain's values
+    // do not depend on the batch it is evaluated in.
+    const bool async = be.submit != nullptr && be.wait != nullptr && c.chain_count >= 2;
+    const int ngroups = async ? 2 : 1;
+    const size_t half = static_cast<size_t>((c.chain_count + 1) / 2);
+    struct Group {
+        std::vector<cdet_vertex> hv;
+        std::vector<int32_t> ho;
+        std::vector<double> hout, hcond;
+        std::vector<size_t> batch, hoff;
+        size_t lo = 0, hi = 0;  // chain index range
+        bool in_flight = false;
+    } groups[2];
+    for (int g = 0; g < ngroups; ++g) {
+        groups[g].lo = ngroups == 1 ? 0 : (g == 0 ? 0 : half);
+        groups[g].hi = ngroups == 1 ? chains.size() : (g == 0 ? half : chains.size());
+    }
+    // the group's live chains and their envelopes; false when none is left
+    auto prepare = [&](Group& gr) {
+        gr.batch.clear();
+        for (size_t k = gr.lo; k < gr.hi; ++k) {
+            Chain& ch = chains[k];
             if (!live(ch)) continue;
             if (ch.result.warmup_proposed >= static_cast<uint64_t>(c.lambda_warmup_steps)) enter_production(ch);
-            batch.push_back(ch.index);
+            gr.batch.push_back(ch.index);
         }
-        if (batch.empty()) break;
-        hv.clear();
-        ho.clear();
-        hoff.assign(1, 0);
-        for (size_t b : batch) {
+        if (gr.batch.empty()) return false;
+        gr.hv.clear();
+        gr.ho.clear();
+        gr.hoff.assign(1, 0);
+        for (size_t b : gr.batch) {
             const Chain& ch = chains[b];
-            hv.insert(hv.end(), ch.vertices.begin(), ch.vertices.end());
-            ho.push_back(ch.cfg.envelope_order);
-            hoff.push_back(hoff.back() + (size_t{1} << ch.cfg.envelope_order));
+            gr.hv.insert(gr.hv.end(), ch.vertices.begin(), ch.vertices.end());
+            gr.ho.push_back(ch.cfg.envelope_order);
+            gr.hoff.push_back(gr.hoff.back() + (size_t{1} << ch.cfg.envelope_order));
         }
-        hout.assign(hoff.back(), 0.0);
-        const int rc = be.eval(be.eval_user, hv.data(), ho.data(), static_cast<int64_t>(batch.size()), hout.data());
+        gr.hout.assign(gr.hoff.back(), 0.0);
+        return true;
+    };
+    auto submit = [&](int g) {
+        Group& gr = groups[g];
+        const int rc = be.submit(be.eval_user, g, gr.hv.data(), gr.ho.data(), static_cast<int64_t>(gr.batch.size()));
+        if (rc != 0) throw std::runtime_error("integrand evaluation failed (status " + std::to_string(rc) + ")");
+        gr.in_flight = true;
+    };
+    // values of the group's batch (waiting for it, or evaluating it now), then
+    // the chains' steps
+    auto complete = [&](int g) {
+        Group& gr = groups[g];
+        int rc = 0;
+        if (async) {
+            gr.in_flight = false;
+            rc = be.wait(be.eval_user, g, gr.hout.data());
+        } else {
+            rc = be.eval(be.eval_user, gr.hv.data(), gr.ho.data(), static_cast<int64_t>(gr.batch.size()),
+                         gr.hout.data());
+        }
         if (rc != 0) throw std::runtime_error("integrand evaluation failed (status " + std::to_string(rc) + ")");
         if (gpu_cond) {
-            hcond.assign(hoff.back(), 0.0);
-            const int rcc =
-                be.cond(be.cond_user, hseed, hv.data(), ho.data(), static_cast<int64_t>(batch.size()), hcond.data());
+            gr.hcond.assign(gr.hoff.back(), 0.0);
+            const int rcc = be.cond(be.cond_user, hseed, gr.hv.data(), gr.ho.data(),
+                                    static_cast<int64_t>(gr.batch.size()), gr.hcond.data());
             if (rcc != 0)
                 throw std::runtime_error("conditional generation probabilities failed (status " + std::to_string(rcc) +
                                          ")");
         }
-        pool.run(batch.size(), [&](size_t bi) {
-            Chain& ch = chains[batch[bi]];
+        pool.run(gr.batch.size(), [&](size_t bi) {
+            Chain& ch = chains[gr.batch[bi]];
             try {
-                advance(ch, hout.data() + hoff[bi], gpu_cond ? hcond.data() + hoff[bi] : nullptr);
+                advance(ch, gr.hout.data() + gr.hoff[bi], gpu_cond ? gr.hcond.data() + gr.hoff[bi] : nullptr);
             } catch (const std::exception& e) {
                 ch.error = e.what();
                 ch.stopped = true;
             }
         });
-        for (size_t b : batch)  // the lowest failing chain reports, as a serial loop would
+        for (size_t b : gr.batch)  // the lowest failing chain reports, as a serial loop would
             if (!chains[b].error.empty()) throw std::runtime_error(chains[b].error);
+    };
+    // a failure leaves no batch in flight behind
+    auto drain = [&]() {
+        for (int g = 0; g < ngroups; ++g)
+            if (groups[g].in_flight) {
+                groups[g].in_flight = false;
+                (void)be.wait(be.eval_user, g, groups[g].hout.data());
+            }
+    };
+    try {
+        bool active[2] = {false, false};
+        for (int g = 0; g < ngroups; ++g) {
+            active[g] = prepare(groups[g]);
+            if (active[g] && async) submit(g);
+        }
+        while (active[0] || active[1]) {
+            for (int g = 0; g < ngroups; ++g) {
+                if (!active[g]) continue;
+                complete(g);
+                active[g] = prepare(groups[g]);
+                if (active[g] && async) submit(g);
+            }
+        }
+    } catch (...) {
+        drain();
+        throw;
     }
     for (auto& ch : chains) {
         if (!ch.stopped) {

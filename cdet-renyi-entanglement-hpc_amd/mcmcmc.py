@@ -101,7 +101,14 @@ class Backend(ctypes.Structure):
     """cdet_mcmcmc_backend (include/cdet_mcmcmc.h)."""
 
     _fields_ = [("eval", ctypes.c_void_p), ("eval_user", ctypes.c_void_p),
-                ("cond", ctypes.c_void_p), ("cond_user", ctypes.c_void_p)]
+                ("cond", ctypes.c_void_p), ("cond_user", ctypes.c_void_p),
+                ("submit", ctypes.c_void_p), ("wait", ctypes.c_void_p)]
+
+
+# cdet_batch_submit_fn / cdet_batch_wait_fn, for callbacks written in Python
+SUBMIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.POINTER(ctypes.c_int32), ctypes.c_int64)
+WAIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double))
 
 
 class ChainResult(ctypes.Structure):
@@ -302,14 +309,15 @@ def run_envelopes(config, envelopes, eval_fn, eval_user, seed=None, trace_steps=
 
 
 def run_backend(config, eval_fn, eval_user, cond_fn=None, cond_user=None, seed=None, envelopes=None,
-                trace_steps=0):
+                trace_steps=0, submit_fn=None, wait_fn=None):
     """Chains through a backend (cdet_mcmcmc_run_backend): integrand callback
     and, optionally, the conditional-generation callback of the histogram
-    seed.  envelopes: per-chain envelope orders or None."""
+    seed and the asynchronous submit/wait pair (two groups of chains in
+    flight).  envelopes: per-chain envelope orders or None."""
     n = config.chain_count
     env = None if envelopes is None else np.ascontiguousarray(envelopes, dtype=np.int32)
-    be = Backend(ctypes.cast(eval_fn, ctypes.c_void_p), eval_user,
-                 ctypes.cast(cond_fn, ctypes.c_void_p) if cond_fn is not None else None, cond_user)
+    cast = lambda f: ctypes.cast(f, ctypes.c_void_p) if f is not None else None
+    be = Backend(cast(eval_fn), eval_user, cast(cond_fn), cond_user, cast(submit_fn), cast(wait_fn))
     results = (ChainResult * n)()
     ratios = (ctypes.c_double * (MAXE + 1))()
     trace = np.zeros((n, max(trace_steps, 1)), dtype=np.uint64)

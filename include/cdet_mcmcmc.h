@@ -102,6 +102,19 @@ typedef struct cdet_histogram_seed {
 typedef int (*cdet_batch_eval_fn)(void* user, const cdet_vertex* h_vertices, const int32_t* h_orders,
                                   int64_t count, double* h_out);
 
+/*
+ * The same evaluation in two steps over two slots (0, 1): submit takes a copy
+ * of the batch, starts it and returns; wait blocks until the slot's batch is
+ * done and writes its values (the layout of cdet_batch_eval_fn).  A slot holds
+ * one batch at a time.  With both set, the driver keeps two groups of chains
+ * in flight, so one group's heat-bath steps overlap the other's evaluation.
+ * No counterpart in the reference, whose chains call the evaluator one at a
+ * time from their own threads (include/cdet/mcmcmc_sampler.h:1107).
+ */
+typedef int (*cdet_batch_submit_fn)(void* user, int slot, const cdet_vertex* h_vertices, const int32_t* h_orders,
+                                    int64_t count);
+typedef int (*cdet_batch_wait_fn)(void* user, int slot, double* h_out);
+
 /* Per-chain results (MCMCMCChainResult, include/cdet/mcmcmc_sampler.h:120-134). */
 typedef struct cdet_mcmcmc_chain_result {
     uint64_t seed;
@@ -211,6 +224,8 @@ typedef struct cdet_mcmcmc_backend {
     void* eval_user;
     cdet_batch_cond_fn cond;
     void* cond_user;
+    cdet_batch_submit_fn submit;  /* optional, with wait: two groups of chains in flight (user = eval_user) */
+    cdet_batch_wait_fn wait;
 } cdet_mcmcmc_backend;
 
 /* cdet_mcmcmc_run_envelopes with a backend; envelope_orders may be NULL (every
@@ -228,6 +243,12 @@ int cdet_mcmcmc_run_backend(const cdet_mcmcmc_config* config, const cdet_histogr
  */
 int cdet_eval_batch_host(void* ctx, const cdet_vertex* h_vertices, const int32_t* h_orders, int64_t count,
                          double* h_out);
+/* GPU implementation of cdet_batch_submit_fn / cdet_batch_wait_fn over a
+ * context: each slot has its own plan cache, stream and pinned staging, so the
+ * two slots' batches run concurrently; same values as cdet_eval_batch_host. */
+int cdet_eval_batch_submit(void* ctx, int slot, const cdet_vertex* h_vertices, const int32_t* h_orders,
+                           int64_t count);
+int cdet_eval_batch_wait(void* ctx, int slot, double* h_out);
 int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode);
 
 /* GPU implementation of cdet_batch_cond_fn over a context (user = cdet_ctx*):

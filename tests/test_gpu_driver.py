@@ -171,3 +171,43 @@ def test_driver_histogram_gpu_conditional_backend(gpu_available, tmp_path):
     assert np.array_equal(t_host, t_gpu)
     assert chk(tmp_path / "h") == chk(tmp_path / "g")
     ctx.close()
+
+
+def test_driver_two_groups_in_flight_gpu(gpu_available, tmp_path):
+    # cdet_eval_batch_submit / wait: two groups of chains in flight on the
+    # two slots' streams take exactly the steps of the synchronous host
+    # callback (5 chains: groups of 3 and 2), in both batch modes.
+    ctx = gpu_ctx()
+    L = cdet_amd.lib()
+    hfn = ctypes.cast(L.cdet_eval_batch_host, ctypes.c_void_p)
+    sfn = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p)
+    wfn = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p)
+    env = [7, 5, 6, 3, 7]
+    for mode in ("exact", "mfma64"):
+        assert L.cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[mode]) == 0
+        c = c1_config(tmp_path / mode, chain_count=5, checkpoint_enabled=0, lambda_warmup_steps=300,
+                      production_steps=500)
+        print(f"driver two groups {mode} sync", flush=True)
+        rs, rats, ts = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=800)
+        print(f"driver two groups {mode} async", flush=True)
+        ra, rata, ta = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=800, submit_fn=sfn,
+                                          wait_fn=wfn)
+        assert np.array_equal(ts, ta)
+        assert np.array_equal(rats, rata, equal_nan=True)
+        for a, b in zip(rs, ra):
+            assert list(a.production_chosen_counts) == list(b.production_chosen_counts)
+    # a slot holds one batch: a second submit before its wait is refused
+    v = np.zeros((2, 4))
+    v[:, 0] = [0.1, 0.2]
+    o = np.array([2], dtype=np.int32)
+    vp = v.ctypes.data_as(ctypes.c_void_p)
+    op = o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    assert L.cdet_eval_batch_submit(ctx.handle, 0, vp, op, 1) == 0
+    assert L.cdet_eval_batch_submit(ctx.handle, 0, vp, op, 1) != 0
+    out = np.zeros(4)
+    assert L.cdet_eval_batch_wait(ctx.handle, 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    assert L.cdet_eval_batch_wait(ctx.handle, 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) != 0
+    ref = np.zeros(4)
+    assert L.cdet_eval_batch_host(ctx.handle, vp, op, 1, ref.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    assert np.array_equal(out, ref)
+    ctx.close()

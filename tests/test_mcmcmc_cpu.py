@@ -155,3 +155,50 @@ def test_per_chain_envelopes_resume_and_ratios(tmp_path):
                                                                      ctypes.c_void_p), None)
     assert all(abs(ratios[o] - 1.0) < 0.05 for o in (2, 3, 4)), ratios
     assert res[3].order_signal_count[3] == 0 and res[0].order_signal_count[4] == 200000
+
+
+def test_two_groups_in_flight_take_the_lockstep_steps(tmp_path):
+    # An asynchronous backend (submit + wait) puts the two halves of the
+    # chains in flight in turn; each chain takes exactly its lockstep steps.
+    # The submit/wait pair here wraps the oracle's batch callback and records
+    # the calls, so the test also checks that the slots alternate and every
+    # submitted batch is waited for once.
+    fn, orc = oracle_eval()
+    eval_c = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                              ctypes.c_int64, ctypes.POINTER(ctypes.c_double))(fn.value)
+    pending, calls = {}, []
+
+    def submit(user, slot, verts, orders, count):
+        n = [orders[i] for i in range(count)]
+        nv = sum(n)
+        v = np.ctypeslib.as_array(ctypes.cast(verts, ctypes.POINTER(ctypes.c_double)), shape=(4 * nv,)).copy()
+        if slot in pending:
+            return 7
+        pending[slot] = (v, np.array(n, dtype=np.int32))
+        calls.append(("submit", slot, count))
+        return 0
+
+    def wait(user, slot, out):
+        if slot not in pending:
+            return 8
+        v, n = pending.pop(slot)
+        calls.append(("wait", slot))
+        return eval_c(orc._h, v.ctypes.data_as(ctypes.c_void_p), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                      len(n), out)
+
+    sfn, wfn = mcmcmc.SUBMIT_FN(submit), mcmcmc.WAIT_FN(wait)
+    env = [4, 3, 4, 2, 3]
+    c = base_config(tmp_path / "a", chain_count=5, checkpoint_enabled=0)
+    c.envelope_order = 4
+    c.lambda_warmup_steps, c.production_steps = 60, 90
+    r_sync, _, t_sync = mcmcmc.run_backend(c, fn, orc._h, envelopes=env, trace_steps=150)
+    r_async, _, t_async = mcmcmc.run_backend(c, fn, orc._h, envelopes=env, trace_steps=150, submit_fn=sfn,
+                                             wait_fn=wfn)
+    assert np.array_equal(t_sync, t_async)
+    for a, b in zip(r_sync, r_async):
+        assert list(a.order_signal_mean) == list(b.order_signal_mean)
+        assert list(a.production_chosen_counts) == list(b.production_chosen_counts)
+    assert not pending
+    subs = [x for x in calls if x[0] == "submit"]
+    assert {s[1] for s in subs} == {0, 1} and sum(1 for x in calls if x[0] == "wait") == len(subs)
+    assert {s[2] for s in subs if s[1] == 0} == {3} and {s[2] for s in subs if s[1] == 1} == {2}

@@ -66,8 +66,9 @@ def parse():
                          "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice; "
                          "c5: 1024 configurations at orders 2..14 on the 10x10 lattice")
     ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
-    ap.add_argument("--sweep-groups", type=int, default=2, choices=[1, 2],
-                    help="c4/c5 sweeps: chain groups in flight (2: two GPU streams, submit/wait; 1: one batch)")
+    ap.add_argument("--sweep-groups", type=int, default=1, choices=[1, 2],
+                    help="c4/c5 sweeps: chain groups in flight (1: one batch per sweep; 2: two groups on the two "
+                         "submit/wait slots)")
     ap.add_argument("--c5-sweeps", type=int, default=200, help="c5: lockstep sweeps of the 1024 chains (0: none)")
     return ap.parse_args()
 
@@ -294,7 +295,7 @@ C4 = dict(N1=6, N2=6, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="left_col
           chains=256, envelope=12, base_seed=2026101401)
 
 
-def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=2):
+def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):
     """Lockstep MCMCMC sweeps of len(envelopes) chains through the GPU batch
     callback, uniform seed.  groups=2: the chains advance as two groups whose
     batches alternate on two GPU streams (cdet_eval_batch_submit / wait), so

@@ -23,8 +23,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <condition_variable>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cdet_chunks.h"
@@ -830,6 +833,12 @@ struct Scratch {
     size_t cap = 0;
 };
 
+constexpr int kMinorLanes = 4;  // streams the order buckets of one plan run spread over
+// Lanes of the plans of cdet_eval_batch_submit, whose two slots already run
+// concurrently: two slots of two lanes fill the device's hardware queues (4
+// per process by default).
+constexpr int kSlotLanes = 2;
+
 struct cdet_ctx {
     cdet_params p{};
     int device = 0;
@@ -841,7 +850,7 @@ struct cdet_ctx {
     uint16_t* d_masks = nullptr;  // masks by popcount for every order, (kMaxOrder+1) * 2^kMaxOrder
     int* d_popoff = nullptr;      // (kMaxOrder+1) * (kMaxOrder+2)
     uint64_t* d_fallbacks = nullptr;
-    uint64_t calls = 0;
+    std::atomic<uint64_t> calls{0};  // slot workers count from their own threads
     // plain-path scratch
     double* d_verts = nullptr;
     double* d_gup = nullptr;
@@ -859,8 +868,19 @@ struct cdet_ctx {
     cdet_batch_plan* host_plan = nullptr;
     std::vector<int32_t> host_plan_orders;
     // cdet_eval_batch_submit / wait: two slots, each with its own plan cache,
-    // stream, device buffers and pinned host staging
+    // stream, device buffers, pinned host staging and launch thread (a launch
+    // of a graph whose order buckets fork over several streams holds the
+    // calling thread until the graph is about done, so the slot's copies and
+    // graph launch run on the slot's own thread)
     struct HostSlot {
+        std::thread worker;
+        std::mutex mu;
+        std::condition_variable cv;
+        bool job = false;       // a launch is queued for the worker
+        bool launched = true;   // the worker has finished the queued launch
+        bool quit = false;
+        int launch_rc = 0;
+        std::string launch_err;
         cdet_batch_plan* plan = nullptr;
         std::vector<int32_t> orders;
         hipStream_t stream = nullptr;
@@ -879,7 +899,10 @@ struct cdet_ctx {
         int64_t count = 0;
         size_t out_bytes = 0;
         bool pending = false;
+        hipStream_t lanes[kSlotLanes - 1] = {};  // fork streams of the slot's plans
     } slots[2];
+    // fork streams of the other plans of the context, created on first use
+    hipStream_t lanes[kMinorLanes - 1] = {};
     void* d_hv = nullptr;
     size_t d_hv_cap = 0;
     void* d_hout = nullptr;
@@ -1341,9 +1364,22 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->plain_scratch.ptr);
     if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
     for (auto& sl : ctx->slots) {
+        if (sl.worker.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(sl.mu);
+                sl.quit = true;
+            }
+            sl.cv.notify_all();
+            sl.worker.join();
+        }
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.plan) cdet_batch_plan_destroy(sl.plan);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        for (hipStream_t l : sl.lanes)
+            if (l) {
+                (void)hipStreamSynchronize(l);
+                (void)hipStreamDestroy(l);
+            }
         (void)hipFree(sl.d_v);
         (void)hipFree(sl.d_out);
         (void)hipFree(sl.d_status);
@@ -1351,6 +1387,11 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
         (void)hipHostFree(sl.h_out);
         (void)hipHostFree(sl.h_status);
     }
+    for (hipStream_t l : ctx->lanes)
+        if (l) {
+            (void)hipStreamSynchronize(l);
+            (void)hipStreamDestroy(l);
+        }
     (void)hipFree(ctx->d_hv);
     (void)hipFree(ctx->d_hout);
     (void)hipFree(ctx->d_hstatus);

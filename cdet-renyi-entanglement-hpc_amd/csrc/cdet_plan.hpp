@@ -5,6 +5,8 @@
 // is a fixed launch sequence with no allocation and no host synchronisation;
 // the sequence is captured once into a hipGraph and replayed.
 
+constexpr double kForkMinUs = 40.0;  // bucket work beyond the longest bucket worth a fork
+
 struct cdet_batch_plan {
     cdet_ctx* ctx = nullptr;
     int contract_mode = CDET_MODE_EXACT;
@@ -42,8 +44,21 @@ struct cdet_batch_plan {
     double* d_single = nullptr;
     int32_t* d_fb_list = nullptr;
     int32_t* d_fb_count = nullptr;  // one queue length per order
-    Scratch scratch;                // minors global scratch, sized at creation
+    Scratch scratch;                // minors global scratch of lane 0, sized at creation
     int contract_grid = 0;
+    // Order buckets spread over streams: bucket n runs on lane lane_of[n]
+    // (lane 0 = the run's stream, lane k > 0 = aux[k - 1], forked from and
+    // joined back to the run's stream by events); each lane has its own
+    // minors scratch, since its buckets run one after another.  The aux
+    // streams belong to the context (or to the submit slot the plan serves)
+    // and are shared by its plans.
+    int lanes = 1;
+    int max_lanes = kMinorLanes;
+    std::vector<int> lane_of;
+    Scratch lane_scratch[kMinorLanes];  // [0] unused (lane 0 uses scratch)
+    hipStream_t aux[kMinorLanes - 1] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kMinorLanes - 1] = {};
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -66,6 +81,12 @@ static void plan_free(cdet_batch_plan* p) {
     if (!p) return;
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (int k = 0; k + 1 < kMinorLanes; ++k) {
+        if (p->aux[k]) (void)hipStreamSynchronize(p->aux[k]);
+        if (p->ev_join[k]) (void)hipEventDestroy(p->ev_join[k]);
+        (void)hipFree(p->lane_scratch[k + 1].ptr);
+    }
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     void* bufs[] = {p->d_order, p->d_vbase, p->d_pbase, p->d_obase, p->d_lists, p->d_gup, p->d_gdn, p->d_qkey,
                     p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
                     p->d_ntiles, p->d_su, p->d_sv, p->d_sdest, p->d_out_ord, p->d_single, p->d_fb_list,
@@ -83,16 +104,90 @@ static int dev_alloc(T** p, size_t n) {
 
 static int contract_launch(cdet_batch_plan* p, hipStream_t s);
 
+// Estimated duration (microseconds) of one order bucket's minors launches on
+// an otherwise idle device: a launch floor plus, per round of workgroups, the
+// latency of one connected-value recursion (about 3^n dependent terms per
+// workgroup; fitted to the order-10 and order-12 buckets of the 4x4 and 6x6
+// workloads).
+static double bucket_cost_us(int n, int count, int mode, int ncu) {
+    const double per_wg = n >= 2 ? 3.3e-4 * std::pow(3.0, n) : 0.0;
+    const size_t lds = std::max<size_t>(CdetLds::bytes(n, mode), 1);
+    const double fit = std::max<double>(1.0, std::min<double>(8.0, static_cast<double>(kMaxLds / lds)));
+    const double rounds = std::max(1.0, 2.0 * count / (static_cast<double>(ncu) * fit));
+    return 10.0 + per_wg * rounds;
+}
+
+// Spread the order buckets over up to max_lanes streams, longest first onto
+// the least-loaded lane, when the buckets other than the longest carry enough
+// work to pay for the fork (about 10-15 us per forked stream); size each
+// lane's scratch and create the lanes' streams and events.
+static int plan_lanes(cdet_batch_plan* p, hipStream_t* lane_streams) {
+    cdet_ctx* ctx = p->ctx;
+    p->lane_of.assign(kMaxOrder + 1, 0);
+    std::vector<std::pair<double, int>> order;  // (cost, n)
+    double total = 0.0, longest = 0.0;
+    for (int n = 1; n <= kMaxOrder; ++n) {
+        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
+        if (!cnt) continue;
+        const double c = bucket_cost_us(n, cnt, p->eval_kind, ctx->ncu);
+        order.emplace_back(c, n);
+        total += c;
+        longest = std::max(longest, c);
+    }
+    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
+        return a.first != b.first ? a.first > b.first : a.second > b.second;
+    });
+    int lanes = 1;
+    if (order.size() >= 2 && total - longest >= kForkMinUs)
+        lanes = static_cast<int>(std::min<double>({static_cast<double>(p->max_lanes), static_cast<double>(order.size()),
+                                                  std::ceil(total / std::max(longest, kForkMinUs))}));
+    lanes = std::max(lanes, 1);
+    std::vector<double> load(static_cast<size_t>(lanes), 0.0);
+    for (const auto& b : order) {
+        const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        load[static_cast<size_t>(k)] += b.first;
+        p->lane_of[static_cast<size_t>(b.second)] = k;
+    }
+    p->lanes = lanes;
+    for (int k = 0; k < lanes; ++k) {
+        size_t need = 0;
+        for (int n = 1; n <= kMaxOrder; ++n)
+            if (p->bucket_cnt[static_cast<size_t>(n)] && p->lane_of[static_cast<size_t>(n)] == k)
+                need = std::max(need, minors_scratch_bytes(n, p->bucket_cnt[static_cast<size_t>(n)], p->eval_kind));
+        Scratch& sc = k ? p->lane_scratch[k] : p->scratch;
+        int rc = grow(&sc.ptr, sc.cap, need);
+        if (rc) return rc;
+        if (k) {
+            if (!lane_streams[k - 1]) HIP_TRY(hipStreamCreateWithFlags(&lane_streams[k - 1], hipStreamNonBlocking));
+            p->aux[k - 1] = lane_streams[k - 1];
+            HIP_TRY(hipEventCreateWithFlags(&p->ev_join[k - 1], hipEventDisableTiming));
+        }
+    }
+    if (lanes > 1) HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    return 0;
+}
+
 static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hipStream_t s) {
     cdet_ctx* ctx = p->ctx;
+    if (p->lanes > 1) {
+        HIP_TRY(hipEventRecord(p->ev_fork, s));
+        for (int k = 1; k < p->lanes; ++k) HIP_TRY(hipStreamWaitEvent(p->aux[k - 1], p->ev_fork, 0));
+    }
     for (int n = kMaxOrder; n >= 1; --n) {
         const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
         if (!cnt) continue;
         const int off = p->bucket_off[static_cast<size_t>(n)];
+        const int lane = p->lane_of[static_cast<size_t>(n)];
         MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n};
-        int rc = launch_minors(ctx, p->scratch, n, p->eval_kind, p->d_lists + off, cnt, p->d_pbase, p->d_obase,
-                               p->d_gup, p->d_gdn, o, s);
+        int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off, cnt,
+                               p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s);
         if (rc) return rc;
+    }
+    if (p->lanes > 1) {
+        for (int k = 1; k < p->lanes; ++k) {
+            HIP_TRY(hipEventRecord(p->ev_join[k - 1], p->aux[k - 1]));
+            HIP_TRY(hipStreamWaitEvent(s, p->ev_join[k - 1], 0));
+        }
     }
     MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, nullptr, nullptr};
     return launch_combine(p->eval_kind, p->count, p->out_total, p->d_obase, o, s);
@@ -126,6 +221,36 @@ static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertice
     return 0;
 }
 
+// Capture the plan's launch sequence for these buffers and this stream into
+// its graph, unless the graph captured last matches them and the tables.
+static int plan_capture(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
+                        hipStream_t s) {
+    if (p->graph_exec && p->g_verts == d_vertices && p->g_out == d_out && p->g_status == d_status && p->g_stream == s &&
+        p->g_tables == p->ctx->tables_gen)
+        return 0;
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(CDET_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+    p->graph = g;
+    HIP_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+    p->g_verts = d_vertices;
+    p->g_out = d_out;
+    p->g_status = d_status;
+    p->g_stream = s;
+    p->g_tables = p->ctx->tables_gen;
+    return 0;
+}
+
 static int contract_launch(cdet_batch_plan* p, hipStream_t s) {
     cdet_ctx* ctx = p->ctx;
     switch (p->contract_mode) {
@@ -146,13 +271,16 @@ static int contract_launch(cdet_batch_plan* p, hipStream_t s) {
 
 extern "C" {
 
-int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int mode, int eval_kind,
-                           cdet_batch_plan** out) {
+// Plan whose order buckets spread over the run's stream and at most
+// max_lanes - 1 of lane_streams (created there on first use).
+static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int mode, int eval_kind, int max_lanes,
+                       hipStream_t* lane_streams, cdet_batch_plan** out) {
     if (!ctx || !orders || !out || count <= 0) return fail(CDET_ERR_ARGUMENT, "cdet_batch_plan_create: bad argument");
     if (mode < 0 || mode > 2) return fail(CDET_ERR_ARGUMENT, "unknown evaluation mode");
     if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
     auto* p = new cdet_batch_plan();
     p->ctx = ctx;
+    p->max_lanes = std::max(1, std::min(max_lanes, kMinorLanes));
     p->contract_mode = mode;
     if (eval_kind < 0 || eval_kind > 2) {
         delete p;
@@ -244,12 +372,7 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     p->contract_grid = contract_grid(ctx, p->max_tiles);
     // Global scratch (minors of every spin matrix, workspaces of large
     // orders), sized now so that runs never allocate and can be graph-captured.
-    size_t scratch_need = 0;
-    for (int n = 1; n <= kMaxOrder; ++n)
-        if (p->bucket_cnt[static_cast<size_t>(n)])
-            scratch_need = std::max(scratch_need,
-                                    minors_scratch_bytes(n, p->bucket_cnt[static_cast<size_t>(n)], p->eval_kind));
-    rc = grow(&p->scratch.ptr, p->scratch.cap, scratch_need);
+    rc = plan_lanes(p, lane_streams);
     if (rc) {
         plan_free(p);
         return rc;
@@ -261,6 +384,12 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
     }
     *out = p;
     return 0;
+}
+
+int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int mode, int eval_kind,
+                           cdet_batch_plan** out) {
+    if (!ctx) return fail(CDET_ERR_ARGUMENT, "cdet_batch_plan_create: bad argument");
+    return plan_create(ctx, orders, count, mode, eval_kind, kMinorLanes, ctx->lanes, out);
 }
 
 int cdet_batch_plan_destroy(cdet_batch_plan* plan) {
@@ -284,29 +413,8 @@ int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_
         if (!rc) p->ctx->calls += static_cast<uint64_t>(p->count);
         return rc;
     }
-    if (!p->graph_exec || p->g_verts != d_vertices || p->g_out != d_out || p->g_status != d_status || p->g_stream != s ||
-        p->g_tables != p->ctx->tables_gen) {
-        if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
-        if (p->graph) (void)hipGraphDestroy(p->graph);
-        p->graph_exec = nullptr;
-        p->graph = nullptr;
-        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return fail(CDET_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
-        p->graph = g;
-        HIP_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
-        p->g_verts = d_vertices;
-        p->g_out = d_out;
-        p->g_status = d_status;
-        p->g_stream = s;
-        p->g_tables = p->ctx->tables_gen;
-    }
+    int rc = plan_capture(p, d_vertices, d_out, d_status, s);
+    if (rc) return rc;
     HIP_TRY(hipGraphLaunch(p->graph_exec, s));
     p->ctx->calls += static_cast<uint64_t>(p->count);
     return 0;
@@ -512,10 +620,43 @@ static int grow_pinned(void** p, size_t& cap, size_t need) {
     return 0;
 }
 
+// The slot's launch: vertices to the device, the plan's graph (captured by
+// the submitting thread) on the slot's stream, values and status words back
+// to pinned memory.
+static int slot_launch(cdet_ctx::HostSlot& sl) {
+    cdet_batch_plan* p = sl.plan;
+    HIP_TRY(hipMemcpyAsync(sl.d_v, sl.h_v, sizeof(cdet_vertex) * static_cast<size_t>(p->vert_total),
+                           hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(hipGraphLaunch(p->graph_exec, sl.stream));
+    p->ctx->calls += static_cast<uint64_t>(p->count);
+    HIP_TRY(hipMemcpyAsync(sl.h_out, sl.d_out, sl.out_bytes, hipMemcpyDeviceToHost, sl.stream));
+    HIP_TRY(hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int32_t) * static_cast<size_t>(sl.count),
+                           hipMemcpyDeviceToHost, sl.stream));
+    return 0;
+}
+
+static void slot_worker(int device, cdet_ctx::HostSlot* sl) {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(sl->mu);
+    for (;;) {
+        sl->cv.wait(lk, [sl] { return sl->quit || sl->job; });
+        if (sl->quit) return;
+        sl->job = false;
+        lk.unlock();
+        const int rc = slot_launch(*sl);
+        std::string err = rc ? g_last_error : std::string();
+        lk.lock();
+        sl->launch_rc = rc;
+        sl->launch_err = std::move(err);
+        sl->launched = true;
+        sl->cv.notify_all();
+    }
+}
+
 // Asynchronous form of cdet_eval_batch_host over two slots: the vertices are
-// copied into the slot's pinned staging, the plan runs (graph) on the slot's
-// stream, and the values come back to pinned memory; cdet_eval_batch_wait
-// synchronizes the slot and copies them out.
+// copied into the slot's pinned staging and the slot's thread launches the
+// copies and the plan on the slot's stream; cdet_eval_batch_wait waits for
+// the launch and the stream and copies the values out.
 extern "C" int cdet_eval_batch_submit(void* user, int slot, const cdet_vertex* h_vertices, const int32_t* h_orders,
                                       int64_t count) {
     cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
@@ -527,32 +668,45 @@ extern "C" int cdet_eval_batch_submit(void* user, int slot, const cdet_vertex* h
     const bool same = sl.plan && static_cast<int64_t>(sl.orders.size()) == count &&
                       std::equal(sl.orders.begin(), sl.orders.end(), h_orders);
     if (!same) {
-        if (sl.plan) cdet_batch_plan_destroy(sl.plan);
+        if (sl.plan) {
+            (void)hipStreamSynchronize(sl.stream);
+            cdet_batch_plan_destroy(sl.plan);
+        }
         sl.plan = nullptr;
-        int rc = cdet_batch_plan_create(ctx, h_orders, count, ctx->batch_mode, 0, &sl.plan);
+        int rc = plan_create(ctx, h_orders, count, ctx->batch_mode, 0, kSlotLanes, sl.lanes, &sl.plan);
         if (rc) return rc;
         sl.orders.assign(h_orders, h_orders + count);
     }
     cdet_batch_plan* p = sl.plan;
+    // derived table copies are made here, so that the slot threads only read
+    // the context's tables
+    int rc = plan_tables(p);
+    if (rc) return rc;
     const size_t vbytes = sizeof(cdet_vertex) * static_cast<size_t>(p->vert_total);
     const size_t obytes = sizeof(double) * static_cast<size_t>(p->out_total);
     const size_t sbytes = sizeof(int32_t) * static_cast<size_t>(count);
-    int rc = grow(&sl.d_v, sl.d_v_cap, vbytes);
+    rc = grow(&sl.d_v, sl.d_v_cap, vbytes);
     if (!rc) rc = grow(&sl.d_out, sl.d_out_cap, obytes);
     if (!rc) rc = grow(&sl.d_status, sl.d_status_cap, sbytes);
     if (!rc) rc = grow_pinned(&sl.h_v, sl.h_v_cap, vbytes);
     if (!rc) rc = grow_pinned(&sl.h_out, sl.h_out_cap, obytes);
     if (!rc) rc = grow_pinned(&sl.h_status, sl.h_status_cap, sbytes);
     if (rc) return rc;
-    std::memcpy(sl.h_v, h_vertices, vbytes);
-    HIP_TRY(hipMemcpyAsync(sl.d_v, sl.h_v, vbytes, hipMemcpyHostToDevice, sl.stream));
-    rc = cdet_batch_run(p, static_cast<const cdet_vertex*>(sl.d_v), static_cast<double*>(sl.d_out),
-                        static_cast<int32_t*>(sl.d_status), 1, sl.stream);
+    // the graph is captured here, on the submitting thread, so that the slot
+    // threads make only stream-ordered calls (copies, graph launches)
+    rc = plan_capture(p, static_cast<const cdet_vertex*>(sl.d_v), static_cast<double*>(sl.d_out),
+                      static_cast<int32_t*>(sl.d_status), sl.stream);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(sl.h_out, sl.d_out, obytes, hipMemcpyDeviceToHost, sl.stream));
-    HIP_TRY(hipMemcpyAsync(sl.h_status, sl.d_status, sbytes, hipMemcpyDeviceToHost, sl.stream));
+    std::memcpy(sl.h_v, h_vertices, vbytes);
     sl.count = count;
     sl.out_bytes = obytes;
+    if (!sl.worker.joinable()) sl.worker = std::thread(slot_worker, ctx->device, &sl);
+    {
+        std::lock_guard<std::mutex> lk(sl.mu);
+        sl.job = true;
+        sl.launched = false;
+    }
+    sl.cv.notify_all();
     sl.pending = true;
     return 0;
 }
@@ -562,7 +716,12 @@ extern "C" int cdet_eval_batch_wait(void* user, int slot, double* h_out) {
     if (!ctx || slot < 0 || slot > 1 || !h_out) return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_wait: bad argument");
     auto& sl = ctx->slots[slot];
     if (!sl.pending) return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_wait: no batch was submitted to the slot");
+    {
+        std::unique_lock<std::mutex> lk(sl.mu);
+        sl.cv.wait(lk, [&sl] { return sl.launched; });
+    }
     sl.pending = false;
+    if (sl.launch_rc) return fail(sl.launch_rc, sl.launch_err);
     HIP_TRY(hipStreamSynchronize(sl.stream));
     const int32_t* st = static_cast<const int32_t*>(sl.h_status);
     for (int64_t i = 0; i < sl.count; ++i)

@@ -190,8 +190,11 @@ def test_driver_two_groups_in_flight_gpu(gpu_available, tmp_path):
         print(f"driver two groups {mode} sync", flush=True)
         rs, rats, ts = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=800)
         print(f"driver two groups {mode} async", flush=True)
-        ra, rata, ta = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=800, submit_fn=sfn,
-                                          wait_fn=wfn)
+        try:
+            ra, rata, ta = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=800, submit_fn=sfn,
+                                              wait_fn=wfn)
+        except RuntimeError as e:
+            raise RuntimeError(f"{e}; library: {L.cdet_last_error().decode()}") from e
         assert np.array_equal(ts, ta)
         assert np.array_equal(rats, rata, equal_nan=True)
         for a, b in zip(rs, ra):

@@ -13,30 +13,36 @@ namespace cdet {
 // (include/cdet/integrand_replica_randomly_sampled.h:326-343).  Operands are
 // read eight terms at a time, one chunk ahead of the dependent subtractions,
 // so that LDS latency overlaps the serial chain; the arithmetic order is
-// unchanged.
+// unchanged.  Since s is a subset of rest and the anchor bit is outside it,
+// dp[anchor | s] = dp[anchor + s] and a[mask ^ (anchor | s)] = a[rest - s]:
+// both operands are one add from a per-mask base, with s kept as a byte
+// offset (8 s) and stepped as (8 s - 8) & 8 rest.
 __device__ __forceinline__ double connected_chain(const double* a, const double* dp, uint32_t mask, uint32_t anchor,
                                          uint32_t rest, int terms, double thr, bool filter, double& rhs_out) {
     double acc = a[mask];
     double rhs = fabs(acc);
-    uint32_t s = (rest - 1u) & rest;
+    const char* dpb = reinterpret_cast<const char*>(dp + anchor);
+    const char* ab = reinterpret_cast<const char*>(a + rest);
+    const uint32_t rest8 = rest << 3;
+    uint32_t s8 = ((rest - 1u) & rest) << 3;
+    auto dp_at = [dpb](uint32_t o) { return *reinterpret_cast<const double*>(dpb + o); };
+    auto a_at = [ab](uint32_t o) { return *reinterpret_cast<const double*>(ab - o); };
     int done = 0;
     if (terms >= 8) {
         double dv[8], av[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const uint32_t sub = anchor | s;
-            dv[u] = dp[sub];
-            av[u] = a[mask ^ sub];
-            s = (s - 1u) & rest;
+            dv[u] = dp_at(s8);
+            av[u] = a_at(s8);
+            s8 = (s8 - 8u) & rest8;
         }
         for (; done + 16 <= terms; done += 8) {
             double dn[8], an[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const uint32_t sub = anchor | s;
-                dn[u] = dp[sub];
-                an[u] = a[mask ^ sub];
-                s = (s - 1u) & rest;
+                dn[u] = dp_at(s8);
+                an[u] = a_at(s8);
+                s8 = (s8 - 8u) & rest8;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -61,10 +67,9 @@ __device__ __forceinline__ double connected_chain(const double* a, const double*
 #pragma unroll
         for (int u = 0; u < 7; ++u) {
             if (u < tail) {
-                const uint32_t sub = anchor | s;
-                dv[u] = dp[sub];
-                av[u] = a[mask ^ sub];
-                s = (s - 1u) & rest;
+                dv[u] = dp_at(s8);
+                av[u] = a_at(s8);
+                s8 = (s8 - 8u) & rest8;
             }
         }
 #pragma unroll

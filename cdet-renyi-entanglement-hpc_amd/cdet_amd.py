@@ -122,6 +122,7 @@ def _signatures():
         "cdet_batch_out_size": (i64, [vp]),
         "cdet_batch_vertex_count": (i64, [vp]),
         "cdet_batch_query_count": (i64, [vp]),
+        "cdet_batch_plan_streams": (i, [vp]),
         "cdet_batch_run": (i, [vp, vp, vp, vp, i, vp]),
         "cdet_batch_profile": (i, [vp, vp, vp, vp, vp, P(ctypes.c_float)]),
         "cdet_eval_batch": (i, [vp, vp, P(ctypes.c_int32), i64, vp, vp, i, vp]),
@@ -312,6 +313,7 @@ class BatchPlan:
         self.out_size = lib().cdet_batch_out_size(h)
         self.vertex_count = lib().cdet_batch_vertex_count(h)
         self.query_count = lib().cdet_batch_query_count(h)
+        self.streams = lib().cdet_batch_plan_streams(h)
 
     def close(self):
         if getattr(self, "_h", None):

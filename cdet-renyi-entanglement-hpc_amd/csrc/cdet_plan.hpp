@@ -401,6 +401,7 @@ int cdet_batch_plan_destroy(cdet_batch_plan* plan) {
 int64_t cdet_batch_out_size(const cdet_batch_plan* plan) { return plan ? plan->out_total : -1; }
 int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan) { return plan ? plan->vert_total : -1; }
 int64_t cdet_batch_query_count(const cdet_batch_plan* plan) { return plan ? plan->nq : -1; }
+int cdet_batch_plan_streams(const cdet_batch_plan* plan) { return plan ? plan->lanes : -1; }
 
 int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
                    int use_graph, void* stream) {

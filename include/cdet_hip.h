@@ -208,6 +208,10 @@ int cdet_batch_plan_destroy(cdet_batch_plan* plan);
 int64_t cdet_batch_out_size(const cdet_batch_plan* plan);     /* doubles written by one run */
 int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan); /* vertices read by one run */
 int64_t cdet_batch_query_count(const cdet_batch_plan* plan);  /* Green-pair lookups per run */
+/* Streams a run spreads the plan's order buckets over (1 = the run's stream
+ * only; up to 4 for batches of several orders, forked from and joined back to
+ * the run's stream).  No counterpart in the reference. */
+int cdet_batch_plan_streams(const cdet_batch_plan* plan);
 
 /*
  * Run a plan on DEVICE buffers: d_vertices (cdet_vertex[vertex_count]),

@@ -140,6 +140,51 @@ def test_batched_graph_replays_match_eager(gpu_available):
         assert bits_equal(r, eager[0])
 
 
+def test_forked_order_buckets_bit_identical(gpu_available):
+    # A batch of many orders spreads its order buckets over several streams
+    # inside the captured graph; eager runs and graph replays give the
+    # oracle's values bit for bit, and so does the same batch split into
+    # single-order plans (one stream each).
+    import torch
+    ctx = gpu_ctx("c1")
+    orc = helpers.oracle_of("c1")
+    orders = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12] * 3 + [2, 3, 4] * 20
+    configs = helpers.seeded_configs(9100, orders)
+    verts = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
+    plan = ctx.plan([len(v) for v in configs], "exact")
+    assert plan.streams > 1
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    sptr = torch.cuda.current_stream().cuda_stream
+    say(f"forked buckets: {plan.streams} streams, eager")
+    plan.run(verts, out, None, use_graph=False, stream=sptr)
+    torch.cuda.synchronize()
+    eager = out.cpu().numpy().copy()
+    _, ob = cdet_amd.packed_offsets([len(v) for v in configs], False)
+    for k, v in enumerate(configs):
+        if k % 3 == 0 or len(v) >= 10:
+            assert bits_equal(eager[ob[k]:ob[k + 1]], orc.all_subsets(v)), k
+    for r in range(3):
+        say(f"forked buckets: graph replay {r}")
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run(verts, out, None, use_graph=True, stream=sptr)
+        torch.cuda.synchronize()
+        assert bits_equal(out.cpu().numpy(), eager)
+    plan.close()
+    for n in (12, 3):
+        sel = [k for k, v in enumerate(configs) if len(v) == n]
+        single = ctx.plan([n] * len(sel), "exact")
+        assert single.streams == 1
+        o1 = torch.empty(single.out_size, dtype=torch.float64, device="cuda")
+        v1 = torch.tensor(np.concatenate([configs[k] for k in sel]), dtype=torch.float64, device="cuda")
+        single.run(v1, o1, None, use_graph=False, stream=sptr)
+        torch.cuda.synchronize()
+        got = o1.cpu().numpy().reshape(len(sel), -1)
+        for j, k in enumerate(sel):
+            assert bits_equal(got[j], eager[ob[k]:ob[k + 1]]), (n, k)
+        single.close()
+
+
 def rel_err(got, want):
     scale = np.maximum(np.abs(want), 1e-300)
     return np.abs(got - want) / scale

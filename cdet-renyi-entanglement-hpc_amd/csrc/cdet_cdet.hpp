@@ -87,11 +87,15 @@ __device__ __forceinline__ double connected_chain(const double* a, const double*
 }
 
 constexpr int kFoldSteps = 2;  // steps per exchange of the wave-wide chains
+constexpr int kFoldWaves = 4;  // waves that can run wave-wide chains (fold buffer: kFoldWaves x kFoldSteps x 64)
 
 // Whether a popcount layer of M masks with chains of R - 1 terms runs as
-// wave-wide chains: only when every mask can have a wave of its own and the
-// chains are long enough that one lane per mask would leave the wave idle.
-__device__ inline bool layer_wave_chains(int M, int R, int waves) { return M <= waves && R >= 128; }
+// wave-wide chains: only when every mask can have a wave of its own (among
+// the first kFoldWaves) and the chains are long enough that one lane per mask
+// would leave the wave idle.
+__device__ inline bool layer_wave_chains(int M, int R, int waves) {
+    return M <= (waves < kFoldWaves ? waves : kFoldWaves) && R >= 128;
+}
 
 // Subset recursion over all popcount layers, workgroup-cooperative.
 //   all-subset (single = false; include/cdet/integrand_replica_randomly_sampled.h:292-347):
@@ -113,7 +117,8 @@ __device__ inline bool layer_wave_chains(int M, int R, int waves) { return M <= 
 // go to tb[] and lane 0 folds them in descending order, so the arithmetic
 // sequence is the reference's.  The one excluded subset (s = rest, the mask
 // itself) is the first term of the first step and enters as 0, which leaves
-// acc and rhs unchanged.  tb holds kFoldSteps doubles per thread.
+// acc and rhs unchanged.  tb holds kFoldWaves x kFoldSteps x 64 doubles, one
+// kFoldSteps x 64 block per wave; waves without a mask skip the layer.
 __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, int n, double thr, bool filter,
                                          const uint16_t* masks_by_pop, const int* pop_off, double* rhs, bool single,
                                          double* tb) {
@@ -143,7 +148,7 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
             // one mask per wave; lane l holds s_lo = the l-th subset of the
             // lowest six bits of the rest, steps walk the high bits downwards
             const int it = beg + (tid >> 6);
-            const bool have = it < end;
+            const bool have = it < end;  // uniform per wave
             const uint32_t mask = have ? masks_by_pop[it] : 1u;
             const bool rec = have && (!single || (mask & 1u));
             const uint32_t anchor = mask & (~mask + 1u);
@@ -172,8 +177,8 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
             uint32_t shi = rh;
             double acc = rec ? a[mask] : 0.0;
             double rs = fabs(acc);
-            const int steps = R >> 6;
-            double* wtb = tb + (tid & ~63);
+            const int steps = have ? (R >> 6) : 0;  // waves without a mask write no fold buffer
+            double* wtb = tb + (tid >> 6) * (kFoldSteps * 64);
             for (int st0 = 0; st0 < steps; st0 += kFoldSteps) {
                 // every index below is a subset of the mask, so the loads are
                 // in bounds for all lanes; invalid terms are replaced by 0
@@ -183,7 +188,7 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
                     const uint32_t sub = base_lo | shi;
                     const double prod = dp[sub] * a[mask ^ sub];
                     const bool valid = rec && st < steps && !(st == 0 && lane == 63);
-                    wtb[u * nt + lane] = valid ? prod : 0.0;
+                    wtb[u * 64 + lane] = valid ? prod : 0.0;
                     shi = (shi - 1u) & rh;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -195,7 +200,7 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
                     // time, the next eight terms in flight while eight fold
 #pragma unroll
                     for (int u = 0; u < kFoldSteps; ++u) {
-                        const double2* src = reinterpret_cast<const double2*>(wtb + u * nt);
+                        const double2* src = reinterpret_cast<const double2*>(wtb + u * 64);
                         double2 cur[4], nxt[4];
 #pragma unroll
                         for (int i = 0; i < 4; ++i) cur[i] = src[31 - i];

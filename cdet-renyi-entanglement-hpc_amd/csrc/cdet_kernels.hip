@@ -557,8 +557,8 @@ struct CdetLds {
     static __host__ __device__ inline size_t bytes(int n, int mode) { return sizeof(double) * doubles(n, mode); }
 };
 
-template <bool GSCR>
-__global__ __launch_bounds__(256) void k_fpm(KScalars ks, int n, int items, const int32_t* __restrict__ cfg_list,
+template <bool GSCR, int NT>
+__global__ __launch_bounds__(NT) void k_fpm(KScalars ks, int n, int items, const int32_t* __restrict__ cfg_list,
                                              const int64_t* __restrict__ cfg_pbase, const double* __restrict__ g_up,
                                              const double* __restrict__ g_dn, double* __restrict__ pm,
                                              int32_t* __restrict__ fb_list, int32_t* __restrict__ fb_count,
@@ -670,8 +670,8 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
     }
 }
 
-template <bool GSCR>
-__global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int items,
+template <bool GSCR, int NT>
+__global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int items,
                                               const int32_t* __restrict__ cfg_list,
                                               const int64_t* __restrict__ cfg_pbase,
                                               const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up,
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(256) void k_cdet(KScalars ks, int n, int mode, int 
                                               const int* __restrict__ pop_off, double* __restrict__ gscratch,
                                               size_t gscratch_stride, int32_t* __restrict__ fb_count) {
     extern __shared__ double lds_dyn[];
-    __shared__ __attribute__((aligned(16))) double tb[kFoldSteps * 256];
+    __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
     __shared__ int s_popoff[kMaxOrder + 2];  // popcount-layer offsets, read every layer
     // The bucket's fallback queue has been consumed (k_fallback_dets ran
     // before); clear its length for the next run.
@@ -1061,7 +1061,12 @@ static constexpr int kDetGrid = 512;
 static constexpr int kGscrGrid = 1024;  // workgroups of the global-scratch forms
 static bool fpm_gscr(int n) { return FpmLds::bytes(n) > kMaxLds; }
 // k_cdet's static LDS (wave-wide chain terms) comes out of its dynamic budget.
-static constexpr size_t kCdetMaxDyn = kMaxLds - sizeof(double) * kFoldSteps * 256;
+static constexpr size_t kCdetMaxDyn = kMaxLds - sizeof(double) * kFoldWaves * kFoldSteps * 64;
+// Orders from which k_fpm and k_cdet run 1024-thread workgroups: their LDS
+// workspaces leave room for one workgroup per CU (or they work in global
+// scratch), and 256 threads would leave each SIMD a single wave.  At order 12
+// (two workgroups per CU) the wider form measured slower.
+static constexpr int kWideOrder = 13;
 static bool cdet_gscr(int n, int mode) { return CdetLds::bytes(n, mode) > kCdetMaxDyn; }
 
 // Outputs and queues of one minors launch.
@@ -1103,12 +1108,20 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         const int items = (ctx->ks.same_spin[0] && ctx->ks.same_spin[1] ? 2 : 4) * count;
         if (fpm_gscr(n)) {
             const int grid = std::min(items, kGscrGrid);
-            hipLaunchKernelGGL(k_fpm<true>, dim3(grid), dim3(256), 0, stream, ctx->ks, n, items, cfg_list, pbase, gup,
-                               gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
+            if (n >= kWideOrder)
+                hipLaunchKernelGGL((k_fpm<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, items, cfg_list,
+                                   pbase, gup, gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
+            else
+                hipLaunchKernelGGL((k_fpm<true, 256>), dim3(grid), dim3(256), 0, stream, ctx->ks, n, items, cfg_list,
+                                   pbase, gup, gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
             next += static_cast<size_t>(grid) * FpmLds::doubles(n);
         } else {
-            hipLaunchKernelGGL(k_fpm<false>, dim3(items), dim3(256), FpmLds::bytes(n), stream, ctx->ks, n, items,
-                               cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
+            if (n >= kWideOrder)
+                hipLaunchKernelGGL((k_fpm<false, 1024>), dim3(items), dim3(1024), FpmLds::bytes(n), stream, ctx->ks, n,
+                                   items, cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
+            else
+                hipLaunchKernelGGL((k_fpm<false, 256>), dim3(items), dim3(256), FpmLds::bytes(n), stream, ctx->ks, n,
+                                   items, cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
         }
         if (n <= kDetRegMax) {
             const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
@@ -1123,13 +1136,23 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     const int items = 2 * count;
     if (cdet_gscr(n, mode)) {
         const int grid = std::min(items, kGscrGrid);
-        hipLaunchKernelGGL(k_cdet<true>, dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items, cfg_list, pbase,
-                           obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks, popoff, next,
-                           CdetLds::doubles(n, mode), o.fb_count);
+        if (n >= kWideOrder)
+            hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, mode, items,
+                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
+                               popoff, next, CdetLds::doubles(n, mode), o.fb_count);
+        else
+            hipLaunchKernelGGL((k_cdet<true, 256>), dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items,
+                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
+                               popoff, next, CdetLds::doubles(n, mode), o.fb_count);
     } else {
-        hipLaunchKernelGGL(k_cdet<false>, dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks, n, mode,
-                           items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
-                           masks, popoff, nullptr, 0, o.fb_count);
+        if (n >= kWideOrder)
+            hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ctx->ks,
+                               n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
+                               o.status, masks, popoff, nullptr, 0, o.fb_count);
+        else
+            hipLaunchKernelGGL((k_cdet<false, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks,
+                               n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
+                               o.status, masks, popoff, nullptr, 0, o.fb_count);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1325,9 +1348,13 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
     HIP_TRY(hipMemset(ctx->d_one, 0, sizeof(int32_t)));
     HIP_TRY(hipMemset(ctx->d_zero64, 0, sizeof(int64_t)));
     // Allow the order-bucketed kernel the full LDS of a CU.
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false, 256>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false, 1024>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false, 256>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false, 1024>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));

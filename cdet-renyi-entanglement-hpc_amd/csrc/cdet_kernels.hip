@@ -903,6 +903,12 @@ struct cdet_ctx {
     } slots[2];
     // fork streams of the other plans of the context, created on first use
     hipStream_t lanes[kMinorLanes - 1] = {};
+    void* h_hv = nullptr;  // pinned staging of cdet_eval_batch_host
+    size_t h_hv_cap = 0;
+    void* h_hout = nullptr;
+    size_t h_hout_cap = 0;
+    void* h_hstatus = nullptr;
+    size_t h_hstatus_cap = 0;
     void* d_hv = nullptr;
     size_t d_hv_cap = 0;
     void* d_hout = nullptr;
@@ -1419,6 +1425,9 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
             (void)hipStreamSynchronize(l);
             (void)hipStreamDestroy(l);
         }
+    (void)hipHostFree(ctx->h_hv);
+    (void)hipHostFree(ctx->h_hout);
+    (void)hipHostFree(ctx->h_hstatus);
     (void)hipFree(ctx->d_hv);
     (void)hipFree(ctx->d_hout);
     (void)hipFree(ctx->d_hstatus);

@@ -574,6 +574,17 @@ extern "C" int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode) {
     return 0;
 }
 
+// Pinned host staging that only grows.
+static int grow_pinned(void** p, size_t& cap, size_t need) {
+    if (need <= cap) return 0;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    cap = 0;
+    HIP_TRY(hipHostMalloc(p, need, hipHostMallocDefault));
+    cap = need;
+    return 0;
+}
+
 extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, const int32_t* h_orders, int64_t count,
                                     double* h_out) {
     cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
@@ -595,29 +606,25 @@ extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, c
     int rc = grow(&ctx->d_hv, ctx->d_hv_cap, vbytes);
     if (!rc) rc = grow(&ctx->d_hout, ctx->d_hout_cap, obytes);
     if (!rc) rc = grow(&ctx->d_hstatus, ctx->d_hstatus_cap, sbytes);
+    // pinned staging: the copies run as DMA transfers instead of going
+    // through the runtime's bounce buffer for pageable memory
+    if (!rc) rc = grow_pinned(&ctx->h_hv, ctx->h_hv_cap, vbytes);
+    if (!rc) rc = grow_pinned(&ctx->h_hout, ctx->h_hout_cap, obytes);
+    if (!rc) rc = grow_pinned(&ctx->h_hstatus, ctx->h_hstatus_cap, sbytes);
     if (rc) return rc;
     hipStream_t s = ctx->stream;
-    HIP_TRY(hipMemcpyAsync(ctx->d_hv, h_vertices, vbytes, hipMemcpyHostToDevice, s));
+    std::memcpy(ctx->h_hv, h_vertices, vbytes);
+    HIP_TRY(hipMemcpyAsync(ctx->d_hv, ctx->h_hv, vbytes, hipMemcpyHostToDevice, s));
     rc = cdet_batch_run(p, static_cast<const cdet_vertex*>(ctx->d_hv), static_cast<double*>(ctx->d_hout),
                         static_cast<int32_t*>(ctx->d_hstatus), 1, s);
     if (rc) return rc;
-    std::vector<int32_t> st(static_cast<size_t>(count));
-    HIP_TRY(hipMemcpyAsync(h_out, ctx->d_hout, obytes, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(st.data(), ctx->d_hstatus, sbytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ctx->h_hout, ctx->d_hout, obytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ctx->h_hstatus, ctx->d_hstatus, sbytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    for (int32_t x : st)
-        if (x & 1) return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
-    return 0;
-}
-
-// Pinned host staging that only grows.
-static int grow_pinned(void** p, size_t& cap, size_t need) {
-    if (need <= cap) return 0;
-    if (*p) (void)hipHostFree(*p);
-    *p = nullptr;
-    cap = 0;
-    HIP_TRY(hipHostMalloc(p, need, hipHostMallocDefault));
-    cap = need;
+    const int32_t* st = static_cast<const int32_t*>(ctx->h_hstatus);
+    for (int64_t i = 0; i < count; ++i)
+        if (st[i] & 1) return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
+    std::memcpy(h_out, ctx->h_hout, obytes);
     return 0;
 }
 

@@ -438,39 +438,75 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
         for (uint32_t t = tid; t < top; t += nt) out[top + t] += oldl[t];
     }
     __syncthreads();
-    // Prefix products (:217-223): final[m] = r[m] * final[m - topbit(m)],
-    // formed per mask from the lowest bit upward into bufA.
-    for (uint32_t m = tid; m < sc; m += nt) {
-        if (m == 0) {
-            bufA[0] = 1.0;
-            continue;
+    // Prefix products (:217-223): final[m] = r[m] * final[m - topbit(m)], the
+    // running product over the bits of m from the lowest upward (final of a
+    // one-bit mask = r[m]).  Thread t < T = min(2^n, nt) forms final[t] as
+    // that running product, then the masks h * T + t in increasing h, each one
+    // product with final[(h - topbit(h)) * T + t], which the same thread wrote
+    // before (nt is a power of two).
+    {
+        const uint32_t T = sc < static_cast<uint32_t>(nt) ? sc : static_cast<uint32_t>(nt);
+        if (static_cast<uint32_t>(tid) < T) {
+            const uint32_t t = static_cast<uint32_t>(tid);
+            double val = 1.0;
+            if (t != 0) {
+                uint32_t pre = t & (~t + 1u);
+                uint32_t rest = t ^ pre;
+                val = out[pre];
+                while (rest) {
+                    const uint32_t b = rest & (~rest + 1u);
+                    rest ^= b;
+                    pre |= b;
+                    val = out[pre] * val;
+                }
+            }
+            bufA[t] = val;
+            const uint32_t hn = sc / T;
+            for (uint32_t h = 1; h < hn; ++h) {
+                const uint32_t m = h * T + t;
+                const uint32_t r = (h ^ (1u << (31 - __builtin_clz(h)))) * T + t;
+                bufA[m] = r == 0 ? out[m] : out[m] * bufA[r];
+            }
         }
-        uint32_t pre = m & (~m + 1u);
-        uint32_t rest = m ^ pre;
-        double val = out[pre];
-        while (rest) {
-            const uint32_t b = rest & (~rest + 1u);
-            rest ^= b;
-            pre |= b;
-            val = out[pre] * val;
-        }
-        bufA[m] = val;
     }
     __syncthreads();
-    // Removal of the uniform shift, highest bit first (:225-239).
-    for (int l = n; l > 0; --l) {
-        const uint32_t half = 1u << (l - 1);
-        const uint32_t cnt = sc >> 1;
-        for (uint32_t it = tid; it < cnt; it += nt) {
-            const uint32_t lo = it & (half - 1);
-            const uint32_t hi = it >> (l - 1);
-            const uint32_t m = (hi << l) | half | lo;
-            bufA[m] = __builtin_fma(-p, bufA[m - half], bufA[m]);
+    // Removal of the uniform shift, highest bit first (:225-239): level l
+    // takes bufA[m] = fma(-p, bufA[m - 2^(l-1)], bufA[m]) for the masks with
+    // bit l-1 set.  Two levels per pass: a thread holds the four masks that
+    // differ in bits l-1 and l-2 and applies level l, then level l-1, in
+    // registers.  The last pass writes the minors to out.
+    for (int l = n; l > 0;) {
+        const bool last = l <= 2;
+        double* dst = last ? out : bufA;
+        if (l >= 2) {
+            const uint32_t H = 1u << (l - 1);
+            const uint32_t L = 1u << (l - 2);
+            for (uint32_t g = tid; g < (sc >> 2); g += nt) {
+                const uint32_t x = ((g >> (l - 2)) << l) | (g & (L - 1));
+                const double b0 = bufA[x];
+                double bl = bufA[x + L];
+                double bh = bufA[x + H];
+                double bhl = bufA[x + H + L];
+                bh = __builtin_fma(-p, b0, bh);
+                bhl = __builtin_fma(-p, bl, bhl);
+                bl = __builtin_fma(-p, b0, bl);
+                bhl = __builtin_fma(-p, bh, bhl);
+                dst[x] = b0;
+                dst[x + L] = bl;
+                dst[x + H] = bh;
+                dst[x + H + L] = bhl;
+            }
+            l -= 2;
+        } else {
+            for (uint32_t q = tid; q < (sc >> 1); q += nt) {
+                const double b0 = bufA[2 * q];
+                dst[2 * q] = b0;
+                dst[2 * q + 1] = __builtin_fma(-p, b0, bufA[2 * q + 1]);
+            }
+            l -= 1;
         }
         __syncthreads();
     }
-    for (uint32_t m = tid; m < sc; m += nt) out[m] = bufA[m];
-    __syncthreads();
     return 0;
 }
 

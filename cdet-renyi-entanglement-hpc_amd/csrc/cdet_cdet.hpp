@@ -86,15 +86,135 @@ __device__ __forceinline__ double connected_chain(const double* a, const double*
     return acc;
 }
 
-constexpr int kFoldSteps = 2;  // steps per exchange of the wave-wide chains
-constexpr int kFoldWaves = 4;  // waves that can run wave-wide chains (fold buffer: kFoldWaves x kFoldSteps x 64)
+constexpr int kFoldSteps = 2;  // steps per exchange of the grouped chains
+constexpr int kFoldWaves = 4;  // waves that can run grouped chains (fold buffer: kFoldWaves x kFoldSteps x 64)
 
-// Whether a popcount layer of M masks with chains of R - 1 terms runs as
-// wave-wide chains: only when every mask can have a wave of its own (among
-// the first kFoldWaves) and the chains are long enough that one lane per mask
-// would leave the wave idle.
-__device__ inline bool layer_wave_chains(int M, int R, int waves) {
-    return M <= (waves < kFoldWaves ? waves : kFoldWaves) && R >= 128;
+// Lanes per mask for a popcount layer of M masks with chains of R - 1 terms
+// (0: one lane per mask).  A single mask with a long chain takes a whole wave
+// (64 lanes); up to four masks per wave, on at most kFoldWaves waves, take
+// 16 lanes each.  Per 64 terms a group of G lanes issues about 10 + 2 G wave
+// instructions (the products, then the leader's fold), against 9 per term on
+// one lane, so grouping shortens the chain about 3.5x (G = 16) with no more
+// instructions in total when a one-lane-per-mask layer would leave most lanes
+// of its wave idle.
+__device__ inline int layer_group(int M, int R, int waves) {
+    const int wf = waves < kFoldWaves ? waves : kFoldWaves;
+    if (M == 1 && R >= 128) return 64;
+    if (M <= 4 * wf && R >= 64) return 16;
+    return 0;
+}
+
+// One popcount layer as grouped chains of G lanes per mask.  The chain's terms
+// are enumerated as x = G X + j (descending x is descending X, then
+// descending j): lane j of the group holds s_lo = the j-th subset of the
+// lowest log2(G) bits of the rest and steps the high part s_hi = (s_hi - 1) &
+// rest_hi.  Every kFoldSteps steps the lanes' products go to tb[] and the
+// group's lane 0 folds them in descending order, so the arithmetic sequence is
+// the reference's.  The one excluded subset (s = rest, the mask itself) is
+// the first term of the first step and enters as 0, which leaves acc and rhs
+// unchanged.  Waves without a mask return at once; the caller synchronises.
+template <int G>
+__device__ __forceinline__ void group_chains(const double* a, double* dp, int beg, int end, int R, double thr,
+                                             bool filter, const uint16_t* masks_by_pop, double* rhs, bool single,
+                                             double* tb) {
+    static_assert(G == 16 || G == 32 || G == 64, "groups fold eight terms at a time");
+    constexpr int kPerWave = 64 / G;
+    constexpr int kLoBits = G == 64 ? 6 : (G == 32 ? 5 : (G == 16 ? 4 : (G == 8 ? 3 : (G == 4 ? 2 : 1))));
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    if (w >= kFoldWaves || beg + w * kPerWave >= end) return;  // uniform per wave
+    const int gi = lane / G;
+    const int j = lane - gi * G;
+    const int it = beg + w * kPerWave + gi;
+    const bool have = it < end;
+    const uint32_t mask = have ? masks_by_pop[it] : 1u;
+    const bool rec = have && (!single || (mask & 1u));
+    const uint32_t anchor = mask & (~mask + 1u);
+    const uint32_t rest = mask ^ anchor;
+    uint32_t rl = 0, rr = rest;
+#pragma unroll
+    for (int q = 0; q < kLoBits; ++q) {
+        const uint32_t b = rr & (~rr + 1u);
+        rl |= b;
+        rr ^= b;
+    }
+    const uint32_t rh = rest ^ rl;
+    uint32_t slo = 0;
+    {
+        uint32_t bl = rl;
+        int v = j;
+#pragma unroll
+        for (int q = 0; q < kLoBits; ++q) {
+            const uint32_t b = bl & (~bl + 1u);
+            if (v & 1) slo |= b;
+            v >>= 1;
+            bl ^= b;
+        }
+    }
+    const uint32_t base_lo = anchor | slo;
+    uint32_t shi = rh;
+    double acc = rec ? a[mask] : 0.0;
+    double rs = fabs(acc);
+    const int steps = R / G;
+    double* wtb = tb + w * (kFoldSteps * 64);
+    for (int st0 = 0; st0 < steps; st0 += kFoldSteps) {
+        // every index below is a subset of the mask, so the loads are in
+        // bounds for all lanes; invalid terms are replaced by 0
+#pragma unroll
+        for (int u = 0; u < kFoldSteps; ++u) {
+            const int st = st0 + u;
+            const uint32_t sub = base_lo | shi;
+            const double prod = dp[sub] * a[mask ^ sub];
+            const bool valid = rec && st < steps && !(st == 0 && j == G - 1);
+            wtb[u * 64 + lane] = valid ? prod : 0.0;
+            shi = (shi - 1u) & rh;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (rec && j == 0) {
+            // the group's lane 0 folds the G terms of each step in descending
+            // lane order, reading them as pairs (q - 1, q) sixteen bytes at a
+            // time, the next eight terms in flight while eight fold
+            constexpr int kChunks = G / 8;
+#pragma unroll
+            for (int u = 0; u < kFoldSteps; ++u) {
+                const double2* src = reinterpret_cast<const double2*>(wtb + u * 64 + gi * G);
+                double2 cur[4], nxt[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cur[i] = src[G / 2 - 1 - i];
+#pragma unroll
+                for (int c = 0; c < kChunks; ++c) {
+                    if (c + 1 < kChunks) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) nxt[i] = src[G / 2 - 1 - 4 * (c + 1) - i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        acc = acc - cur[i].y;
+                        rs += fabs(cur[i].y);
+                        acc = acc - cur[i].x;
+                        rs += fabs(cur[i].x);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (have && j == 0) {
+        if (rec) {
+            if (filter && fabs(acc) < thr * rs) acc = 0.0;
+            dp[mask] = acc;
+            if (rhs) rhs[mask] = rs;
+        } else {
+            dp[mask] = a[mask];
+        }
+    }
 }
 
 // Subset recursion over all popcount layers, workgroup-cooperative.
@@ -106,19 +226,9 @@ __device__ inline bool layer_wave_chains(int M, int R, int waves) {
 // connected value and, with rhs non-null, rhs[mask] the sum of |a[mask]| and
 // |terms| of its chain (the round-off filter's scale, :330-341).  masks_by_pop
 // lists every mask ordered by popcount, pop_off[k] the first index of
-// popcount k; masks of one popcount are independent.
-//
-// A layer with many masks runs one chain per lane (connected_chain).  A layer
-// with at most one mask per wave and long chains (the top layers) runs each
-// chain on a whole wave: the chain's terms are enumerated as x = 64 X + l
-// (descending x is descending X, then descending l), lane l holding s_lo =
-// the l-th subset of the lowest six bits of the rest and stepping the high
-// part s_hi = (s_hi - 1) & rest_hi; every kFoldSteps steps the lanes' products
-// go to tb[] and lane 0 folds them in descending order, so the arithmetic
-// sequence is the reference's.  The one excluded subset (s = rest, the mask
-// itself) is the first term of the first step and enters as 0, which leaves
-// acc and rhs unchanged.  tb holds kFoldWaves x kFoldSteps x 64 doubles, one
-// kFoldSteps x 64 block per wave; waves without a mask skip the layer.
+// popcount k; masks of one popcount are independent.  A layer runs one chain
+// per lane (connected_chain) or as grouped chains (group_chains, see
+// layer_group).  tb holds kFoldWaves x kFoldSteps x 64 doubles.
 __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, int n, double thr, bool filter,
                                          const uint16_t* masks_by_pop, const int* pop_off, double* rhs, bool single,
                                          double* tb) {
@@ -127,12 +237,16 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
     const uint32_t sc = 1u << n;
     for (uint32_t m = tid; m < sc; m += nt) dp[m] = (m != 0 && (m & (m - 1)) == 0) ? a[m] : 0.0;
     __syncthreads();
-    const int lane = tid & 63;
     for (int bits = 2; bits <= n; ++bits) {
         const int beg = pop_off[bits];
         const int end = pop_off[bits + 1];
         const int R = 1 << (bits - 1);
-        if (!layer_wave_chains(end - beg, R, nt >> 6)) {
+        const int G = layer_group(end - beg, R, nt >> 6);
+        if (G == 64) {
+            group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, single, tb);
+        } else if (G == 16) {
+            group_chains<16>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, single, tb);
+        } else {
             for (int it = beg + tid; it < end; it += nt) {
                 const uint32_t mask = masks_by_pop[it];
                 if (single && !(mask & 1u)) {
@@ -143,97 +257,6 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
                 double r;
                 dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, R - 1, thr, filter, r);
                 if (rhs) rhs[mask] = r;
-            }
-        } else {
-            // one mask per wave; lane l holds s_lo = the l-th subset of the
-            // lowest six bits of the rest, steps walk the high bits downwards
-            const int it = beg + (tid >> 6);
-            const bool have = it < end;  // uniform per wave
-            const uint32_t mask = have ? masks_by_pop[it] : 1u;
-            const bool rec = have && (!single || (mask & 1u));
-            const uint32_t anchor = mask & (~mask + 1u);
-            const uint32_t rest = mask ^ anchor;
-            uint32_t rl = 0, rr = rest;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const uint32_t b = rr & (~rr + 1u);
-                rl |= b;
-                rr ^= b;
-            }
-            const uint32_t rh = rest ^ rl;
-            uint32_t slo = 0;
-            {
-                uint32_t bl = rl;
-                int v = lane;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const uint32_t b = bl & (~bl + 1u);
-                    if (v & 1) slo |= b;
-                    v >>= 1;
-                    bl ^= b;
-                }
-            }
-            const uint32_t base_lo = anchor | slo;
-            uint32_t shi = rh;
-            double acc = rec ? a[mask] : 0.0;
-            double rs = fabs(acc);
-            const int steps = have ? (R >> 6) : 0;  // waves without a mask write no fold buffer
-            double* wtb = tb + (tid >> 6) * (kFoldSteps * 64);
-            for (int st0 = 0; st0 < steps; st0 += kFoldSteps) {
-                // every index below is a subset of the mask, so the loads are
-                // in bounds for all lanes; invalid terms are replaced by 0
-#pragma unroll
-                for (int u = 0; u < kFoldSteps; ++u) {
-                    const int st = st0 + u;
-                    const uint32_t sub = base_lo | shi;
-                    const double prod = dp[sub] * a[mask ^ sub];
-                    const bool valid = rec && st < steps && !(st == 0 && lane == 63);
-                    wtb[u * 64 + lane] = valid ? prod : 0.0;
-                    shi = (shi - 1u) & rh;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (rec && lane == 0) {
-                    // lane 0 folds the 64 terms of each step in descending lane
-                    // order, reading them as pairs (q - 1, q) sixteen bytes at a
-                    // time, the next eight terms in flight while eight fold
-#pragma unroll
-                    for (int u = 0; u < kFoldSteps; ++u) {
-                        const double2* src = reinterpret_cast<const double2*>(wtb + u * 64);
-                        double2 cur[4], nxt[4];
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) cur[i] = src[31 - i];
-#pragma unroll
-                        for (int c = 0; c < 8; ++c) {
-                            if (c + 1 < 8) {
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) nxt[i] = src[31 - 4 * (c + 1) - i];
-                            }
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                acc = acc - cur[i].y;
-                                rs += fabs(cur[i].y);
-                                acc = acc - cur[i].x;
-                                rs += fabs(cur[i].x);
-                            }
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
-                        }
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
-            if (have && lane == 0) {
-                if (rec) {
-                    if (filter && fabs(acc) < thr * rs) acc = 0.0;
-                    dp[mask] = acc;
-                    if (rhs) rhs[mask] = rs;
-                } else {
-                    dp[mask] = a[mask];
-                }
             }
         }
         __syncthreads();

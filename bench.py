@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
     ap.add_argument("--no-per-order", action="store_true", help="skip the per-order evals/s table")
-    ap.add_argument("--inflight", type=int, default=2,
+    ap.add_argument("--inflight", type=int, default=3,
                     help="c2: batches in flight (plans with their own buffers on their own streams)")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")

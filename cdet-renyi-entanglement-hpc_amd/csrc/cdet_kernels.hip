@@ -25,6 +25,7 @@
 #include <cstring>
 #include <atomic>
 #include <condition_variable>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -529,15 +530,17 @@ __global__ __launch_bounds__(256) void k_contract_exact(KScalars ks, const int32
 //         fallback is queued instead (entry = (li << 3) | (class << 2) | spin
 //         mask) and k_fallback_dets writes its pivoted determinants to the same
 //         place.  Identical up and down tables (same_spin) use spin 0 for both.
-// k_cdet  one workgroup per (li, class), w = 2 li + class:
-//         a = pm_up * pm_down, subset recursion, assembly.
-//   mode 0 (all subsets): class values into out (twisted) / out_ord
-//          (ordinary) at [obase, obase + 2^n); k_combine_all forms
-//          twisted - 2 * ordinary.
-//   mode 1 (single order): class value as a double-double pair into
+// k_cdet  per class: a = pm_up * pm_down, subset recursion, assembly; one
+//         workgroup per (li, class), w = 2 li + class, or (paired) one per
+//         configuration li taking the twisted class, then the ordinary one.
+//   mode 0 (all subsets): out[obase, obase + 2^n) receives twisted -
+//          2 * ordinary: formed in place when paired, else the ordinary
+//          values go to out_ord and k_combine_all forms it.
+//   mode 1 (single order): the class value as a double-double pair into
 //          single_tmp[4 c + 2 class + {0, 1}]; k_combine_single forms the
 //          double-double difference.
-//   mode 2: conditioning scales of the mode-0 values, same layout.
+//   mode 2: conditioning scales of the mode-0 values (twisted + 2 *
+//          ordinary), same layout.
 // With GSCR (orders whose workspace exceeds LDS) the workspace is global
 // scratch and a fixed grid loops over the work items; GSCR is a template
 // argument so that LDS accesses compile to ds_read/ds_write.
@@ -680,7 +683,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
                                               double* __restrict__ single_tmp, int32_t* __restrict__ status,
                                               const uint16_t* __restrict__ masks_by_pop,
                                               const int* __restrict__ pop_off, double* __restrict__ gscratch,
-                                              size_t gscratch_stride, int32_t* __restrict__ fb_count) {
+                                              size_t gscratch_stride, int32_t* __restrict__ fb_count, int paired) {
     extern __shared__ double lds_dyn[];
     __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
     __shared__ int s_popoff[kMaxOrder + 2];  // popcount-layer offsets, read every layer
@@ -698,9 +701,14 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
     double* dU = rhs + (mode == 2 ? sc : 0);
     double* dD = dU + n;
     uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + n);
+    // paired (modes 0 and 2): one work item per configuration, its two
+    // classes in turn, the ordinary class combined into the twisted values in
+    // place; otherwise one work item per (configuration, class)
+    const int classes = paired ? 2 : 1;
     for (int w = blockIdx.x; w < items; w += gridDim.x) {
-        const int li = w >> 1;
-        const int cls = w & 1;
+      for (int ci = 0; ci < classes; ++ci) {
+        const int li = paired ? w : (w >> 1);
+        const int cls = paired ? ci : (w & 1);
         const int c = cfg_list[li];
         const bool same = ks.same_spin[cls] != 0;
         const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
@@ -718,7 +726,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
             __syncthreads();
             continue;
         }
-        const double* pu = pm + static_cast<size_t>(2 * w) * sc;
+        const double* pu = pm + static_cast<size_t>(4 * li + 2 * cls) * sc;
         const double* pd = same ? pu : pu + sc;
         // a[mask] = pm_up * pm_down with a[0] = 0
         for (uint32_t m = tid; m < sc; m += nt) {
@@ -731,7 +739,11 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
             // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
             ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, false, tb);
             // Assembly (:833-877).
-            double* dst = (cls == 0 ? out : out_ord) + cfg_obase[c];
+            // paired: twisted - 2 * ordinary (include/cdet/cdet_integrand_evaluator.h:348-350),
+            // or for magnitude scales twisted + 2 * ordinary, formed in place;
+            // each thread reads back only the twisted values it wrote itself.
+            // Otherwise the ordinary values go to out_ord for k_combine_all.
+            double* dst = (cls == 0 || paired ? out : out_ord) + cfg_obase[c];
             bool bad = false;
             for (uint32_t m = tid; m < sc; m += nt) {
                 double v;
@@ -746,7 +758,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
                                   : ks.pref[cls][__builtin_popcount(m)] * dp[m];
                 }
                 if (!isfinite(v)) bad = true;
-                dst[m] = v;
+                dst[m] = (cls == 0 || !paired) ? v : (mode == 2 ? dst[m] + 2.0 * v : dst[m] - 2.0 * v);
             }
             if (bad && status) atomicOr(&status[c], 1);
         } else {
@@ -760,6 +772,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
             }
         }
         __syncthreads();
+      }
     }
 }
 
@@ -1083,6 +1096,7 @@ struct MinorsOut {
     int32_t* status;
     int32_t* fb_list;   // capacity 4 * count entries
     int32_t* fb_count;
+    bool paired = false;  // modes 0, 2: k_cdet takes both classes of a configuration
 };
 
 // Global scratch of one order bucket: the principal minors of every spin
@@ -1139,26 +1153,27 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
                                cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
         }
     }
-    const int items = 2 * count;
+    const int paired = (mode != 1 && o.paired) ? 1 : 0;
+    const int items = paired ? count : 2 * count;
     if (cdet_gscr(n, mode)) {
         const int grid = std::min(items, kGscrGrid);
         if (n >= kWideOrder)
             hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, mode, items,
                                cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
-                               popoff, next, CdetLds::doubles(n, mode), o.fb_count);
+                               popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired);
         else
             hipLaunchKernelGGL((k_cdet<true, 256>), dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items,
                                cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
-                               popoff, next, CdetLds::doubles(n, mode), o.fb_count);
+                               popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired);
     } else {
         if (n >= kWideOrder)
             hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ctx->ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
-                               o.status, masks, popoff, nullptr, 0, o.fb_count);
+                               o.status, masks, popoff, nullptr, 0, o.fb_count, paired);
         else
             hipLaunchKernelGGL((k_cdet<false, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
-                               o.status, masks, popoff, nullptr, 0, o.fb_count);
+                               o.status, masks, popoff, nullptr, 0, o.fb_count, paired);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1166,12 +1181,12 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
 
 static int launch_combine(int mode, int64_t count, int64_t out_total, const int64_t* obase, const MinorsOut& o,
                           hipStream_t stream) {
-    if (mode == 0 || mode == 2) {
-        hipLaunchKernelGGL(k_combine_all, dim3(static_cast<unsigned>((out_total + 255) / 256)), dim3(256), 0, stream,
-                           out_total, o.out, o.out_ord, mode == 2 ? 1 : 0);
-    } else {
+    if (mode == 1) {
         hipLaunchKernelGGL(k_combine_single, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, stream,
                            count, obase, o.single_tmp, o.out);
+    } else if (!o.paired) {  // paired runs combined the classes in k_cdet
+        hipLaunchKernelGGL(k_combine_all, dim3(static_cast<unsigned>((out_total + 255) / 256)), dim3(256), 0, stream,
+                           out_total, o.out, o.out_ord, mode == 2 ? 1 : 0);
     }
     HIP_TRY(hipGetLastError());
     return 0;

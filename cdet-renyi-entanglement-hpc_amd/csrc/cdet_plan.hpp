@@ -6,6 +6,7 @@
 // the sequence is captured once into a hipGraph and replayed.
 
 constexpr double kForkMinUs = 40.0;  // bucket work beyond the longest bucket worth a fork
+constexpr int kPairMin = 2;          // configurations per CU in every bucket for paired classes
 
 struct cdet_batch_plan {
     cdet_ctx* ctx = nullptr;
@@ -52,6 +53,11 @@ struct cdet_batch_plan {
     // minors scratch, since its buckets run one after another.  The aux
     // streams belong to the context (or to the submit slot the plan serves)
     // and are shared by its plans.
+    // Both classes of a configuration in one k_cdet workgroup, combined in
+    // place (no k_combine_all pass over the outputs): when every order bucket
+    // holds at least kPairMin configurations per CU, so that the longer
+    // workgroups still fill the device.
+    bool paired = false;
     int lanes = 1;
     int max_lanes = kMinorLanes;
     std::vector<int> lane_of;
@@ -178,7 +184,7 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
         if (!cnt) continue;
         const int off = p->bucket_off[static_cast<size_t>(n)];
         const int lane = p->lane_of[static_cast<size_t>(n)];
-        MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n};
+        MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n, p->paired};
         int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off, cnt,
                                p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s);
         if (rc) return rc;
@@ -189,7 +195,7 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
             HIP_TRY(hipStreamWaitEvent(s, p->ev_join[k - 1], 0));
         }
     }
-    MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, nullptr, nullptr};
+    MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, nullptr, nullptr, p->paired};
     return launch_combine(p->eval_kind, p->count, p->out_total, p->d_obase, o, s);
 }
 
@@ -372,6 +378,12 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     p->contract_grid = contract_grid(ctx, p->max_tiles);
     // Global scratch (minors of every spin matrix, workspaces of large
     // orders), sized now so that runs never allocate and can be graph-captured.
+    {
+        int min_cnt = std::numeric_limits<int>::max();
+        for (int n = 1; n <= kMaxOrder; ++n)
+            if (p->bucket_cnt[static_cast<size_t>(n)]) min_cnt = std::min(min_cnt, p->bucket_cnt[static_cast<size_t>(n)]);
+        p->paired = p->eval_kind != 1 && min_cnt >= kPairMin * ctx->ncu;
+    }
     rc = plan_lanes(p, lane_streams);
     if (rc) {
         plan_free(p);

@@ -185,6 +185,39 @@ def test_forked_order_buckets_bit_identical(gpu_available):
         single.close()
 
 
+def test_paired_classes_bit_identical(gpu_available):
+    # A single-order batch with at least two configurations per CU runs both
+    # classes of a configuration in one workgroup and combines them in place;
+    # values and magnitude scales equal those of the same configurations in
+    # small batches (one workgroup per class, separate combine pass) bit for
+    # bit, and the oracle's on a sample.
+    import torch
+    ctx = gpu_ctx("c1")
+    orc = helpers.oracle_of("c1")
+    count = 540  # >= 2 per CU on a 256-CU device
+    configs = helpers.seeded_configs(9300, [6] * count)
+    sptr = torch.cuda.current_stream().cuda_stream
+
+    def run(cfgs, kind):
+        plan = ctx.plan([len(v) for v in cfgs], "exact", kind=kind)
+        verts = torch.tensor(np.concatenate(cfgs), dtype=torch.float64, device="cuda")
+        out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+        plan.run(verts, out, None, use_graph=False, stream=sptr)
+        torch.cuda.synchronize()
+        plan.close()
+        return out.cpu().numpy()
+
+    for kind in (0, 2):
+        say(f"paired classes kind {kind}")
+        big = run(configs, kind)
+        small = np.concatenate([run(configs[i:i + 90], kind) for i in range(0, count, 90)])
+        assert bits_equal(big, small)
+        if kind == 0:
+            per = 1 << 6
+            for k in range(0, count, 53):
+                assert bits_equal(big[k * per:(k + 1) * per], orc.all_subsets(configs[k])), k
+
+
 def rel_err(got, want):
     scale = np.maximum(np.abs(want), 1e-300)
     return np.abs(got - want) / scale

@@ -417,16 +417,35 @@ __device__ inline int fpm_block(const double* A, int n, double tol, double thres
         const int items = sets * nms;
         const float inv_nms = 1.0f / static_cast<float>(nms);
         const float inv_ns = 1.0f / static_cast<float>(ns);
+        // item it = s nms + rem, rem = r ns + c; from the thread's first item
+        // the indices step by nt as (ds, drem = dr ns + dc) with carries
+        const int ds = div_small(nt, nms, inv_nms);
+        const int drem = nt - ds * nms;
+        const int dr = div_small(drem, ns, inv_ns);
+        const int dc = drem - dr * ns;
+        int s = div_small(tid, nms, inv_nms);
+        int rem = tid - s * nms;
+        int r = div_small(rem, ns, inv_ns);
+        int c = rem - r * ns;
         for (int it = tid; it < items; it += nt) {
-            const int s = div_small(it, nms, inv_nms);
-            const int rem = it - s * nms;
-            const int r = div_small(rem, ns, inv_ns);
-            const int c = rem - r * ns;
             const int src = s * oms;
             const double f = oldl[src + (r + 1) * os] * inv[s];
             const double base = oldl[src + (r + 1) * os + (c + 1)];
             newl[s * nms + rem] = base;
             newl[(s + sets) * nms + rem] = __builtin_fma(-oldl[src + 1 + c], f, base);
+            s += ds;
+            rem += drem;
+            r += dr;
+            c += dc;
+            if (c >= ns) {
+                c -= ns;
+                ++r;
+            }
+            if (rem >= nms) {
+                rem -= nms;
+                r -= ns;
+                ++s;
+            }
         }
         __syncthreads();
         double* t = oldl;

@@ -214,3 +214,28 @@ def test_driver_two_groups_in_flight_gpu(gpu_available, tmp_path):
     assert L.cdet_eval_batch_host(ctx.handle, vp, op, 1, ref.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     assert np.array_equal(out, ref)
     ctx.close()
+
+
+def test_driver_two_groups_forked_slot_plans(gpu_available, tmp_path):
+    # Envelopes up to 12: each slot's plan spreads its order buckets over two
+    # streams and launches from the slot's own thread; the chains still take
+    # exactly the steps of the synchronous host callback.
+    ctx = gpu_ctx()
+    L = cdet_amd.lib()
+    hfn = ctypes.cast(L.cdet_eval_batch_host, ctypes.c_void_p)
+    sfn = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p)
+    wfn = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p)
+    env = [12, 11, 10, 9, 12, 11, 10, 8]
+    c = c1_config(tmp_path, chain_count=len(env), checkpoint_enabled=0, lambda_warmup_steps=100,
+                  production_steps=200)
+    print("forked slots: sync", flush=True)
+    rs, rats, ts = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=300)
+    print("forked slots: async", flush=True)
+    try:
+        ra, rata, ta = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=300, submit_fn=sfn,
+                                          wait_fn=wfn)
+    except RuntimeError as e:
+        raise RuntimeError(f"{e}; library: {L.cdet_last_error().decode()}") from e
+    assert np.array_equal(ts, ta)
+    assert np.array_equal(rats, rata, equal_nan=True)
+    ctx.close()

@@ -741,7 +741,11 @@ extern "C" int cdet_eval_batch_wait(void* user, int slot, double* h_out) {
         sl.cv.wait(lk, [&sl] { return sl.launched; });
     }
     sl.pending = false;
-    if (sl.launch_rc) return fail(sl.launch_rc, sl.launch_err);
+    if (sl.launch_rc) {
+        // copies queued before the failure may still read the staging
+        (void)hipStreamSynchronize(sl.stream);
+        return fail(sl.launch_rc, sl.launch_err);
+    }
     HIP_TRY(hipStreamSynchronize(sl.stream));
     const int32_t* st = static_cast<const int32_t*>(sl.h_status);
     for (int64_t i = 0; i < sl.count; ++i)

@@ -204,7 +204,7 @@ int cdet_eval_all_subsets(cdet_ctx* ctx, const cdet_vertex* vertices, int envelo
 typedef struct cdet_batch_plan cdet_batch_plan;
 int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* h_orders, int64_t count, int mode, int eval_kind,
                            cdet_batch_plan** out_plan);
-int cdet_batch_plan_destroy(cdet_batch_plan* plan);
+int cdet_batch_plan_destroy(cdet_batch_plan* plan);  /* before its context's cdet_ctx_destroy */
 int64_t cdet_batch_out_size(const cdet_batch_plan* plan);     /* doubles written by one run */
 int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan); /* vertices read by one run */
 int64_t cdet_batch_query_count(const cdet_batch_plan* plan);  /* Green-pair lookups per run */

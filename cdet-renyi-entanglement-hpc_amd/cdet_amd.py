@@ -15,6 +15,7 @@ HIP library is missing, loading fails loudly.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -115,6 +116,7 @@ def _signatures():
         "cdet_tables_alias": (i, [vp, i, i]),
         "cdet_tables_build": (i, [vp, P(CdetLattice)]),
         "cdet_tables_download": (i, [vp, i, dp, dp, dp, dp]),
+        "cdet_tables_download_blocks": (i, [vp, i, P(ctypes.c_int64), i64, dp, dp, dp, dp]),
         "cdet_eval_single": (i, [vp, dp, i, dp]),
         "cdet_eval_all_subsets": (i, [vp, dp, i, dp]),
         "cdet_batch_plan_create": (i, [vp, P(ctypes.c_int32), i64, i, i, P(vp)]),
@@ -192,12 +194,17 @@ class Context:
 
     def __init__(self, params):
         self.params = params
+        self._plans = weakref.WeakSet()
         h = ctypes.c_void_p()
         _check(lib().cdet_ctx_create(ctypes.byref(params), ctypes.byref(h)))
         self._h = h
 
     def close(self):
+        """Destroy the context, after closing every plan made on it (a plan
+        must not outlive its context)."""
         if getattr(self, "_h", None):
+            for plan in list(getattr(self, "_plans", ())):
+                plan.close()
             lib().cdet_ctx_destroy(self._h)
             self._h = None
 
@@ -268,6 +275,19 @@ class Context:
         _check(lib().cdet_tables_download(self._h, kind, _dptr(up), _dptr(lo), _dptr(uc), _dptr(lc)))
         return up, lo, uc, lc
 
+    def download_blocks(self, kind, sitepairs):
+        """Coefficient blocks and corners of the given sitepairs of a table
+        kind: (upper, lower) as (len, nu, nv) arrays, (ucorner, lcorner)."""
+        p = self.params
+        sp = np.ascontiguousarray(sitepairs, dtype=np.int64)
+        n = len(sp)
+        up = np.zeros((n, p.cheb_nu, p.cheb_nv))
+        lo = np.zeros((n, p.cheb_nu, p.cheb_nv))
+        uc, lc = np.zeros(n), np.zeros(n)
+        _check(lib().cdet_tables_download_blocks(self._h, kind, sp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                 n, _dptr(up), _dptr(lo), _dptr(uc), _dptr(lc)))
+        return up, lo, uc, lc
+
     # -- integrand (reference API) ----------------------------------------
     def evaluate_integrand(self, vertices):
         """evaluate_cdet_density_integrand_double (cdet_integrand_evaluator.h:542-560)."""
@@ -292,7 +312,9 @@ class Context:
         _check(lib().cdet_reset_counters(self._h))
 
     def plan(self, orders, mode="exact", single=False, kind=None):
-        return BatchPlan(self, orders, mode, single, kind)
+        plan = BatchPlan(self, orders, mode, single, kind)
+        self._plans.add(plan)
+        return plan
 
 
 class BatchPlan:

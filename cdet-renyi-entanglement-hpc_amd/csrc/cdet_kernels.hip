@@ -852,6 +852,36 @@ constexpr int kMinorLanes = 4;  // streams the order buckets of one plan run spr
 // per process by default).
 constexpr int kSlotLanes = 2;
 
+// Workspace of one plain (one configuration per launch) evaluation: its own
+// stream and device buffers.  A context keeps a pool of them, so that host
+// threads evaluating at the same time (the reference calls the integrand from
+// one std::async thread per chain) each hold a workspace of their own.
+struct PlainWork {
+    hipStream_t stream = nullptr;
+    double* d_verts = nullptr;
+    double* d_gup = nullptr;
+    double* d_gdn = nullptr;
+    double* d_out = nullptr;
+    double* d_out_ord = nullptr;
+    double* d_single = nullptr;
+    int32_t* d_small_i32 = nullptr;  // [0] status, [1] fallback queue length, [2..5] queue
+    Scratch scratch;
+};
+
+// Scratch of cdet_green_pair_eval, grown to the largest call.  Each call's
+// launches first wait (on the device) for the previous call's event, so calls
+// on different streams never overlap on these buffers.
+struct QueryScratch {
+    int64_t cap = 0;
+    int64_t tiles_cap = 0;
+    int32_t nkeys = 0;
+    hipEvent_t done = nullptr;  // recorded after each call's last launch
+    double *gup = nullptr, *gdn = nullptr, *qu = nullptr, *qv = nullptr, *su = nullptr, *sv = nullptr;
+    int32_t *qkey = nullptr, *qrank = nullptr, *sdest = nullptr, *hist = nullptr, *keyoff = nullptr;
+    int32_t *tkey = nullptr, *tstart = nullptr, *tlen = nullptr, *ntiles = nullptr;
+    int2* wgtot = nullptr;
+};
+
 struct cdet_ctx {
     cdet_params p{};
     int device = 0;
@@ -863,19 +893,20 @@ struct cdet_ctx {
     uint16_t* d_masks = nullptr;  // masks by popcount for every order, (kMaxOrder+1) * 2^kMaxOrder
     int* d_popoff = nullptr;      // (kMaxOrder+1) * (kMaxOrder+2)
     uint64_t* d_fallbacks = nullptr;
-    std::atomic<uint64_t> calls{0};  // slot workers count from their own threads
-    // plain-path scratch
-    double* d_verts = nullptr;
-    double* d_gup = nullptr;
-    double* d_gdn = nullptr;
-    double* d_out = nullptr;
-    int32_t* d_status = nullptr;
-    double* d_out_ord = nullptr;
-    double* d_single = nullptr;
-    int32_t* d_small_i32 = nullptr;
+    std::atomic<uint64_t> calls{0};  // counted from every calling thread
     int32_t* d_one = nullptr;   // cfg list {0}
     int64_t* d_zero64 = nullptr;  // pbase/obase {0}
-    Scratch plain_scratch;
+    // plain-path workspaces: every one made (freed at destroy) and the idle ones
+    std::mutex plain_mu;
+    std::vector<PlainWork*> plain_all;
+    std::vector<PlainWork*> plain_idle;
+    // cdet_green_pair_eval
+    std::mutex query_mu;
+    QueryScratch query;
+    // cdet_eval_batch_host (one call at a time per context)
+    std::mutex host_mu;
+    // derived table copies (step order, split FP32), made once per table
+    std::mutex derive_mu;
     // host-buffer batch path (driver callback)
     int batch_mode = CDET_MODE_EXACT;
     cdet_batch_plan* host_plan = nullptr;
@@ -1073,6 +1104,80 @@ int grow(void** p, size_t& cap, size_t need) {
     return 0;
 }
 
+void plain_work_free(PlainWork* w) {
+    if (!w) return;
+    if (w->stream) {
+        (void)hipStreamSynchronize(w->stream);
+        (void)hipStreamDestroy(w->stream);
+    }
+    void* bufs[] = {w->d_verts, w->d_gup, w->d_gdn, w->d_out, w->d_out_ord, w->d_single, w->d_small_i32,
+                    w->scratch.ptr};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete w;
+}
+
+// A plain-path workspace sized for CDET_MAX_ORDER (the minors scratch grows
+// on first use of an order).
+int plain_work_make(PlainWork** out) {
+    auto* w = new PlainWork();
+    const int Mx = kMaxOrder;
+    auto made = [&]() -> int {
+        HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+        HIP_TRY(hipMalloc(&w->d_verts, sizeof(double) * 4 * Mx));
+        HIP_TRY(hipMalloc(&w->d_gup, sizeof(double) * 2 * Mx * Mx));
+        HIP_TRY(hipMalloc(&w->d_gdn, sizeof(double) * 2 * Mx * Mx));
+        HIP_TRY(hipMalloc(&w->d_out, sizeof(double) << Mx));
+        HIP_TRY(hipMalloc(&w->d_out_ord, sizeof(double) << Mx));
+        HIP_TRY(hipMalloc(&w->d_single, sizeof(double) * 4));
+        HIP_TRY(hipMalloc(&w->d_small_i32, sizeof(int32_t) * 8));
+        return 0;
+    };
+    const int rc = made();
+    if (rc) {
+        plain_work_free(w);
+        return rc;
+    }
+    *out = w;
+    return 0;
+}
+
+// Take an idle workspace of the context, or make one.
+int plain_work_acquire(cdet_ctx* ctx, PlainWork** out) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->plain_mu);
+        if (!ctx->plain_idle.empty()) {
+            *out = ctx->plain_idle.back();
+            ctx->plain_idle.pop_back();
+            return 0;
+        }
+    }
+    PlainWork* w = nullptr;
+    const int rc = plain_work_make(&w);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->plain_mu);
+    ctx->plain_all.push_back(w);
+    *out = w;
+    return 0;
+}
+
+void plain_work_release(cdet_ctx* ctx, PlainWork* w) {
+    std::lock_guard<std::mutex> lk(ctx->plain_mu);
+    ctx->plain_idle.push_back(w);
+}
+
+// Free the buffers once the last call that used them has finished.
+void query_scratch_free(QueryScratch& q) {
+    if (q.done) (void)hipEventSynchronize(q.done);
+    void* bufs[] = {q.gup, q.gdn, q.qu, q.qv, q.su, q.sv, q.qkey, q.qrank, q.sdest, q.hist, q.keyoff,
+                    q.tkey, q.tstart, q.tlen, q.ntiles, q.wgtot};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    hipEvent_t done = q.done;
+    q = QueryScratch();
+    q.done = done;
+}
+
 }  // namespace
 
 static constexpr size_t kMaxLds = 160 * 1024 - 2048;
@@ -1213,6 +1318,7 @@ static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner);
 // contraction (nu, nv multiples of 16), built once per table.
 static int ensure_step_tables(cdet_ctx* ctx) {
     if (!use_f64_v2(ctx->ks)) return 0;
+    std::lock_guard<std::mutex> lock(ctx->derive_mu);
     size_t nc, nk;
     table_sizes(ctx, nc, nk);
     bool made = false;
@@ -1238,10 +1344,12 @@ static int ensure_step_tables(cdet_ctx* ctx) {
 
 // FP32 hi/lo copies of every resident table (split-FP32 mode), built once.
 static int ensure_split_tables(cdet_ctx* ctx) {
+    std::lock_guard<std::mutex> lock(ctx->derive_mu);
     size_t nc, nk;
     const size_t S = static_cast<size_t>(ctx->p.N1) * ctx->p.N2;
     nk = 4 * S * S;
     nc = nk * static_cast<size_t>(ctx->p.cheb_nu) * ctx->p.cheb_nv;
+    bool made = false;
     for (int kind = 0; kind < 4; ++kind) {
         DevTable& t = ctx->tab[kind];
         if (t.alias >= 0 || !t.upper) continue;
@@ -1253,10 +1361,13 @@ static int ensure_split_tables(cdet_ctx* ctx) {
             hipLaunchKernelGGL(k_split_f32, dim3(static_cast<unsigned>((nc + 255) / 256)), dim3(256), 0, ctx->stream, src,
                                static_cast<int64_t>(nc), t.split[br][0], t.split[br][1]);
             HIP_TRY(hipGetLastError());
+            made = true;
         }
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    refresh_scalars(ctx);
+    if (made) {  // only a change of the table pointers re-captures the plans' graphs
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        refresh_scalars(ctx);
+    }
     return 0;
 }
 
@@ -1328,24 +1439,14 @@ int cdet_params_derive(cdet_params* p) {
     return 0;
 }
 
-int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
-    if (!params || !out_ctx) return fail(CDET_ERR_ARGUMENT, "null argument");
-    if (params->N1 <= 0 || params->N2 <= 0 || params->cheb_nu <= 0 || params->cheb_nv <= 0)
-        return fail(CDET_ERR_ARGUMENT, "lattice extents and Chebyshev counts must be positive");
-    if (params->cheb_nu > 256) return fail(CDET_ERR_CAPACITY, "cheb_nu above 256 is not supported");
-    auto* ctx = new cdet_ctx();
-    ctx->p = *params;
-    if (hipGetDevice(&ctx->device) != hipSuccess) {
-        delete ctx;
-        return fail(CDET_ERR_DEVICE, "no HIP device");
-    }
+// Device state of a new context: stream, mask lists, counters, the first
+// plain-path workspace, kernel attributes.
+static int ctx_init(cdet_ctx* ctx) {
+    if (hipGetDevice(&ctx->device) != hipSuccess) return fail(CDET_ERR_DEVICE, "no HIP device");
     if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
         ctx->ncu <= 0)
         ctx->ncu = 256;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete ctx;
-        return fail(CDET_ERR_DEVICE, "hipStreamCreate failed");
-    }
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     std::vector<uint16_t> masks;
     std::vector<int> off;
     masks_by_popcount(masks, off);
@@ -1355,19 +1456,16 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
     HIP_TRY(hipMemcpy(ctx->d_popoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&ctx->d_fallbacks, sizeof(uint64_t)));
     HIP_TRY(hipMemset(ctx->d_fallbacks, 0, sizeof(uint64_t)));
-    const int Mx = kMaxOrder;
-    HIP_TRY(hipMalloc(&ctx->d_verts, sizeof(double) * 4 * Mx));
-    HIP_TRY(hipMalloc(&ctx->d_gup, sizeof(double) * 2 * Mx * Mx));
-    HIP_TRY(hipMalloc(&ctx->d_gdn, sizeof(double) * 2 * Mx * Mx));
-    HIP_TRY(hipMalloc(&ctx->d_out, sizeof(double) << Mx));
-    HIP_TRY(hipMalloc(&ctx->d_status, sizeof(int32_t)));
-    HIP_TRY(hipMalloc(&ctx->d_out_ord, sizeof(double) << Mx));
-    HIP_TRY(hipMalloc(&ctx->d_single, sizeof(double) * 4));
-    HIP_TRY(hipMalloc(&ctx->d_small_i32, sizeof(int32_t) * 8));
     HIP_TRY(hipMalloc(&ctx->d_one, sizeof(int32_t)));
     HIP_TRY(hipMalloc(&ctx->d_zero64, sizeof(int64_t)));
     HIP_TRY(hipMemset(ctx->d_one, 0, sizeof(int32_t)));
     HIP_TRY(hipMemset(ctx->d_zero64, 0, sizeof(int64_t)));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->query.done, hipEventDisableTiming));
+    PlainWork* w = nullptr;
+    int rc = plain_work_make(&w);
+    if (rc) return rc;
+    ctx->plain_all.push_back(w);
+    ctx->plain_idle.push_back(w);
     // Allow the order-bucketed kernel the full LDS of a CU.
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false, 256>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
@@ -1380,13 +1478,58 @@ int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     refresh_scalars(ctx);
+    return 0;
+}
+
+int cdet_ctx_create(const cdet_params* params, cdet_ctx** out_ctx) {
+    if (!params || !out_ctx) return fail(CDET_ERR_ARGUMENT, "null argument");
+    if (params->N1 <= 0 || params->N2 <= 0 || params->cheb_nu <= 0 || params->cheb_nv <= 0)
+        return fail(CDET_ERR_ARGUMENT, "lattice extents and Chebyshev counts must be positive");
+    if (params->cheb_nu > 256) return fail(CDET_ERR_CAPACITY, "cheb_nu above 256 is not supported");
+    auto* ctx = new cdet_ctx();
+    ctx->p = *params;
+    const int rc = ctx_init(ctx);
+    if (rc) {
+        const std::string msg = g_last_error;
+        cdet_ctx_destroy(ctx);
+        return fail(rc, msg);
+    }
     *out_ctx = ctx;
     return 0;
 }
 
+// Teardown in three phases: first everything that can still run on the
+// device is stopped or waited for (slot threads joined, every stream and the
+// last lookup call synchronised), then the plans the context made are
+// destroyed, then the buffers are freed and the streams destroyed.  Plans a
+// caller created with cdet_batch_plan_create must be destroyed before.
 int cdet_ctx_destroy(cdet_ctx* ctx) {
     if (!ctx) return 0;
-    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& sl : ctx->slots) {
+        if (sl.worker.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(sl.mu);
+                sl.quit = true;
+            }
+            sl.cv.notify_all();
+            sl.worker.join();
+        }
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        for (hipStream_t l : sl.lanes)
+            if (l) (void)hipStreamSynchronize(l);
+    }
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t l : ctx->lanes)
+        if (l) (void)hipStreamSynchronize(l);
+    for (PlainWork* w : ctx->plain_all)
+        if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (ctx->query.done) (void)hipEventSynchronize(ctx->query.done);
+    if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
+    ctx->host_plan = nullptr;
+    for (auto& sl : ctx->slots) {
+        if (sl.plan) cdet_batch_plan_destroy(sl.plan);
+        sl.plan = nullptr;
+    }
     for (auto& t : ctx->tab) {
         if (t.alias < 0) {
             (void)hipFree(t.upper);
@@ -1399,35 +1542,17 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_masks);
     (void)hipFree(ctx->d_popoff);
     (void)hipFree(ctx->d_fallbacks);
-    (void)hipFree(ctx->d_verts);
-    (void)hipFree(ctx->d_gup);
-    (void)hipFree(ctx->d_gdn);
-    (void)hipFree(ctx->d_out);
-    (void)hipFree(ctx->d_status);
-    (void)hipFree(ctx->d_out_ord);
-    (void)hipFree(ctx->d_single);
-    (void)hipFree(ctx->d_small_i32);
     (void)hipFree(ctx->d_one);
     (void)hipFree(ctx->d_zero64);
-    (void)hipFree(ctx->plain_scratch.ptr);
-    if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
+    for (PlainWork* w : ctx->plain_all) plain_work_free(w);
+    ctx->plain_all.clear();
+    ctx->plain_idle.clear();
+    query_scratch_free(ctx->query);
+    if (ctx->query.done) (void)hipEventDestroy(ctx->query.done);
     for (auto& sl : ctx->slots) {
-        if (sl.worker.joinable()) {
-            {
-                std::lock_guard<std::mutex> lk(sl.mu);
-                sl.quit = true;
-            }
-            sl.cv.notify_all();
-            sl.worker.join();
-        }
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.plan) cdet_batch_plan_destroy(sl.plan);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         for (hipStream_t l : sl.lanes)
-            if (l) {
-                (void)hipStreamSynchronize(l);
-                (void)hipStreamDestroy(l);
-            }
+            if (l) (void)hipStreamDestroy(l);
         (void)hipFree(sl.d_v);
         (void)hipFree(sl.d_out);
         (void)hipFree(sl.d_status);
@@ -1436,10 +1561,7 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
         (void)hipHostFree(sl.h_status);
     }
     for (hipStream_t l : ctx->lanes)
-        if (l) {
-            (void)hipStreamSynchronize(l);
-            (void)hipStreamDestroy(l);
-        }
+        if (l) (void)hipStreamDestroy(l);
     (void)hipHostFree(ctx->h_hv);
     (void)hipHostFree(ctx->h_hout);
     (void)hipHostFree(ctx->h_hstatus);
@@ -1449,7 +1571,7 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_seed_tab);
     (void)hipFree(ctx->d_seed_in);
     (void)hipFree(ctx->d_seed_out);
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
 }
@@ -1547,25 +1669,49 @@ int cdet_tables_download(cdet_ctx* ctx, int kind, double* upper, double* lower, 
     return 0;
 }
 
-static int plain_eval(cdet_ctx* ctx, const cdet_vertex* v, int n, int mode, double* out, size_t out_count) {
-    if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
-    hipStream_t s = ctx->stream;
-    HIP_TRY(hipMemcpyAsync(ctx->d_verts, v, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->d_small_i32, static_cast<int64_t>(8), 0);
-    hipLaunchKernelGGL(k_pairs_plain, dim3(2 * n * n), dim3(64), 0, s, ctx->ks, ctx->d_verts, n, ctx->d_gup,
-                       ctx->d_gdn);
+int cdet_tables_download_blocks(cdet_ctx* ctx, int kind, const int64_t* sitepairs, int64_t nblocks, double* upper,
+                                double* lower, double* ucorner, double* lcorner) {
+    if (!ctx || kind < 0 || kind > 3 || nblocks < 0 || (nblocks > 0 && !sitepairs))
+        return fail(CDET_ERR_ARGUMENT, "cdet_tables_download_blocks: bad argument");
+    const int k = ctx->tab[kind].alias >= 0 ? ctx->tab[kind].alias : kind;
+    const DevTable& t = ctx->tab[k];
+    if (!t.upper) return fail(CDET_ERR_TABLES, "cdet_tables_download_blocks: kind has no table");
+    size_t nc, nk;
+    table_sizes(ctx, nc, nk);
+    const size_t blk = static_cast<size_t>(ctx->p.cheb_nu) * ctx->p.cheb_nv;
+    for (int64_t i = 0; i < nblocks; ++i)
+        if (sitepairs[i] < 0 || static_cast<size_t>(sitepairs[i]) >= nk)
+            return fail(CDET_ERR_ARGUMENT, "cdet_tables_download_blocks: sitepair out of range");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < nblocks; ++i) {
+        const size_t sp = static_cast<size_t>(sitepairs[i]);
+        if (upper) HIP_TRY(hipMemcpy(upper + i * blk, t.upper + sp * blk, blk * sizeof(double), hipMemcpyDeviceToHost));
+        if (lower) HIP_TRY(hipMemcpy(lower + i * blk, t.lower + sp * blk, blk * sizeof(double), hipMemcpyDeviceToHost));
+        if (ucorner) HIP_TRY(hipMemcpy(ucorner + i, t.ucorner + sp, sizeof(double), hipMemcpyDeviceToHost));
+        if (lcorner) HIP_TRY(hipMemcpy(lcorner + i, t.lcorner + sp, sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+// One configuration on a workspace of the context's pool: host-to-device
+// copy, launches, device-to-host copy, wait, all on the workspace's stream.
+// Threads that call at the same time hold different workspaces.
+static int plain_run(cdet_ctx* ctx, PlainWork* w, const cdet_vertex* v, int n, int mode, double* out,
+                     size_t out_count) {
+    hipStream_t s = w->stream;
+    HIP_TRY(hipMemcpyAsync(w->d_verts, v, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, w->d_small_i32, static_cast<int64_t>(8), 0);
+    hipLaunchKernelGGL(k_pairs_plain, dim3(2 * n * n), dim3(64), 0, s, ctx->ks, w->d_verts, n, w->d_gup, w->d_gdn);
     HIP_TRY(hipGetLastError());
-    // d_small_i32: [0] status, [1] queue length, [2..5] queue
-    MinorsOut o{ctx->d_out, ctx->d_out_ord, ctx->d_single, ctx->d_small_i32, ctx->d_small_i32 + 2,
-                ctx->d_small_i32 + 1};
-    int rc = launch_minors(ctx, ctx->plain_scratch, n, mode, ctx->d_one, 1, ctx->d_zero64, ctx->d_zero64, ctx->d_gup,
-                           ctx->d_gdn, o, s);
+    MinorsOut o{w->d_out, w->d_out_ord, w->d_single, w->d_small_i32, w->d_small_i32 + 2, w->d_small_i32 + 1};
+    int rc = launch_minors(ctx, w->scratch, n, mode, ctx->d_one, 1, ctx->d_zero64, ctx->d_zero64, w->d_gup, w->d_gdn,
+                           o, s);
     if (rc) return rc;
     rc = launch_combine(mode, 1, static_cast<int64_t>(out_count), ctx->d_zero64, o, s);
     if (rc) return rc;
     int32_t st = 0;
-    HIP_TRY(hipMemcpyAsync(out, ctx->d_out, sizeof(double) * out_count, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&st, ctx->d_small_i32, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out, w->d_out, sizeof(double) * out_count, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&st, w->d_small_i32, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     ctx->calls += 1;
     if ((st & 1) && n > 1) {
@@ -1573,6 +1719,17 @@ static int plain_eval(cdet_ctx* ctx, const cdet_vertex* v, int n, int mode, doub
         return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
     }
     return 0;
+}
+
+static int plain_eval(cdet_ctx* ctx, const cdet_vertex* v, int n, int mode, double* out, size_t out_count) {
+    if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
+    PlainWork* w = nullptr;
+    int rc = plain_work_acquire(ctx, &w);
+    if (rc) return rc;
+    rc = plain_run(ctx, w, v, n, mode, out, out_count);
+    if (rc) (void)hipStreamSynchronize(w->stream);  // nothing of a failed call stays in flight
+    plain_work_release(ctx, w);
+    return rc;
 }
 
 int cdet_eval_all_subsets(cdet_ctx* ctx, const cdet_vertex* vertices, int envelope_order, double* out_values) {

@@ -73,7 +73,17 @@ struct cdet_batch_plan {
     void* g_out = nullptr;
     void* g_status = nullptr;
     hipStream_t g_stream = nullptr;
+    // recorded on the run's stream after each run's last launch; the plan is
+    // freed only once it has completed
+    hipEvent_t done = nullptr;
+    bool ran = false;
 };
+
+static int plan_mark_done(cdet_batch_plan* p, hipStream_t s) {
+    HIP_TRY(hipEventRecord(p->done, s));
+    p->ran = true;
+    return 0;
+}
 
 // The derived table copies the plan's contraction reads (split FP32, step
 // order), made before a run when the tables changed since.
@@ -85,6 +95,8 @@ static int plan_tables(cdet_batch_plan* p) {
 
 static void plan_free(cdet_batch_plan* p) {
     if (!p) return;
+    if (p->ran) (void)hipEventSynchronize(p->done);
+    if (p->done) (void)hipEventDestroy(p->done);
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     for (int k = 0; k + 1 < kMinorLanes; ++k) {
@@ -362,6 +374,7 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     if (!rc && (hipMemset(p->d_hist, 0, sizeof(int32_t) * static_cast<size_t>(p->nkeys) * kKeySub) != hipSuccess ||
                 hipMemset(p->d_fb_count, 0, sizeof(int32_t) * (kMaxOrder + 1)) != hipSuccess))
         rc = 1;
+    if (!rc && hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess) rc = 1;
     if (rc) {
         plan_free(p);
         return CDET_ERR_DEVICE;
@@ -405,7 +418,6 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, 
 }
 
 int cdet_batch_plan_destroy(cdet_batch_plan* plan) {
-    if (plan && plan->ctx) (void)hipStreamSynchronize(plan->ctx->stream);
     plan_free(plan);
     return 0;
 }
@@ -423,12 +435,15 @@ int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
     if (!use_graph) {
         int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
+        if (!rc) rc = plan_mark_done(p, s);
         if (!rc) p->ctx->calls += static_cast<uint64_t>(p->count);
         return rc;
     }
     int rc = plan_capture(p, d_vertices, d_out, d_status, s);
     if (rc) return rc;
     HIP_TRY(hipGraphLaunch(p->graph_exec, s));
+    rc = plan_mark_done(p, s);
+    if (rc) return rc;
     p->ctx->calls += static_cast<uint64_t>(p->count);
     return 0;
 }
@@ -450,6 +465,8 @@ int cdet_batch_profile(cdet_batch_plan* p, const cdet_vertex* d_vertices, double
     rc = plan_minors(p, d_out, d_status, s);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ev[3], s));
+    rc = plan_mark_done(p, s);
+    if (rc) return rc;
     HIP_TRY(hipEventSynchronize(ev[3]));
     for (int k = 0; k < 3; ++k) HIP_TRY(hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]));
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -474,25 +491,6 @@ int cdet_eval_batch(cdet_ctx* ctx, const cdet_vertex* d_vertices, const int32_t*
 // Table evaluation only (cdet_green_pair_eval): lookups grouped by coefficient
 // block and contracted by the same kernels as the integrand batches.
 // ---------------------------------------------------------------------------
-struct QueryScratch {
-    int64_t cap = 0;
-    int64_t tiles_cap = 0;
-    double *gup = nullptr, *gdn = nullptr, *qu = nullptr, *qv = nullptr, *su = nullptr, *sv = nullptr;
-    int32_t *qkey = nullptr, *qrank = nullptr, *sdest = nullptr, *hist = nullptr, *keyoff = nullptr;
-    int32_t *tkey = nullptr, *tstart = nullptr, *tlen = nullptr, *ntiles = nullptr;
-    int2* wgtot = nullptr;
-};
-
-static QueryScratch& query_scratch(cdet_ctx* ctx) {
-    static std::mutex mu;
-    static std::vector<std::pair<cdet_ctx*, QueryScratch*>> all;
-    std::lock_guard<std::mutex> lock(mu);
-    for (auto& e : all)
-        if (e.first == ctx) return *e.second;
-    all.emplace_back(ctx, new QueryScratch());
-    return *all.back().second;
-}
-
 extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus, int64_t count,
                                     double* out, int mode, void* stream) {
     if (!ctx || !query_ints || !query_taus || !out || count <= 0)
@@ -503,16 +501,16 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
         int rc = mode == CDET_MODE_SPLIT32 ? ensure_split_tables(ctx) : ensure_step_tables(ctx);
         if (rc) return rc;
     }
-    QueryScratch& q = query_scratch(ctx);
+    // The context's lookup scratch is shared by every call: one call at a
+    // time sets it up and enqueues (query_mu), and each call's launches wait
+    // on the device for the previous call's to finish.
+    std::lock_guard<std::mutex> lock(ctx->query_mu);
+    QueryScratch& q = ctx->query;
     const int32_t nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
     const int tile = mode == CDET_MODE_EXACT ? 64 : mfma_tile_queries(ctx->ks, mode, count, ctx->ncu);
     const int64_t max_tiles = count / tile + std::min<int64_t>(nkeys, count) + 1;
-    if (q.cap < count || q.tiles_cap < max_tiles || !q.hist) {
-        void* bufs[] = {q.gup, q.gdn, q.qu, q.qv, q.su, q.sv, q.qkey, q.qrank, q.sdest, q.hist, q.keyoff,
-                        q.tkey, q.tstart, q.tlen, q.ntiles, q.wgtot};
-        for (void* b : bufs)
-            if (b) (void)hipFree(b);
-        q = QueryScratch();
+    if (q.cap < count || q.tiles_cap < max_tiles || q.nkeys != nkeys || !q.hist) {
+        query_scratch_free(q);
         int rc = 0;
         rc |= dev_alloc(&q.gup, count);
         rc |= dev_alloc(&q.gdn, count);
@@ -530,11 +528,16 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
         rc |= dev_alloc(&q.tstart, max_tiles);
         rc |= dev_alloc(&q.tlen, max_tiles);
         rc |= dev_alloc(&q.ntiles, 1);
-        if (rc) return CDET_ERR_DEVICE;
+        if (rc) {
+            query_scratch_free(q);
+            return CDET_ERR_DEVICE;
+        }
         q.cap = count;
         q.tiles_cap = max_tiles;
+        q.nkeys = nkeys;
     }
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    HIP_TRY(hipStreamWaitEvent(s, q.done, 0));
     const unsigned gq = static_cast<unsigned>((count + 255) / 256);
     hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nkeys * kKeySub + 255) / 256)), dim3(256), 0, s,
                        q.hist, static_cast<int64_t>(nkeys) * kKeySub, 0);
@@ -557,6 +560,7 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     }
     hipLaunchKernelGGL(k_interleave, dim3(gq), dim3(256), 0, s, count, q.gup, q.gdn, out);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(q.done, s));
     return 0;
 }
 
@@ -602,6 +606,7 @@ extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, c
     cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
     if (!ctx || !h_vertices || !h_orders || !h_out || count <= 0)
         return fail(CDET_ERR_ARGUMENT, "cdet_eval_batch_host: bad argument");
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
     const bool same = ctx->host_plan && static_cast<int64_t>(ctx->host_plan_orders.size()) == count &&
                       std::equal(ctx->host_plan_orders.begin(), ctx->host_plan_orders.end(), h_orders);
     if (!same) {
@@ -648,6 +653,8 @@ static int slot_launch(cdet_ctx::HostSlot& sl) {
     HIP_TRY(hipMemcpyAsync(sl.d_v, sl.h_v, sizeof(cdet_vertex) * static_cast<size_t>(p->vert_total),
                            hipMemcpyHostToDevice, sl.stream));
     HIP_TRY(hipGraphLaunch(p->graph_exec, sl.stream));
+    int rc = plan_mark_done(p, sl.stream);
+    if (rc) return rc;
     p->ctx->calls += static_cast<uint64_t>(p->count);
     HIP_TRY(hipMemcpyAsync(sl.h_out, sl.d_out, sl.out_bytes, hipMemcpyDeviceToHost, sl.stream));
     HIP_TRY(hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int32_t) * static_cast<size_t>(sl.count),

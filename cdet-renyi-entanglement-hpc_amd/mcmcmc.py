@@ -236,6 +236,68 @@ def config_from_params(params, **over):
     return c
 
 
+def _normalize_name(name):
+    """normalize_real_type_name (include/cdet/precision_config.h:28-36)."""
+    return name.lower().replace("-", "_")
+
+
+def table_workflow_from_params(params, environ=None):
+    """LOAD_DATA_WORKFLOW, MCMCMC_TABLE_BACKEND and LOAD_DATA_CHUNK_DIR with
+    their environment overrides and aliases (src/main.cpp:322-345, :428-446).
+    Returns (workflow, backend, chunk_dir); raises RuntimeError with the
+    reference's message for an unknown backend."""
+    env = os.environ if environ is None else environ
+    workflow = _normalize_name(_one(params, "LOAD_DATA_WORKFLOW", "resident"))
+    if env.get("CDET_LOAD_DATA_WORKFLOW"):
+        workflow = _normalize_name(env["CDET_LOAD_DATA_WORKFLOW"])
+    workflow = {"build_resident": "resident", "load_prebuilt": "load_chunks",
+                "load_prebuilt_coeffs": "load_chunks", "sample": "sample_nodes",
+                "precompute": "precompute_calculators", "calculator_cache": "precompute_calculators",
+                "transform": "transform_sitepairs", "diagonal_observable": "diagonal_reduce",
+                "diagonal_lines": "diagonal_reduce"}.get(workflow, workflow)
+    backend = _normalize_name(_one(params, "MCMCMC_TABLE_BACKEND", "resident"))
+    if env.get("CDET_MCMCMC_TABLE_BACKEND"):
+        backend = _normalize_name(env["CDET_MCMCMC_TABLE_BACKEND"])
+    if backend in ("lru", "on_demand", "disk", "disk_cache"):
+        backend = "disk_lru"
+    if backend not in ("resident", "disk_lru"):
+        raise RuntimeError("MCMCMC_TABLE_BACKEND must be resident or disk_lru")
+    chunk_dir = _one(params, "LOAD_DATA_CHUNK_DIR", "chunked_green_tables")
+    if env.get("CDET_LOAD_DATA_CHUNK_DIR"):
+        chunk_dir = env["CDET_LOAD_DATA_CHUNK_DIR"]
+    return workflow, backend, chunk_dir
+
+
+def load_data_from_params(ctx, params, environ=None):
+    """load_data() (include/cdet/load_data.h:119-178) onto a GPU context.
+
+    LOAD_DATA_WORKFLOW=load_chunks streams the chunk directory's coefficient
+    files into HBM.  Both table backends are accepted: `resident` as the
+    reference means it, and `disk_lru` (coefficient chunks kept on disk
+    behind an LRU cache, include/cdet/replica_chebyshev_tables.h:56-152) maps
+    to the same full residency in HBM, since the largest tables of this path
+    (10x10 lattice, 192 coefficients: 4 x 23.6 GB) fit one GPU's 288 GB; an
+    MCMCMC_LRU_CACHE_GB budget is accepted and not needed.
+    LOAD_DATA_WORKFLOW=resident builds the tables on the GPU.  Other workflows
+    and disk_lru without load_chunks raise the reference's errors.
+    Returns {"workflow", "backend", "residency"}."""
+    workflow, backend, chunk_dir = table_workflow_from_params(params, environ)
+    if workflow == "load_chunks":
+        ctx.load_chunks(chunk_dir)
+        return {"workflow": workflow, "backend": backend, "residency": "hbm"}
+    if backend == "disk_lru":
+        raise RuntimeError("MCMCMC_TABLE_BACKEND=disk_lru requires LOAD_DATA_WORKFLOW=load_chunks "
+                           "so coefficient chunks can be read on demand")
+    if workflow != "resident":
+        raise RuntimeError("load_data() can only run with LOAD_DATA_WORKFLOW=resident or load_chunks; "
+                           "standalone chunk stages must exit before MCMCMC starts")
+    mask = _normalize_name(_one(params, "SUBREGION_MASK"))
+    radius = int(_one(params, "SUBREGION_SQUARE_RADIUS", "0"))
+    ctx.build_tables(pbc=_bool(_one(params, "PBC")), mask=mask,
+                     mask_param=radius if mask == "origin_square" else 0)
+    return {"workflow": workflow, "backend": backend, "residency": "hbm"}
+
+
 def compare_uniform_seed(params):
     """MCMCMC_COMPARE_UNIFORM_SEED (src/main.cpp:508-509)."""
     return _bool(_one(params, "MCMCMC_COMPARE_UNIFORM_SEED", "false"))

@@ -12,6 +12,26 @@
  * error on the calling host thread is returned by cdet_last_error().  The
  * messages of errors that the reference raises as std::runtime_error carry the
  * reference's text so that a host shim can rethrow them unchanged.
+ *
+ * Threading.  The reference calls the integrand concurrently from one
+ * std::async thread per chain (include/cdet/mcmcmc_sampler.h:1104-1110), each
+ * with thread_local workspaces.  Here:
+ *  - cdet_eval_single / cdet_eval_all_subsets may be called from any number of
+ *    host threads at once on one context: each call takes a workspace (stream
+ *    and device buffers) of the context's pool, or makes one, and returns it
+ *    when it has finished; results do not depend on the interleaving.
+ *  - cdet_green_pair_eval calls from several threads are serialised on the
+ *    context's lookup scratch (the later call's launches wait on the device
+ *    for the earlier call's).
+ *  - cdet_eval_batch_host holds a context lock for the whole call;
+ *    cdet_eval_batch_submit / _wait on one slot belong to one thread.
+ *  - A cdet_batch_plan is used by one thread at a time; different plans of a
+ *    context may run concurrently on different streams.
+ *  - Tables (upload, alias, build, chunk loads) change only while no
+ *    evaluation runs, as the reference fills its globals before the chains
+ *    start (include/cdet/load_data.h:119-178).
+ *  - Counters are atomic; call counts are exact, the fallback count covers
+ *    work that has completed on the device.
  */
 #ifndef CDET_HIP_H
 #define CDET_HIP_H
@@ -165,6 +185,17 @@ int cdet_tables_build(cdet_ctx* ctx, const cdet_lattice* lattice);
 int cdet_tables_download(cdet_ctx* ctx, int kind,
                          double* upper_coeffs, double* lower_coeffs,
                          double* upper_corners, double* lower_corners);
+
+/*
+ * Copy selected coefficient blocks of a device table kind back to host
+ * buffers: block i of each branch (nu*nv doubles) is the table's block
+ * sitepairs[i], written at upper/lower + i*nu*nv; corner i at ucorner/lcorner
+ * + i.  Any output may be NULL.  The partial form of cdet_tables_download for
+ * tables too large to copy whole (the 10x10, 192-coefficient tables hold
+ * 23.6 GB per kind); no counterpart in the reference.
+ */
+int cdet_tables_download_blocks(cdet_ctx* ctx, int kind, const int64_t* sitepairs, int64_t nblocks,
+                                double* upper, double* lower, double* ucorner, double* lcorner);
 
 /*
  * Single-order integrand C(n) of all n vertices, twisted - 2*ordinary combined

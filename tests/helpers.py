@@ -61,3 +61,79 @@ def seeded_configs(seed, orders, lo=0.0, hi=1.0):
         out.append(u[k:k + n].copy())
         k += n
     return out
+
+
+# ---------------------------------------------------------------------------
+# Oracle views of large device tables: only the coefficient blocks a sample of
+# lookups reads is copied back; the rest of each full-size array stays an
+# untouched zero mapping (np.zeros maps pages lazily), so the oracle indexes
+# the reference layout directly while host memory holds only those blocks.
+# ---------------------------------------------------------------------------
+def unit_to_index(coordinate, extent):
+    """sampled_coordinate_to_index (include/cdet/g_funcs.h:315-342)."""
+    i = int(np.floor(extent * coordinate))
+    return min(max(i, 0), extent - 1)
+
+
+def sitepair_index(ra, rb, x1, y1, x2, y2, N1, N2):
+    """replica_sitepair_index (include/cdet/replica_chebyshev_tables.h:365-382)."""
+    return ((((ra * 2 + rb) * N1 + x1) * N2 + y1) * N1 + x2) * N2 + y2
+
+
+def config_sitepairs(verts, N1, N2):
+    """Sitepairs the twisted (replicas decoded, rc = 2) and ordinary (replica
+    0) classes of a configuration read, as two sets."""
+    dec = [(unit_to_index(v[1], N1), unit_to_index(v[2], N2), unit_to_index(v[3], 2)) for v in verts]
+    tw, od = set(), set()
+    for (x1, y1, r1) in dec:
+        for (x2, y2, r2) in dec:
+            tw.add(sitepair_index(r1, r2, x1, y1, x2, y2, N1, N2))
+            od.add(sitepair_index(0, 0, x1, y1, x2, y2, N1, N2))
+    return tw, od
+
+
+def sparse_tables(ctx, sitepairs_by_kind):
+    """Four oracle tables (kinds 0..3) of the context's geometry holding the
+    device values of the given sitepair blocks per kind, zeros elsewhere.
+    Kinds whose sitepair set is the same object share one table."""
+    p = ctx.params
+    S = p.N1 * p.N2
+    SP = 4 * S * S
+    blk = p.cheb_nu * p.cheb_nv
+    made = {}
+    out = []
+    for kind in range(4):
+        sps = sitepairs_by_kind[kind]
+        if id(sps) in made:
+            out.append(made[id(sps)])
+            continue
+        up = np.zeros(SP * blk)
+        lo = np.zeros(SP * blk)
+        uc = np.zeros(SP)
+        lc = np.zeros(SP)
+        idx = np.array(sorted(sps), dtype=np.int64)
+        if len(idx):
+            bu, bl, cu, cl = ctx.download_blocks(kind, idx)
+            for k, sp in enumerate(idx):
+                up[sp * blk:(sp + 1) * blk] = bu[k].ravel()
+                lo[sp * blk:(sp + 1) * blk] = bl[k].ravel()
+            uc[idx] = cu
+            lc[idx] = cl
+        t = O.Table(up, lo, uc, lc)
+        made[id(sps)] = t
+        out.append(t)
+    return out
+
+
+def direct_twisted_entry(calc, ra, rb, x1, y1, x2, y2, tau, taup, upper, N1):
+    """Replica entry of the graded two-replica Green function from the
+    oracle's sector matrices (include/cdet/replica_chebyshev_tables.h:489-510):
+    (g0 + g1)/2 on the replica diagonal, +-i (g0 - g1)/2 off it."""
+    g0 = calc.sector_matrix(0, tau, taup, upper)
+    g1 = calc.sector_matrix(1, tau, taup, upper)
+    s1, s2 = y1 * N1 + x1, y2 * N1 + x2
+    a, b = g0[s1, s2], g1[s1, s2]
+    if ra == rb:
+        return ((a + b) / 2).real
+    d = a - b
+    return ((1j * d) / 2).real if (ra, rb) == (0, 1) else ((-1j * d) / 2).real

@@ -466,3 +466,124 @@ def test_green_pair_eval_large_batch_tiles(gpu_available):
         res[mode] = out.cpu().numpy()
     assert np.max(np.abs(res["mfma64"] - res["exact"])) < 1e-13
 
+
+
+def test_plain_entry_points_from_concurrent_threads(gpu_available):
+    # The reference calls the integrand from one std::async thread per chain
+    # (include/cdet/mcmcmc_sampler.h:1104-1110).  Four host threads (ctypes
+    # releases the GIL inside the call) evaluate on one context at once; every
+    # value is the oracle's, bit for bit.
+    import threading
+    name = "c1"
+    s = helpers.SETS[name]
+    op = helpers.params_of(name)
+    ctx = cdet_amd.Context(cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"],
+                                                s["alpha_up"], s["alpha_down"],
+                                                derived=(op.scale, op.diag_up_shift, op.diag_down_shift)))
+    ctx.upload_tables(helpers.tables_of(name))
+    orc = helpers.oracle_of(name)
+    per_thread = [helpers.seeded_configs(9700 + t, [7, 5, 8, 6, 7, 4, 8, 3]) for t in range(4)]
+    want = [[(orc.all_subsets(v), orc.single(v)) for v in cfgs] for cfgs in per_thread]
+    got = [[None] * len(c) for c in per_thread]
+    errors = []
+    start = threading.Barrier(4)
+
+    def work(t):
+        try:
+            start.wait()
+            for rep in range(3):
+                for k, v in enumerate(per_thread[t]):
+                    a = ctx.evaluate_all_subset_integrands(v)
+                    b = ctx.evaluate_integrand(v)
+                    if rep == 2:
+                        got[t][k] = (a, b)
+        except Exception as e:  # reported on the main thread
+            errors.append(e)
+
+    say("plain entry points from 4 threads")
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for t in range(4):
+        for k in range(len(per_thread[t])):
+            assert bits_equal(got[t][k][0], want[t][k][0]), (t, k)
+            assert bits_equal(got[t][k][1], want[t][k][1]), (t, k)
+    assert ctx.counters()["integrand_calls"] == 4 * 3 * 2 * len(per_thread[0])
+    ctx.close()
+
+
+def test_green_pair_scratch_follows_the_context(gpu_available):
+    # The lookup scratch belongs to its context: lookups on a 3x3 context,
+    # that context closed, then a few lookups on a new 8x8 context (four
+    # times the key count) give the oracle's values bit for bit.
+    import ctypes
+    import torch
+    from oracle import oracle as O
+    c1 = helpers.SETS["c1"]
+    op1 = helpers.params_of("c1")
+    ctx = cdet_amd.Context(cdet_amd.make_params(c1["N1"], c1["N2"], c1["nc"], c1["beta"], c1["U"], c1["mu"],
+                                                c1["alpha_up"], c1["alpha_down"],
+                                                derived=(op1.scale, op1.diag_up_shift, op1.diag_down_shift)))
+    ctx.upload_tables(helpers.tables_of("c1"))
+
+    def lookups(cx, qi, qt):
+        out = torch.zeros(2 * len(qi), dtype=torch.float64, device="cuda")
+        d_qi = torch.tensor(qi, device="cuda")
+        d_qt = torch.tensor(qt, device="cuda")
+        rc = cdet_amd.lib().cdet_green_pair_eval(cx.handle, ctypes.c_void_p(d_qi.data_ptr()),
+                                                 ctypes.c_void_p(d_qt.data_ptr()), len(qi),
+                                                 ctypes.c_void_p(out.data_ptr()), cdet_amd.MODE_EXACT, None)
+        assert rc == 0, cdet_amd._err()
+        torch.cuda.synchronize()
+        return out.cpu().numpy().reshape(-1, 2)
+
+    r = np.random.default_rng(8)
+    qi = np.zeros((4096, 7), dtype=np.int32)
+    qi[:, 1:3] = r.integers(0, 2, (4096, 2))
+    qi[:, 3:7] = r.integers(0, 3, (4096, 4))
+    qt = r.uniform(0, c1["beta"], (4096, 2))
+    say("lookups on a 3x3 context")
+    lookups(ctx, qi, qt)
+    ctx.close()
+    s = helpers.SETS["p8x8"]
+    p = cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"], s["alpha_up"], s["alpha_down"])
+    big = cdet_amd.Context(p)
+    say("8x8 context: tables on the GPU, then 64 lookups")
+    big.build_tables(pbc=True, mask=s["mask"], mask_param=s["mask_param"])
+    tabs = [O.Table(*big.download_table(k)) for k in range(4)]
+    orc = O.Oracle(helpers.params_of("p8x8"), tabs)
+    qi2 = np.zeros((64, 7), dtype=np.int32)
+    qi2[:, 0] = r.integers(0, 2, 64)
+    qi2[:, 1:3] = r.integers(0, 2, (64, 2))
+    qi2[qi2[:, 0] == 1, 1:3] = 0
+    qi2[:, 3:7] = r.integers(0, 8, (64, 4))
+    qt2 = r.uniform(0, s["beta"], (64, 2))
+    got = lookups(big, qi2, qt2)
+    want = np.array([orc.green_pair(2 * qi2[k, 0], 2 * qi2[k, 0] + 1, *qi2[k, 1:7], qt2[k, 0], qt2[k, 1])
+                     for k in range(64)])
+    assert bits_equal(got, want)
+    big.close()
+
+
+def test_context_close_closes_its_plans(gpu_available):
+    # A plan must not outlive its context: Context.close() destroys the
+    # context's live plans first, and closing them again is harmless.
+    import torch
+    name = "c1"
+    s = helpers.SETS[name]
+    ctx = cdet_amd.Context(cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"],
+                                                s["alpha_up"], s["alpha_down"]))
+    ctx.upload_tables(helpers.tables_of(name))
+    configs = helpers.seeded_configs(9900, [5, 6, 7])
+    plan = ctx.plan([len(v) for v in configs], "exact")
+    verts = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    say("plan run, then context close with the plan live")
+    plan.run(verts, out, None, use_graph=True, stream=torch.cuda.current_stream().cuda_stream)
+    ctx.close()
+    assert plan._h is None
+    plan.close()
+    torch.cuda.synchronize()

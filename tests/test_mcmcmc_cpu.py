@@ -202,3 +202,51 @@ def test_two_groups_in_flight_take_the_lockstep_steps(tmp_path):
     subs = [x for x in calls if x[0] == "submit"]
     assert {s[1] for s in subs} == {0, 1} and sum(1 for x in calls if x[0] == "wait") == len(subs)
     assert {s[2] for s in subs if s[1] == 0} == {3} and {s[2] for s in subs if s[1] == 1} == {2}
+
+
+def test_table_backend_and_workflow_resolution():
+    # src/main.cpp:322-345, :428-446: names are normalised, aliases mapped,
+    # environment variables override the file
+    base = {"N1": ["3"], "N2": ["3"], "PBC": ["true"], "SUBREGION_MASK": ["first_row"]}
+    wf, be, cd = mcmcmc.table_workflow_from_params(base, environ={})
+    assert (wf, be, cd) == ("resident", "resident", "chunked_green_tables")
+    p = dict(base, LOAD_DATA_WORKFLOW=["Load-Prebuilt"], MCMCMC_TABLE_BACKEND=["LRU"],
+             LOAD_DATA_CHUNK_DIR=["tables"])
+    assert mcmcmc.table_workflow_from_params(p, environ={}) == ("load_chunks", "disk_lru", "tables")
+    for alias in ("on_demand", "disk", "disk-cache", "DISK_LRU"):
+        p = dict(base, MCMCMC_TABLE_BACKEND=[alias])
+        assert mcmcmc.table_workflow_from_params(p, environ={})[1] == "disk_lru"
+    env = {"CDET_MCMCMC_TABLE_BACKEND": "disk_lru", "CDET_LOAD_DATA_WORKFLOW": "load_chunks",
+           "CDET_LOAD_DATA_CHUNK_DIR": "/x"}
+    assert mcmcmc.table_workflow_from_params(base, environ=env) == ("load_chunks", "disk_lru", "/x")
+    with pytest.raises(RuntimeError, match="MCMCMC_TABLE_BACKEND must be resident or disk_lru"):
+        mcmcmc.table_workflow_from_params(dict(base, MCMCMC_TABLE_BACKEND=["mmap"]), environ={})
+
+
+def test_load_data_dispatch_without_gpu():
+    # load_data() (include/cdet/load_data.h:119-178): load_chunks goes to the
+    # chunk loader for both backends (disk_lru maps to HBM residency), the
+    # resident workflow to the table builder; the reference's errors otherwise
+    calls = []
+
+    class Ctx:
+        def load_chunks(self, d):
+            calls.append(("load", d))
+
+        def build_tables(self, **kw):
+            calls.append(("build", kw))
+
+    base = {"N1": ["3"], "N2": ["3"], "PBC": ["true"], "SUBREGION_MASK": ["origin_square"],
+            "SUBREGION_SQUARE_RADIUS": ["1"]}
+    for backend in ("resident", "disk_lru"):
+        p = dict(base, LOAD_DATA_WORKFLOW=["load_chunks"], MCMCMC_TABLE_BACKEND=[backend],
+                 LOAD_DATA_CHUNK_DIR=["d"], MCMCMC_LRU_CACHE_GB=["0.1"])
+        r = mcmcmc.load_data_from_params(Ctx(), p, environ={})
+        assert r == {"workflow": "load_chunks", "backend": backend, "residency": "hbm"}
+    assert calls == [("load", "d"), ("load", "d")]
+    r = mcmcmc.load_data_from_params(Ctx(), base, environ={})
+    assert calls[-1] == ("build", {"pbc": True, "mask": "origin_square", "mask_param": 1})
+    with pytest.raises(RuntimeError, match="requires LOAD_DATA_WORKFLOW=load_chunks"):
+        mcmcmc.load_data_from_params(Ctx(), dict(base, MCMCMC_TABLE_BACKEND=["disk_lru"]), environ={})
+    with pytest.raises(RuntimeError, match="can only run with LOAD_DATA_WORKFLOW=resident or load_chunks"):
+        mcmcmc.load_data_from_params(Ctx(), dict(base, LOAD_DATA_WORKFLOW=["sample"]), environ={})

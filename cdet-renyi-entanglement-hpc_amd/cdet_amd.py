@@ -119,6 +119,7 @@ def _signatures():
         "cdet_tables_download_blocks": (i, [vp, i, P(ctypes.c_int64), i64, dp, dp, dp, dp]),
         "cdet_eval_single": (i, [vp, dp, i, dp]),
         "cdet_eval_all_subsets": (i, [vp, dp, i, dp]),
+        "cdet_eval_all_subsets_noalloc": (i, [vp, vp, i, vp, vp, vp]),
         "cdet_batch_plan_create": (i, [vp, P(ctypes.c_int32), i64, i, i, P(vp)]),
         "cdet_batch_plan_destroy": (i, [vp]),
         "cdet_batch_out_size": (i64, [vp]),

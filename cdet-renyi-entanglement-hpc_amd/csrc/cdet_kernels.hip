@@ -907,6 +907,8 @@ struct cdet_ctx {
     std::mutex host_mu;
     // derived table copies (step order, split FP32), made once per table
     std::mutex derive_mu;
+    // device address of this device's copy of `scratch` (no-allocation entry point)
+    unsigned char* noalloc_scratch = nullptr;
     // host-buffer batch path (driver callback)
     int batch_mode = CDET_MODE_EXACT;
     cdet_batch_plan* host_plan = nullptr;
@@ -1179,6 +1181,38 @@ void query_scratch_free(QueryScratch& q) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (cdet_eval_all_subsets_noalloc):
+// every buffer one configuration of order <= CDET_MAX_ORDER needs beyond its
+// vertices and values.  Each device holds its own copy for the life of the
+// process.  Host code never uses the name: the device address of the current
+// device's copy is looked up once per device (hipGetSymbolAddress) when the
+// first context is created on it and kept per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = size_t(4) << 20;  // 4 MiB; order 14 needs 3.1 MB
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Address of `scratch` on `device`, which must be the current device.
+static int noalloc_scratch_of(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(CDET_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(CDET_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_noalloc_ptr[device];
+    return 0;
+}
 
 static constexpr size_t kMaxLds = 160 * 1024 - 2048;
 static constexpr int kDetGrid = 512;
@@ -1477,6 +1511,8 @@ static int ctx_init(cdet_ctx* ctx) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
+    rc = noalloc_scratch_of(ctx->device, &ctx->noalloc_scratch);
+    if (rc) return rc;
     refresh_scalars(ctx);
     return 0;
 }
@@ -1745,6 +1781,72 @@ int cdet_eval_single(cdet_ctx* ctx, const cdet_vertex* vertices, int n, double* 
     if (n <= 0) return fail(CDET_ERR_ORDER, "The replica integrand is defined only for order >= 1");
     if (n > kMaxOrder) return fail(CDET_ERR_CAPACITY, "order above CDET_MAX_ORDER");
     return plain_eval(ctx, vertices, n, 1, out_value, 1);
+}
+
+// Layout of the no-allocation scratch for one configuration of order n.
+struct NoAllocLayout {
+    size_t gup, gdn, out_ord, single, small, minors, minors_cap, total;
+};
+static NoAllocLayout noalloc_layout(int n) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    NoAllocLayout L{};
+    size_t o = 0;
+    L.gup = o;
+    o += up(sizeof(double) * 2 * static_cast<size_t>(n) * n);
+    L.gdn = o;
+    o += up(sizeof(double) * 2 * static_cast<size_t>(n) * n);
+    L.out_ord = o;
+    o += up(sizeof(double) << n);
+    L.single = o;
+    o += up(sizeof(double) * 4);
+    L.small = o;
+    o += up(sizeof(int32_t) * 8);
+    L.minors = o;
+    L.minors_cap = minors_scratch_bytes(n, 1, 0);
+    o += up(L.minors_cap);
+    L.total = o;
+    return L;
+}
+
+int cdet_eval_all_subsets_noalloc(cdet_ctx* ctx, const cdet_vertex* d_vertices, int envelope_order,
+                                  double* d_out, int32_t* d_status, void* stream) {
+    if (!ctx || !d_vertices || !d_out) return fail(CDET_ERR_ARGUMENT, "null argument");
+    if (envelope_order <= 1)
+        return fail(CDET_ERR_ORDER, "The all-subset replica integrand is defined only for envelope order >= 2");
+    if (envelope_order > kMaxOrder) return fail(CDET_ERR_CAPACITY, "envelope order above CDET_MAX_ORDER");
+    if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
+    const int n = envelope_order;
+    const NoAllocLayout L = noalloc_layout(n);
+    if (L.total > kNoAllocBytes)
+        return fail(CDET_ERR_CAPACITY, "cdet_eval_all_subsets_noalloc: order " + std::to_string(n) + " needs " +
+                                           std::to_string(L.total) + " bytes of scratch, above the " +
+                                           std::to_string(kNoAllocBytes) + "-byte static buffer");
+    unsigned char* base = ctx->noalloc_scratch;
+    if (!base) return fail(CDET_ERR_DEVICE, "no-allocation scratch was not set up");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    double* gup = reinterpret_cast<double*>(base + L.gup);
+    double* gdn = reinterpret_cast<double*>(base + L.gdn);
+    int32_t* small = reinterpret_cast<int32_t*>(base + L.small);
+    int32_t* status = d_status ? d_status : small;
+    // status word and fallback queue start at zero (fill kernels, no memset)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, small, static_cast<int64_t>(8), 0);
+    if (d_status) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_status, static_cast<int64_t>(1), 0);
+    hipLaunchKernelGGL(k_pairs_plain, dim3(2 * n * n), dim3(64), 0, s, ctx->ks,
+                       reinterpret_cast<const double*>(d_vertices), n, gup, gdn);
+    HIP_TRY(hipGetLastError());
+    MinorsOut o{d_out, reinterpret_cast<double*>(base + L.out_ord), reinterpret_cast<double*>(base + L.single),
+                status, small + 2, small + 1};
+    // the minors scratch is a fixed region: its capacity covers this order,
+    // so launch_minors finds it large enough and allocates nothing
+    Scratch region;
+    region.ptr = base + L.minors;
+    region.cap = L.minors_cap;
+    int rc = launch_minors(ctx, region, n, 0, ctx->d_one, 1, ctx->d_zero64, ctx->d_zero64, gup, gdn, o, s);
+    if (rc) return rc;
+    rc = launch_combine(0, 1, static_cast<int64_t>(size_t(1) << n), ctx->d_zero64, o, s);
+    if (rc) return rc;
+    ctx->calls += 1;
+    return 0;
 }
 
 int cdet_get_counters(cdet_ctx* ctx, cdet_counters* out) {

@@ -217,6 +217,26 @@ int cdet_eval_all_subsets(cdet_ctx* ctx, const cdet_vertex* vertices, int envelo
                           double* out_values);
 
 /*
+ * All-subset integrand of one configuration that allocates nothing: the same
+ * kernels and values as cdet_eval_all_subsets (bit for bit), on DEVICE buffers
+ * (d_vertices: cdet_vertex[E]; d_out: double[2^E]; d_status: int32[1] or
+ * NULL, bit 0 = a non-finite value, bit 1 = FPM fallback taken),
+ * asynchronous on `stream` (hipStream_t; NULL = the context stream), with no
+ * host synchronisation.  All other device memory it needs (Green values,
+ * minors, queues: at most 3.1 MB, for E = 14) lies in one 4 MiB file-scope
+ * device array of the library, which each GPU the library runs on holds until
+ * the process ends.  E from 2 to CDET_MAX_ORDER; larger requests are rejected
+ * with CDET_ERR_CAPACITY.  Every call on a GPU shares that GPU's array: calls
+ * that overlap in time on one GPU (different streams, or several contexts)
+ * race and must be serialised by the caller.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).
+ */
+int cdet_eval_all_subsets_noalloc(cdet_ctx* ctx, const cdet_vertex* d_vertices, int envelope_order,
+                                  double* d_out, int32_t* d_status, void* stream);
+
+/*
  * Batched evaluation.  A plan fixes the order of every configuration
  * (h_orders: host int32[count]); configurations are packed in the given order:
  * configuration c owns vertices [sum_{c'<c} n_c', ... + n_c) and, for the

@@ -324,7 +324,8 @@ class BatchPlan:
 
     def __init__(self, ctx, orders, mode="exact", single=False, kind=None):
         """kind: 0 all-subset values, 1 single-order values, 2 conditioning
-        scales of the all-subset values (defaults from `single`)."""
+        scales of the all-subset values, 3 first-order error bounds of the
+        all-subset values (defaults from `single`)."""
         self.ctx = ctx
         self.orders = np.ascontiguousarray(orders, dtype=np.int32)
         self.mode = MODES[mode] if isinstance(mode, str) else int(mode)

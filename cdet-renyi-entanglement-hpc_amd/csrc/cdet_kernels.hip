@@ -34,6 +34,7 @@
 #include "../../include/cdet_chunks.h"
 #include "../../include/cdet_hip.h"
 #include "../../include/cdet_mcmcmc.h"
+#include "cdet_bound.hpp"
 #include "cdet_cdet.hpp"
 #include "cdet_device.hpp"
 #include "cdet_scalars.hpp"
@@ -63,6 +64,7 @@ struct DevTable {
     double* lcorner = nullptr;
     float* split[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [branch][hi, lo]
     double* steps[2] = {nullptr, nullptr};  // [branch] coefficients in matrix-core step order
+    double* abssum[2] = {nullptr, nullptr};  // [branch] sum of |c_ab| per coefficient block
     int alias = -1;  // kind whose storage this kind shares
 };
 
@@ -129,7 +131,8 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
                                               double* __restrict__ g_dn, int32_t* __restrict__ q_key,
                                               int32_t* __restrict__ q_rank, double* __restrict__ q_u,
                                               double* __restrict__ q_v, int32_t* __restrict__ hist,
-                                              int32_t* __restrict__ status) {
+                                              int32_t* __restrict__ status, double* __restrict__ e_up,
+                                              double* __restrict__ e_dn, double err_scale) {
     __shared__ Vtx vs[2][kMaxOrder];
     const int c = blockIdx.x;
     const int n = cfg_order[c];
@@ -160,7 +163,15 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
             g_up[q] = ks.corner[ku][br][d.sitepair];
             g_dn[q] = ks.corner[ku + 1][br][d.sitepair];
             q_key[q] = -1;
+            if (e_up) {  // corner values are read, not summed: no table-value error
+                e_up[q] = 0.0;
+                e_dn[q] = 0.0;
+            }
             continue;
+        }
+        if (e_up) {  // error-bound kind: table-value error scale of the pair's block
+            e_up[q] = err_scale * ks.abssum[ku][br][d.sitepair];
+            e_dn[q] = err_scale * ks.abssum[ku + 1][br][d.sitepair];
         }
         const int32_t key = static_cast<int32_t>((static_cast<int64_t>(cls) * 2 + br) * ks.SP + d.sitepair);
         const int32_t slot = key * kKeySub + sub;
@@ -830,6 +841,21 @@ __global__ void k_step_order(const double* __restrict__ c, int64_t n, int nu, in
 }
 
 // Fill kernel (graph-capture safe replacement of hipMemsetAsync).
+// out[b] = sum over the nu*nv coefficients of block b of |c|: one wave per
+// block (the table-value error scale of the error-bound kind).
+__global__ __launch_bounds__(256) void k_block_abssum(const double* __restrict__ c, int64_t blocks, int per_block,
+                                                      double* __restrict__ out) {
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (b >= blocks) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
+    const double* p = c + b * per_block;
+    double s = 0.0;
+    for (int i = lane; i < per_block; i += 64) s += fabs(p[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) out[b] = s;
+}
+
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1079,6 +1105,8 @@ void refresh_scalars(cdet_ctx* ctx) {
         k.split_lo[kind][0] = t.split[0][1];
         k.split_hi[kind][1] = t.split[1][0];
         k.split_lo[kind][1] = t.split[1][1];
+        k.abssum[kind][0] = t.abssum[0];
+        k.abssum[kind][1] = t.abssum[1];
     }
     for (int cls = 0; cls < 2; ++cls) {
         const int ku = cls == 0 ? 0 : 2;
@@ -1236,6 +1264,8 @@ struct MinorsOut {
     int32_t* fb_list;   // capacity 4 * count entries
     int32_t* fb_count;
     bool paired = false;  // modes 0, 2: k_cdet takes both classes of a configuration
+    const double* err_up = nullptr;  // mode 3: table-value error scale per Green pair
+    const double* err_dn = nullptr;
 };
 
 // Global scratch of one order bucket: the principal minors of every spin
@@ -1244,6 +1274,8 @@ struct MinorsOut {
 // context for the plain form) and only grows; a plan sizes it at creation so
 // that its captured graph never sees a reallocation.
 static size_t minors_scratch_bytes(int n, int count, int mode) {
+    if (mode == 3)  // error-bound kind: one workspace per workgroup of k_bound
+        return sizeof(double) * bound_doubles(n) * std::min<size_t>(2 * static_cast<size_t>(count), kBoundGrid);
     const size_t sc = size_t(1) << n;
     size_t b = sizeof(double) * 4 * static_cast<size_t>(count) * sc;
     if (fpm_gscr(n)) b += FpmLds::bytes(n) * std::min<size_t>(4 * static_cast<size_t>(count), kGscrGrid);
@@ -1259,6 +1291,17 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     int rc = grow(&scratch.ptr, scratch.cap, minors_scratch_bytes(n, count, mode));
     if (rc) return rc;
     const size_t sc = size_t(1) << n;
+    if (mode == 3) {  // error bounds of the all-subset values
+        const uint16_t* masks3 = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
+        const int* popoff3 = ctx->d_popoff + n * (kMaxOrder + 2);
+        const int items = 2 * count;
+        const int grid = std::min(items, kBoundGrid);
+        hipLaunchKernelGGL(k_bound, dim3(grid), dim3(kBoundThreads), 0, stream, ctx->ks, n, items, cfg_list, pbase,
+                           obase, gup, gdn, o.err_up, o.err_dn, o.out, o.out_ord, masks3, popoff3,
+                           static_cast<double*>(scratch.ptr), bound_doubles(n));
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     double* pm = static_cast<double*>(scratch.ptr);
     double* next = pm + 4 * static_cast<size_t>(count) * sc;
     const uint16_t* masks = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
@@ -1325,7 +1368,7 @@ static int launch_combine(int mode, int64_t count, int64_t out_total, const int6
                            count, obase, o.single_tmp, o.out);
     } else if (!o.paired) {  // paired runs combined the classes in k_cdet
         hipLaunchKernelGGL(k_combine_all, dim3(static_cast<unsigned>((out_total + 255) / 256)), dim3(256), 0, stream,
-                           out_total, o.out, o.out_ord, mode == 2 ? 1 : 0);
+                           out_total, o.out, o.out_ord, mode >= 2 ? 1 : 0);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1399,6 +1442,33 @@ static int ensure_split_tables(cdet_ctx* ctx) {
         }
     }
     if (made) {  // only a change of the table pointers re-captures the plans' graphs
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        refresh_scalars(ctx);
+    }
+    return 0;
+}
+
+// Per-block coefficient magnitude sums of every resident table (the
+// error-bound kind), built once per table.
+static int ensure_abssum_tables(cdet_ctx* ctx) {
+    std::lock_guard<std::mutex> lock(ctx->derive_mu);
+    size_t nc, nk;
+    table_sizes(ctx, nc, nk);
+    bool made = false;
+    for (int kind = 0; kind < 4; ++kind) {
+        DevTable& t = ctx->tab[kind];
+        if (t.alias >= 0 || !t.upper) continue;
+        for (int br = 0; br < 2; ++br) {
+            if (t.abssum[br]) continue;
+            HIP_TRY(hipMalloc(&t.abssum[br], nk * sizeof(double)));
+            hipLaunchKernelGGL(k_block_abssum, dim3(static_cast<unsigned>((nk + 3) / 4)), dim3(256), 0, ctx->stream,
+                               br == 0 ? t.upper : t.lower, static_cast<int64_t>(nk),
+                               ctx->p.cheb_nu * ctx->p.cheb_nv, t.abssum[br]);
+            HIP_TRY(hipGetLastError());
+            made = true;
+        }
+    }
+    if (made) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         refresh_scalars(ctx);
     }
@@ -1630,6 +1700,10 @@ static void invalidate_split(DevTable& t) {
             f = nullptr;
         }
     for (double*& d : t.steps) {
+        (void)hipFree(d);
+        d = nullptr;
+    }
+    for (double*& d : t.abssum) {
         (void)hipFree(d);
         d = nullptr;
     }

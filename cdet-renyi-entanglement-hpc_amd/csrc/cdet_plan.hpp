@@ -45,6 +45,8 @@ struct cdet_batch_plan {
     double* d_single = nullptr;
     int32_t* d_fb_list = nullptr;
     int32_t* d_fb_count = nullptr;  // one queue length per order
+    double* d_eup = nullptr;        // error-bound kind: table-value error scale per Green pair
+    double* d_edn = nullptr;
     Scratch scratch;                // minors global scratch of lane 0, sized at creation
     int contract_grid = 0;
     // Order buckets spread over streams: bucket n runs on lane lane_of[n]
@@ -88,6 +90,10 @@ static int plan_mark_done(cdet_batch_plan* p, hipStream_t s) {
 // The derived table copies the plan's contraction reads (split FP32, step
 // order), made before a run when the tables changed since.
 static int plan_tables(cdet_batch_plan* p) {
+    if (p->eval_kind == 3) {
+        int rc = ensure_abssum_tables(p->ctx);
+        if (rc) return rc;
+    }
     if (p->contract_mode == CDET_MODE_SPLIT32) return ensure_split_tables(p->ctx);
     if (p->contract_mode == CDET_MODE_MFMA64) return ensure_step_tables(p->ctx);
     return 0;
@@ -108,7 +114,7 @@ static void plan_free(cdet_batch_plan* p) {
     void* bufs[] = {p->d_order, p->d_vbase, p->d_pbase, p->d_obase, p->d_lists, p->d_gup, p->d_gdn, p->d_qkey,
                     p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
                     p->d_ntiles, p->d_su, p->d_sv, p->d_sdest, p->d_out_ord, p->d_single, p->d_fb_list,
-                    p->d_fb_count, p->d_wgtot, p->scratch.ptr};
+                    p->d_fb_count, p->d_wgtot, p->scratch.ptr, p->d_eup, p->d_edn};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -197,6 +203,8 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
         const int off = p->bucket_off[static_cast<size_t>(n)];
         const int lane = p->lane_of[static_cast<size_t>(n)];
         MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n, p->paired};
+        o.err_up = p->d_eup;
+        o.err_dn = p->d_edn;
         int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off, cnt,
                                p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s);
         if (rc) return rc;
@@ -219,7 +227,8 @@ static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, int32_
     cdet_ctx* ctx = p->ctx;
     hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
                        reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
-                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist, d_status);
+                       p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist, d_status, p->d_eup, p->d_edn,
+                       static_cast<double>(ctx->ks.nu + ctx->ks.nv) * kUnitRoundoff);
     const unsigned kwg = static_cast<unsigned>((p->nkeys + kScanKeys - 1) / kScanKeys);
     hipLaunchKernelGGL(k_key_totals, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot);
     hipLaunchKernelGGL(k_scan_keys, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot,
@@ -300,7 +309,7 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     p->ctx = ctx;
     p->max_lanes = std::max(1, std::min(max_lanes, kMinorLanes));
     p->contract_mode = mode;
-    if (eval_kind < 0 || eval_kind > 2) {
+    if (eval_kind < 0 || eval_kind > 3) {
         delete p;
         return fail(CDET_ERR_ARGUMENT, "unknown evaluation kind");
     }
@@ -371,6 +380,10 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     rc |= dev_alloc(&p->d_single, p->eval_kind == 1 ? 4 * count : 1);
     rc |= dev_alloc(&p->d_fb_list, 4 * count);
     rc |= dev_alloc(&p->d_fb_count, kMaxOrder + 1);
+    if (p->eval_kind == 3) {
+        rc |= dev_alloc(&p->d_eup, p->nq);
+        rc |= dev_alloc(&p->d_edn, p->nq);
+    }
     if (!rc && (hipMemset(p->d_hist, 0, sizeof(int32_t) * static_cast<size_t>(p->nkeys) * kKeySub) != hipSuccess ||
                 hipMemset(p->d_fb_count, 0, sizeof(int32_t) * (kMaxOrder + 1)) != hipSuccess))
         rc = 1;
@@ -395,7 +408,7 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
         int min_cnt = std::numeric_limits<int>::max();
         for (int n = 1; n <= kMaxOrder; ++n)
             if (p->bucket_cnt[static_cast<size_t>(n)]) min_cnt = std::min(min_cnt, p->bucket_cnt[static_cast<size_t>(n)]);
-        p->paired = p->eval_kind != 1 && min_cnt >= kPairMin * ctx->ncu;
+        p->paired = (p->eval_kind == 0 || p->eval_kind == 2) && min_cnt >= kPairMin * ctx->ncu;
     }
     rc = plan_lanes(p, lane_streams);
     if (rc) {

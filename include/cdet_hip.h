@@ -248,7 +248,15 @@ int cdet_eval_all_subsets_noalloc(cdet_ctx* ctx, const cdet_vertex* d_vertices, 
  * |prefactor| * (|a[mask]| + sum of |subtracted terms|) of the subset
  * recursion, summed as twisted + 2*ordinary; a value computed with relative
  * perturbations of size e in its inputs can differ from the exact one by
- * about e times this scale, however small the value itself.
+ * about e times this scale, however small the value itself; 3 = first-order
+ * error bound of every all-subset value (same layout as kind 0): running
+ * error analysis of the reference's algorithm (Schur-complement minors with
+ * shift removal, pivoted fallback, subset recursion with its round-off
+ * filter, assembly) under an absolute error of each table value of at most
+ * (nu + nv) u sum|c_ab| over its coefficient block plus one rounding of
+ * u = 2^-53 per operation.  Two evaluations of a configuration whose table
+ * values satisfy that (any FP64 summation order of the Chebyshev sums) differ
+ * by at most about twice this bound.
  * Batched form of the reference's per-chain calls
  * (include/cdet/mcmcmc_sampler.h:538, :624-625; cdet_integrand_evaluator.h:542-583).
  */

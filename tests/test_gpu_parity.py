@@ -587,3 +587,61 @@ def test_context_close_closes_its_plans(gpu_available):
     assert plan._h is None
     plan.close()
     torch.cuda.synchronize()
+
+
+def perturbed_tables(tables, rel, seed, coherent=False):
+    """Every coefficient c moved by at most rel |c| (random sign per
+    coefficient, or sign(c) for all when coherent), corners kept: each table
+    value moves by at most rel sum|c_ab|, inside the error-bound model."""
+    from oracle import oracle as O
+    r = np.random.default_rng(seed)
+    made = {}
+    out = []
+    for t in tables:
+        if id(t) not in made:
+            arrs = []
+            for a in (t.upper, t.lower):
+                d = np.sign(a) if coherent else r.uniform(-1.0, 1.0, a.shape)
+                arrs.append(a + rel * d * np.abs(a))
+            made[id(t)] = O.Table(arrs[0], arrs[1], t.ucorner, t.lcorner)
+        out.append(made[id(t)])
+    return out
+
+
+@pytest.mark.parametrize("name", ["c1", "c1_neq"])
+def test_error_bound_covers_table_perturbations(gpu_available, name):
+    # Evaluation kind 3 bounds, per value, the change caused by table-value
+    # errors up to (nu + nv) u sum|c_ab| plus the evaluation's own rounding.
+    # Evaluations on tables perturbed inside that model (random and coherent
+    # coefficient moves) and the FP64 matrix-core contraction must stay
+    # within twice the bound on every mask.
+    s = helpers.SETS[name]
+    ctx = gpu_ctx(name)
+    orders = [2, 4, 6, 8, 10, 11]
+    configs = helpers.seeded_configs(5300 + len(name), orders)
+    say(f"error bounds {name}")
+    (bound,), ob, _ = run_batch(ctx, configs, "exact", kind=3)
+    (ex,), _, _ = run_batch(ctx, configs, "exact")
+    (mf,), _, _ = run_batch(ctx, configs, "mfma64")
+    assert np.all(np.isfinite(bound)) and np.all(bound >= 0)
+    u = 2.0 ** -53
+    nonzero = ex != 0
+    assert np.all(bound[nonzero] >= u * np.abs(ex[nonzero]))
+    ratios = {"mfma64": float(np.max(np.abs(mf - ex) / np.maximum(bound, 1e-300)))}
+    kappa = 2 * s["nc"]
+    op = helpers.params_of(name)
+    for label, seed, coherent in (("random", 1, False), ("coherent", 2, True)):
+        pt = perturbed_tables(helpers.tables_of(name), kappa * u, seed, coherent)
+        c2 = cdet_amd.Context(cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"],
+                                                   s["alpha_up"], s["alpha_down"],
+                                                   derived=(op.scale, op.diag_up_shift, op.diag_down_shift)))
+        c2.upload_tables(pt)
+        (pe,), _, _ = run_batch(c2, configs, "exact")
+        c2.close()
+        ratios[label] = float(np.max(np.abs(pe - ex) / np.maximum(bound, 1e-300)))
+    say(f"{name}: max |difference| / bound: {ratios}")
+    for k, r in ratios.items():
+        assert r <= 2.0, (k, r)
+    # the bound is not vacuous: the perturbed evaluations reach a visible
+    # fraction of it somewhere
+    assert max(ratios["random"], ratios["coherent"]) > 1e-4

@@ -337,3 +337,75 @@ def test_c5_tables_against_direct_green_function(gpu_available, c5):
         err = max(err, abs(ex[k, 0] - g))
     say(f"c5: max |table - direct| at {n} lookups = {err:.3e}")
     assert err < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# FP64 matrix-core form against the first-order error bound (evaluation kind
+# 3): at the headline workload (C2 order 10) and the largest (C5, orders up to
+# 14) every value of the matrix-core form is within twice the bound of the
+# bit-identical FP64 form, and so is the FP64 form on tables moved by one ulp.
+# ---------------------------------------------------------------------------
+def bound_report(got, ref, bound):
+    """(max |got - ref| / bound, max relative error over the values the bound
+    resolves to 1e-9 (|ref| >= 2e9 bound), fraction of nonzero values so
+    resolved).  At half filling the odd-order connected values vanish
+    identically: they sit at the round-off floor, far below their bound, and
+    no relative error is defined for them."""
+    ratio = np.abs(got - ref) / np.maximum(bound, 1e-300)
+    nz = ref != 0
+    res = nz & (np.abs(ref) >= 2e9 * bound)
+    rel = np.abs(got - ref)[res] / np.abs(ref)[res]
+    return (float(np.max(ratio)), float(np.max(rel)) if rel.size else 0.0,
+            float(np.count_nonzero(res) / max(1, np.count_nonzero(nz))))
+
+
+def test_c2_mfma64_within_error_bound(gpu_available, c2):
+    orders, verts = c2["orders"], c2["verts"]
+    idx = list(np.nonzero(orders == 10)[0])
+    configs = [verts[i] for i in idx]
+    say(f"c2: {len(configs)} order-10 configurations, exact / mfma64 / bound")
+    ex, _, _ = run_plan(c2["ctx"], configs, "exact")
+    mf, _, _ = run_plan(c2["ctx"], configs, "mfma64")
+    bd, _, _ = run_plan(c2["ctx"], configs, "exact", kind=3)
+    assert np.all(np.isfinite(bd))
+    r_mf, rel_mf, frac_mf = bound_report(mf, ex, bd)
+    # one-ulp control: the bit-identical form on tables with every coefficient
+    # moved by one ulp in a seeded random direction
+    r = np.random.default_rng(7)
+    made = {}
+    pt = []
+    for t in c2["tables"]:
+        if id(t) not in made:
+            made[id(t)] = O.Table(*[np.nextafter(a, np.where(r.random(a.shape) < 0.5, -np.inf, np.inf))
+                                    for a in (t.upper, t.lower)], t.ucorner, t.lcorner)
+        pt.append(made[id(t)])
+    w = c2["w"]
+    ctx_ulp = gpu_context(w, c2["op"])
+    ctx_ulp.upload_tables(pt)
+    ul, _, _ = run_plan(ctx_ulp, configs, "exact")
+    ctx_ulp.close()
+    r_ul, rel_ul, frac_ul = bound_report(ul, ex, bd)
+    say(f"c2 order 10: mfma64 max err/bound {r_mf:.3e}, max rel on resolved values {rel_mf:.3e} "
+        f"(resolved fraction {frac_mf:.4f}); one-ulp control max err/bound {r_ul:.3e}, max rel {rel_ul:.3e}")
+    assert r_mf <= 2.0
+    assert r_ul <= 2.0
+    assert rel_mf <= 1e-9 and frac_mf > 0.4
+
+
+def test_c5_mfma64_within_error_bound(gpu_available, c5):
+    ctx, orders, verts = c5["ctx"], c5["orders"], c5["verts"]
+    idx = pick_per_order(orders, 1, extra={12: 2, 13: 3, 14: 3})
+    configs = [verts[i] for i in idx]
+    say(f"c5: {len(configs)} configurations (orders {sorted(set(int(orders[i]) for i in idx))}), "
+        "exact / mfma64 / bound")
+    ex, ob, _ = run_plan(ctx, configs, "exact")
+    mf, _, _ = run_plan(ctx, configs, "mfma64")
+    bd, _, _ = run_plan(ctx, configs, "exact", kind=3)
+    assert np.all(np.isfinite(bd))
+    for k, i in enumerate(idx):
+        s = slice(ob[k], ob[k + 1])
+        r_mf, rel_mf, frac = bound_report(mf[s], ex[s], bd[s])
+        say(f"c5 order {orders[i]}: mfma64 max err/bound {r_mf:.3e}, max rel on resolved values {rel_mf:.3e} "
+            f"(resolved fraction {frac:.4f})")
+        assert r_mf <= 2.0, (i, orders[i])
+        assert rel_mf <= 1e-9, (i, orders[i])

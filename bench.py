@@ -19,9 +19,12 @@ Printed JSON line fields beyond the driver contract:
   plain_fp64    one-configuration-per-launch FP64 GPU form, evals/s
   per_order     evals/s of every order bucket (1..10) of the seeded set, as
                 all-subset and as single-order integrands
-  errors        against the bit-identical FP64 form (error_metrics), beside
-                the same figures for that FP64 form on tables moved by one
-                ulp (the spread any FP64 evaluation order shows)
+  exact_fp64_batched  the bit-identical FP64 form batched the same way
+  errors        against the bit-identical FP64 form, per order bucket of the
+                seeded set (errors_per_order; order 10 in errors_vs_fp64_exact),
+                measured against each value's first-order error bound
+                (evaluation kind 3), beside the same figures for the FP64 form
+                on tables moved by one ulp (bound_metrics)
 """
 import argparse
 import json
@@ -73,33 +76,113 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+def traffic_from_profiles(kernel, workload="c2"):
+    """HBM bytes per launch of a kernel at a workload, from the committed
+    rocprofv3 PMC summaries (profiles/traffic.json), if measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
-    except (OSError, ValueError):
+            return json.load(f).get(workload, {}).get(kernel)
+    except (OSError, ValueError, AttributeError):
         return None
 
 
-def error_metrics(got, ref, cond):
-    """Errors of `got` against the bit-identical FP64 values `ref` over the
-    masks >= 1e-6 of their configuration's largest |value| (smaller values sit
-    at the round-off floor of the principal minors): max relative error, the
-    same on well-conditioned masks (sum of |terms| <= 1e4 |value|), and max
-    |error| / sum of |terms|."""
-    scale = np.maximum(np.abs(ref), 1e-300)
-    rel = np.abs(got - ref) / scale
+def bound_metrics(got, ref, bound):
+    """Errors of `got` against the bit-identical FP64 values `ref`, measured
+    against the first-order error bound of every value (evaluation kind 3):
+    max |got - ref| / bound over all masks; the values the bound resolves to
+    1e-9 (|ref| >= 2e9 bound) and the max relative error over them; the max
+    relative error over masks >= 1e-6 of their configuration's largest |value|
+    (values below sit at the round-off floor: at half filling the odd-order
+    connected values vanish identically)."""
+    got = got.reshape(ref.shape)
+    err = np.abs(got - ref)
+    nz = ref != 0
+    res = nz & (np.abs(ref) >= 2e9 * bound)
     big = np.abs(ref) >= 1e-6 * np.abs(ref).max(axis=1, keepdims=True)
-    use = big & (cond > 0)
-    well = use & (cond <= 1e4 * np.abs(ref))
+    big &= nz
     return {
-        "max_rel_err": float(np.max(rel[big])) if np.any(big) else 0.0,
-        "max_rel_err_well_conditioned": float(np.max(rel[well])) if np.any(well) else 0.0,
-        "well_conditioned_fraction": round(float(np.count_nonzero(well) / max(1, np.count_nonzero(use))), 4),
-        "max_err_over_conditioning": float(np.max(np.abs(got - ref)[use] / cond[use])) if np.any(use) else 0.0,
+        "max_err_over_bound": float(np.max(err / np.maximum(bound, 1e-300))),
+        "rel_le_1e-9_fraction_above_1e-6_of_max": round(float(np.mean(err[big] <= 1e-9 * np.abs(ref[big]))), 4)
+        if np.any(big) else 1.0,
+        "resolved_fraction": round(float(np.count_nonzero(res) / max(1, np.count_nonzero(nz))), 4),
+        "max_rel_err_resolved": float(np.max(err[res] / np.abs(ref[res]))) if np.any(res) else 0.0,
+        "max_rel_err_above_1e-6_of_max": float(np.max(err[big] / np.abs(ref[big]))) if np.any(big) else 0.0,
     }
+
+
+def run_values(ctx, d_verts, orders, mode, kind, dev, sptr):
+    """One eager run of a plan; values as (configs, 2^n) for one order."""
+    import torch
+    plan = ctx.plan(orders, mode=mode, kind=kind)
+    out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
+    plan.run(d_verts, out, None, use_graph=False, stream=sptr)
+    torch.cuda.synchronize()
+    plan.close()
+    return out.cpu().numpy().reshape(len(orders), -1)
+
+
+def perturbed_context(ctx, params, kinds=(0, 2)):
+    """A second context holding the tables with every coefficient moved by one
+    ulp in a seeded random direction (equal spin shifts: kinds 1, 3 alias)."""
+    import cdet_amd
+    r = np.random.default_rng(7)
+    pert = []
+    for kind in kinds:
+        arrs = ctx.download_table(kind)
+        out = []
+        for a in arrs[:2]:
+            out.append(np.nextafter(a, np.where(r.random(a.shape) < 0.5, -np.inf, np.inf)))
+        pert.append(out + list(arrs[2:]))
+
+    class T:
+        def __init__(self, a):
+            self.upper, self.lower, self.ucorner, self.lcorner = a
+    t0, t2 = T(pert[0]), T(pert[1])
+    c2 = cdet_amd.Context(params)
+    c2.upload_tables([t0, t0, t2, t2])
+    return c2
+
+
+def errors_per_order(ctx, ctx_ulp, orders, verts, mode, dev, sptr, order_list):
+    """Per order bucket of the seeded set: `mode` against the bit-identical
+    FP64 form, and the FP64 form on one-ulp-moved tables (ctx_ulp) against it,
+    both measured against the first-order error bound."""
+    import torch
+    out = {}
+    for n in order_list:
+        idx = np.nonzero(orders == n)[0]
+        if len(idx) == 0 or n < 2:
+            continue
+        ords = [n] * len(idx)
+        d_v = torch.tensor(np.concatenate([verts[i] for i in idx]), dtype=torch.float64, device=dev)
+        ref = run_values(ctx, d_v, ords, "exact", 0, dev, sptr)
+        got = run_values(ctx, d_v, ords, mode, 0, dev, sptr)
+        bound = run_values(ctx, d_v, ords, "exact", 3, dev, sptr)
+        e = {"configs": int(len(idx)), mode: bound_metrics(got, ref, bound)}
+        if ctx_ulp is not None:
+            ul = run_values(ctx_ulp, d_v, ords, "exact", 0, dev, sptr)
+            e["fp64_one_ulp_tables"] = bound_metrics(ul, ref, bound)
+        out[str(n)] = e
+    return out
+
+
+def cpu_info():
+    """(usable host cores, CPU model name)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
 
 
 def per_order_rates(ctx, orders, verts, mode, dev, sptr, reps=20):
@@ -128,37 +211,6 @@ def per_order_rates(ctx, orders, verts, mode, dev, sptr, reps=20):
     return out
 
 
-def ulp_reference(ctx, params, d_verts, count, order, ref, cond, dev, sptr):
-    """The same error metrics for the bit-identical FP64 path evaluated on the
-    tables with every coefficient moved by one ulp (seeded random direction):
-    the spread any FP64 evaluation order of these tables can show."""
-    import torch
-    import cdet_amd
-    r = np.random.default_rng(7)
-    pert = []
-    for kind in (0, 2):
-        arrs = ctx.download_table(kind)
-        out = []
-        for a in arrs:
-            d = np.where(r.random(a.shape) < 0.5, -np.inf, np.inf)
-            out.append(np.nextafter(a, d))
-        pert.append(out)
-    class T:
-        def __init__(self, a):
-            self.upper, self.lower, self.ucorner, self.lcorner = a
-    t0, t2 = T(pert[0]), T(pert[1])
-    c2 = cdet_amd.Context(params)
-    c2.upload_tables([t0, t0, t2, t2])
-    plan = c2.plan([order] * count, mode="exact")
-    out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
-    plan.run(d_verts, out, None, use_graph=False, stream=sptr)
-    torch.cuda.synchronize()
-    got = out.cpu().numpy().reshape(count, -1)
-    plan.close()
-    c2.close()
-    return error_metrics(got, ref, cond)
-
-
 def cpu_baseline(tables, params_gpu, verts, seconds):
     """Oracle evaluations on all host cores, one configuration per thread at a
     time (the reference runs one chain per std::async thread)."""
@@ -167,11 +219,7 @@ def cpu_baseline(tables, params_gpu, verts, seconds):
     p = O.make_params(w["N1"], w["N2"], w["nc"], w["beta"], w["U"], w["mu"], w["alpha"], w["alpha"])
     p.scale, p.diag_up_shift, p.diag_down_shift = (params_gpu.scale, params_gpu.diag_up_shift,
                                                    params_gpu.diag_down_shift)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 64))
+    cores, model = cpu_info()
     counts = [0] * cores
     stop = time.time() + seconds
 
@@ -191,8 +239,99 @@ def cpu_baseline(tables, params_gpu, verts, seconds):
         t.join()
     dt = time.time() - t0
     n = sum(counts)
-    return {"value": n / dt, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"{n} order-10 all-subset evaluations over {dt:.1f} s wall on {cores} threads"}
+    return {"value": n / dt, "unit": "evals/s", "cores": cores, "kind": "port", "cpu_model": model,
+            "sample": f"{n} order-10 all-subset evaluations over {dt:.1f} s wall on {cores} threads "
+                      f"(one configuration per thread at a time, as the reference runs one chain per thread)"}
+
+
+def unit_index(coordinate, extent):
+    """sampled_coordinate_to_index (include/cdet/g_funcs.h:315-342)."""
+    return min(max(int(np.floor(extent * coordinate)), 0), extent - 1)
+
+
+def contraction_blocks(configs, N1, N2, beta, branch_eps=1e-13):
+    """Distinct (class, branch, sitepair) coefficient blocks the table
+    contraction of a batch reads (corner lookups read none), by the lookup
+    rules of include/cdet/g_funcs.h:609-653 and
+    include/cdet/integrand_replica_randomly_sampled.h:729."""
+    keys = set()
+    for v in configs:
+        n = len(v)
+        for cls, rc in ((0, 2), (1, 1)):
+            dec = [(unit_index(q[1], N1), unit_index(q[2], N2), unit_index(q[3], rc) if rc == 2 else 0,
+                    beta * q[0], q[0]) for q in v]
+            for i in range(n):
+                xi, yi, ri, ti, ui = dec[i]
+                for j in range(n):
+                    xj, yj, rj, tj, uj = dec[j]
+                    upper = i != j and ui > uj
+                    if (ti if upper else tj) < branch_eps:
+                        continue
+                    sp = ((((ri * 2 + rj) * N1 + xi) * N2 + yi) * N1 + xj) * N2 + yj
+                    keys.add((cls, 0 if upper else 1, sp))
+    return len(keys)
+
+
+def block_view_tables(ctx, configs):
+    """Oracle tables (kinds 0..3; equal spin shifts) holding the device
+    coefficient blocks these configurations read, zeros elsewhere (np.zeros
+    maps untouched pages lazily, so host memory holds only those blocks)."""
+    from oracle import oracle as O
+    p = ctx.params
+    S = p.N1 * p.N2
+    SP = 4 * S * S
+    blk = p.cheb_nu * p.cheb_nv
+    sets = (set(), set())
+    for v in configs:
+        dec = [(unit_index(q[1], p.N1), unit_index(q[2], p.N2), unit_index(q[3], 2)) for q in v]
+        for (x1, y1, r1) in dec:
+            for (x2, y2, r2) in dec:
+                sets[0].add(((((r1 * 2 + r2) * p.N1 + x1) * p.N2 + y1) * p.N1 + x2) * p.N2 + y2)
+                sets[1].add((((x1 * p.N2 + y1) * p.N1 + x2) * p.N2) + y2)
+    tabs = []
+    for kind, sps in ((0, sets[0]), (2, sets[1])):
+        up, lo, uc, lc = np.zeros(SP * blk), np.zeros(SP * blk), np.zeros(SP), np.zeros(SP)
+        idx = np.array(sorted(sps), dtype=np.int64)
+        bu, bl, cu, cl = ctx.download_blocks(kind, idx)
+        for k, sp in enumerate(idx):
+            up[sp * blk:(sp + 1) * blk] = bu[k].ravel()
+            lo[sp * blk:(sp + 1) * blk] = bl[k].ravel()
+        uc[idx] = cu
+        lc[idx] = cl
+        tabs.append(O.Table(up, lo, uc, lc))
+    return [tabs[0], tabs[0], tabs[1], tabs[1]]
+
+
+def cpu_baseline_configs(w, params_gpu, tables, configs, seconds, per_unit=1, unit="evals/s", what=""):
+    """Oracle evaluations of `configs` on all host cores, thread k taking
+    configurations k, k + cores, ... in turn (the reference runs one chain per
+    std::async thread); the rate is evaluations / per_unit per second."""
+    from oracle import oracle as O
+    p = O.make_params(w["N1"], w["N2"], w["nc"], w["beta"], w["U"], w["mu"], w["alpha"], w["alpha"])
+    p.scale, p.diag_up_shift, p.diag_down_shift = (params_gpu.scale, params_gpu.diag_up_shift,
+                                                   params_gpu.diag_down_shift)
+    cores, model = cpu_info()
+    counts = [0] * cores
+    stop = time.time() + seconds
+
+    def worker(k):
+        orc = O.Oracle(p, tables)
+        i = k
+        while time.time() < stop:
+            orc.all_subsets(configs[i % len(configs)])
+            counts[k] += 1
+            i += cores
+
+    t0 = time.time()
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.time() - t0
+    n = sum(counts)
+    return {"value": n / per_unit / dt, "unit": unit, "cores": cores, "kind": "port", "cpu_model": model,
+            "sample": f"{n} all-subset evaluations of {what} over {dt:.1f} s wall on {cores} threads"}
 
 
 C3 = dict(N1=8, N2=8, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="origin_square", radius=2,
@@ -367,9 +506,40 @@ def main_c4(args):
         plan.run(d_v, d_o, None, use_graph=True, stream=sptr)
     torch.cuda.synchronize()
     batch_ms = (time.perf_counter() - t) / 20 * 1e3
+    stage = np.zeros(3)
+    for _ in range(5):
+        stage += np.array(plan.profile(d_v, d_o, None, stream=sptr))
+    stage /= 5
+    lookups = plan.query_count
     plan.close()
     print(f"c4: GPU batch of {w['chains']} envelopes (orders 2..{E}) {batch_ms:.3f} ms", file=sys.stderr,
           flush=True)
+    # the contraction of one batch: its lookups spread over many coefficient
+    # blocks (about 1.5 lookups per 131 KB block), so it is bound by the block
+    # reads: algorithmic bytes = distinct blocks x 8 Nc^2, against HBM
+    flops = lookups * (2.0 * w["nc"] * w["nc"] + 2.0 * w["nc"])
+    blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"])
+    alg_bytes = blocks * 8.0 * w["nc"] * w["nc"]
+    achieved = alg_bytes / (stage[1] * 1e-3) / 1e9
+    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true>"}[mode]
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, "c4"),
+                "kernel": kname, "algorithmic_bytes_per_launch": alg_bytes, "distinct_blocks": blocks,
+                "lookups_per_launch": int(lookups), "fp64_tflops": round(flops / (stage[1] * 1e-3) / 1e12, 3),
+                "kernel_ms": round(stage[1], 4),
+                "stage_ms": {"plan_group": round(stage[0], 4), "contract": round(stage[1], 4),
+                             "fpm_cdet": round(stage[2], 4)}}
+    # CPU: the oracle evaluating every chain's envelope (orders max(3, E_c),
+    # as the sweeps run them) on all host cores; sweeps/s = envelope
+    # evaluations / 256 per second (host heat-bath steps not counted)
+    cpu = None
+    if not args.no_cpu:
+        env = np.maximum(orders, 3)
+        longest = verts[int(np.argmax(orders))]
+        cfgs = [verts[c] if len(verts[c]) >= env[c] else longest[:env[c]] for c in range(len(env))]
+        tabs = block_view_tables(ctx, cfgs)
+        cpu = cpu_baseline_configs(w, params, tabs, cfgs, args.cpu_seconds, per_unit=len(cfgs), unit="sweeps/s",
+                                   what=f"the {len(cfgs)} chain envelopes (orders max(3, E_c))")
     dt, ratios, warm = run_sweeps(ctx, w, orders, args.sweeps, E - 1, args.sweep_groups)
     sweeps = args.sweeps
     line = {
@@ -384,6 +554,8 @@ def main_c4(args):
                    "sweep_groups": args.sweep_groups,
                    "envelopes": {str(n): int(np.count_nonzero(orders == n)) for n in range(2, E + 1)},
                    "mode": mode, "table_build_s": round(build_s, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "envelope_evals_per_s": round(sweeps * w["chains"] / dt, 1),
         "gpu_batch_ms": round(batch_ms, 4),
         # odd orders vanish at half filling; order 4 is the first nonzero ratio
@@ -441,19 +613,44 @@ def main_c5(args):
         outs[mode] = d_o.cpu().numpy()
         print(f"c5: {mode} {ms:.3f} ms per batch, stages {stage}", file=sys.stderr, flush=True)
         plan.close()
-    # FP64 matrix-core values against the bit-identical form, per order
+    # FP64 matrix-core values against the bit-identical form, per order,
+    # measured against each value's first-order error bound (kind 3)
     _, ob = cdet_amd.packed_offsets(orders)
+    print("c5: error bounds", file=sys.stderr, flush=True)
+    bplan = ctx.plan(orders, mode="exact", kind=3)
+    d_b = torch.empty(bplan.out_size, dtype=torch.float64, device=dev)
+    bplan.run(d_v, d_b, None, use_graph=False, stream=sptr)
+    torch.cuda.synchronize()
+    bplan.close()
+    bnd = d_b.cpu().numpy()
     per_order = {}
     for n in range(w["order_min"], w["order_max"] + 1):
-        rel = []
-        for i in np.nonzero(orders == n)[0]:
-            ref = outs["exact"][ob[i]:ob[i + 1]]
-            got = outs["mfma64"][ob[i]:ob[i + 1]]
-            big = np.abs(ref) >= 1e-6 * np.abs(ref).max()
-            if np.any(big):
-                rel.append(float(np.max(np.abs(got - ref)[big] / np.abs(ref)[big])))
-        if rel:
-            per_order[str(n)] = {"configs": len(rel), "max_rel_err": max(rel)}
+        idx = np.nonzero(orders == n)[0]
+        if len(idx) == 0:
+            continue
+        sl = np.concatenate([np.arange(ob[i], ob[i + 1]) for i in idx])
+        per_order[str(n)] = {"configs": int(len(idx)),
+                             "mfma64": bound_metrics(outs["mfma64"][sl].reshape(len(idx), -1),
+                                                     outs["exact"][sl].reshape(len(idx), -1),
+                                                     bnd[sl].reshape(len(idx), -1))}
+    # contraction roofline: HBM-bound at this size (each lookup reads a
+    # 295 KB coefficient block); algorithmic bytes = distinct blocks x 8 Nc^2
+    blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"])
+    alg_bytes = blocks * 8.0 * w["nc"] * w["nc"]
+    cms = res["mfma64"]["stage_ms"][1]
+    achieved = alg_bytes / (cms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles("cdet::k_contract_f64<true>", "c5"),
+                "kernel": "cdet::k_contract_f64<true>", "algorithmic_bytes_per_launch": alg_bytes,
+                "distinct_blocks": blocks, "kernel_ms": round(cms, 4)}
+    cpu = None
+    if not args.no_cpu:
+        sample = [verts[i] for i in range(0, len(verts), 16)]
+        tabs = block_view_tables(ctx, sample)
+        cpu = cpu_baseline_configs(w, params, tabs, sample, args.cpu_seconds,
+                                   what=f"every 16th of the 1024 configurations ({len(sample)}, orders "
+                                        f"{min(len(v) for v in sample)}..{max(len(v) for v in sample)})")
     # lockstep sweeps of 1024 chains at envelopes E_c (orders 1..E_c-1)
     ctx_mode = "mfma64"
     cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[ctx_mode])
@@ -474,12 +671,14 @@ def main_c5(args):
         "config": {"workload": "10x10 Hubbard, beta=10, x<5 half subsystem, 192x192 Chebyshev; 1024 seeded "
                                "configurations, orders 2 + floor(13 u^2), all-subset integrands",
                    "table_build_s": round(build_s, 2), "node_chunk": 7 * w["nc"] + 3},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "exact_fp64_evals_per_s": round(w["configs"] / (res["exact"]["ms"] * 1e-3), 1),
         "stages_ms": {m: res[m]["stage_ms"] for m in res},
         "lookups_per_batch": res["mfma64"]["lookups"],
         "contraction_block_read_upper_bound_gbs": round(res["mfma64"]["lookups"] * block / (cms * 1e-3) / 1e9, 1),
         "orders": {str(n): int(np.count_nonzero(orders == n)) for n in range(w["order_min"], w["order_max"] + 1)},
-        "mfma64_vs_exact": per_order,
+        "errors_per_order": per_order,
         "sweeps_per_s": round(args.c5_sweeps / sweep_s, 2) if sweep_s else None, "sweeps": args.c5_sweeps,
         "sweep_groups": args.sweep_groups,
         "sweep_ratio_order4_over_order2": (float(sweep_ratios[4]) if sweep_ratios is not None
@@ -532,16 +731,6 @@ def main():
     d_out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
     d_status = torch.zeros(count, dtype=torch.int32, device=dev)
 
-    # exact reference values on the GPU (bit-identical FP64 form) and the
-    # conditioning scale of every value (sum of |terms| the value cancels
-    # from) for the error report
-    plan_exact = ctx.plan([w["order"]] * count, mode="exact")
-    d_ref = torch.empty(plan_exact.out_size, dtype=torch.float64, device=dev)
-    plan_exact.run(d_verts, d_ref, None, use_graph=False, stream=sptr)
-    plan_cond = ctx.plan([w["order"]] * count, mode="exact", kind=2)
-    d_cond = torch.empty(plan_cond.out_size, dtype=torch.float64, device=dev)
-    plan_cond.run(d_verts, d_cond, None, use_graph=False, stream=sptr)
-
     # Timed steps: each step evaluates the whole batch.  With --inflight k > 1
     # the steps go round-robin to k plans (own buffers) on k streams, so k
     # batches are in flight at once, as a driver feeding the evaluator with
@@ -553,15 +742,16 @@ def main():
     stats = [d_status] + [torch.zeros(count, dtype=torch.int32, device=dev) for _ in range(inflight - 1)]
     streams = [stream] + [torch.cuda.Stream() for _ in range(inflight - 1)]
 
-    def timed(k_inflight):
+    def timed(k_inflight, plan_set=None):
+        ps = plans if plan_set is None else plan_set
         for i in range(args.warmup):
             k = i % k_inflight
-            plans[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k].cuda_stream)
+            ps[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k].cuda_stream)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for i in range(args.steps):
             k = i % k_inflight
-            plans[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k].cuda_stream)
+            ps[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k].cuda_stream)
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
@@ -569,15 +759,17 @@ def main():
     dt = timed(inflight)
     ms_per_step = 1e3 * dt / args.steps
     value = count * args.steps / dt
+    got = d_out.cpu().numpy().reshape(count, -1)
+    st = d_status.cpu().numpy()
+    # the bit-identical FP64 form, batched the same way
+    exact_value = None
+    if args.mode != "exact":
+        eplans = [ctx.plan([w["order"]] * count, mode="exact") for _ in range(inflight)]
+        exact_value = count * args.steps / timed(inflight, eplans)
+        for ep in eplans:
+            ep.close()
     for extra in plans[1:]:
         extra.close()
-
-    # accuracy of this mode against the FP64 exact form
-    got = d_out.cpu().numpy().reshape(count, -1)
-    ref = d_ref.cpu().numpy().reshape(count, -1)
-    cond = d_cond.cpu().numpy().reshape(count, -1)
-    errors = error_metrics(got, ref, cond)
-    st = d_status.cpu().numpy()
     status_bad = int(np.count_nonzero(st & 1))
     fallbacks = int(np.count_nonzero(st & 2))
 
@@ -600,7 +792,17 @@ def main():
                 "stage_ms": {"plan_group": round(stage[0], 4), "contract": round(stage[1], 4),
                              "fpm_cdet": round(stage[2], 4)}}
 
-    ulp = None if args.no_ulp else ulp_reference(ctx, params, d_verts, count, w["order"], ref, cond, dev, sptr)
+    # accuracy: this mode and the FP64 form on one-ulp-moved tables against the
+    # bit-identical FP64 form, per order bucket of the seeded set, measured
+    # against each value's first-order error bound (evaluation kind 3)
+    ctx_ulp = None if args.no_ulp else perturbed_context(ctx, params)
+    per_order_err = errors_per_order(ctx, ctx_ulp, orders, verts, args.mode, dev, sptr,
+                                     range(2, w["order"] + 1) if not args.no_per_order else [w["order"]])
+    if ctx_ulp is not None:
+        ctx_ulp.close()
+    ref = run_values(ctx, d_verts, [w["order"]] * count, "exact", 0, dev, sptr)
+    bound = run_values(ctx, d_verts, [w["order"]] * count, "exact", 3, dev, sptr)
+    errors = bound_metrics(got, ref, bound)
     per_order = None if args.no_per_order else per_order_rates(ctx, orders, verts, args.mode, dev, sptr)
 
     # plain one-configuration-per-launch FP64 form
@@ -636,10 +838,12 @@ def main():
         "batches_in_flight": inflight,
         "single_stream_evals_per_s": round(count * args.steps / single_dt, 1) if single_dt else round(value, 1),
         "plain_fp64_evals_per_s": round(plain, 1),
+        "exact_fp64_batched_evals_per_s": round(exact_value, 1) if exact_value else round(value, 1),
         "per_order_evals_per_s": per_order,
         "speedup_vs_plain_fp64": round(value / plain, 2),
+        "exact_fp64_batched_speedup_vs_plain": round((exact_value or value) / plain, 2),
         "errors_vs_fp64_exact": errors,
-        "errors_fp64_one_ulp_tables": ulp,
+        "errors_per_order": per_order_err,
         "nonfinite_configs": status_bad,
         "fpm_fallback_configs": fallbacks,
     }
@@ -650,8 +854,6 @@ def main():
         torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     plan.close()
-    plan_exact.close()
-    plan_cond.close()
     ctx.close()
 
 

@@ -36,6 +36,7 @@
 #include "../../include/cdet_mcmcmc.h"
 #include "cdet_bound.hpp"
 #include "cdet_cdet.hpp"
+#include "cdet_fpm_wave.hpp"
 #include "cdet_device.hpp"
 #include "cdet_scalars.hpp"
 #include "cdet_mfma.hpp"
@@ -739,10 +740,31 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
         }
         const double* pu = pm + static_cast<size_t>(4 * li + 2 * cls) * sc;
         const double* pd = same ? pu : pu + sc;
-        // a[mask] = pm_up * pm_down with a[0] = 0
-        for (uint32_t m = tid; m < sc; m += nt) {
-            a[m] = (m == 0) ? 0.0 : pu[m] * pd[m];
-            lmask[m] = masks_by_pop[m];
+        // a[mask] = pm_up * pm_down with a[0] = 0; the global loads of up to
+        // four masks per thread are issued before any product is stored
+        {
+            constexpr int kPre = 4;
+            double vu[kPre], vd[kPre];
+            uint16_t mk[kPre];
+            for (uint32_t m0 = tid; m0 < sc; m0 += kPre * nt) {
+#pragma unroll
+                for (int q = 0; q < kPre; ++q) {
+                    const uint32_t m = m0 + q * nt;
+                    if (m < sc) {
+                        vu[q] = pu[m];
+                        vd[q] = pd[m];
+                        mk[q] = masks_by_pop[m];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kPre; ++q) {
+                    const uint32_t m = m0 + q * nt;
+                    if (m < sc) {
+                        a[m] = (m == 0) ? 0.0 : vu[q] * vd[q];
+                        lmask[m] = mk[q];
+                    }
+                }
+            }
         }
         __syncthreads();
         if (mode == 0 || mode == 2) {
@@ -1308,7 +1330,28 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     const int* popoff = ctx->d_popoff + n * (kMaxOrder + 2);
     if (n >= 2) {
         const int items = (ctx->ks.same_spin[0] && ctx->ks.same_spin[1] ? 2 : 4) * count;
-        if (fpm_gscr(n)) {
+        if (n <= kWaveFpmMaxOrder) {
+            // one wavefront per spin matrix, no workgroup barriers
+            const size_t lds = sizeof(double) * static_cast<size_t>(fpm_wave_lds_doubles(n));
+            switch (n - fpm_wave_L(n)) {
+                case 1:
+                    hipLaunchKernelGGL(k_fpm_wave<1>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL(k_fpm_wave<2>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                    break;
+                case 3:
+                    hipLaunchKernelGGL(k_fpm_wave<3>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                    break;
+                default:
+                    hipLaunchKernelGGL(k_fpm_wave<4>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                    break;
+            }
+        } else if (fpm_gscr(n)) {
             const int grid = std::min(items, kGscrGrid);
             if (n >= kWideOrder)
                 hipLaunchKernelGGL((k_fpm<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, items, cfg_list,

@@ -40,12 +40,12 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+from workloads import C2 as WORKLOAD, C3, C4, C5, c3_queries  # noqa: E402,F401
+
 METRIC = "CDet integrand evals/s at order 10, batched split-FP32 vs plain FP64, ≤1e-9 rel"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
-WORKLOAD = dict(N1=4, N2=4, beta=6.0, U=2.0, mu=1.0, alpha=1.0, nc=96, mask="origin_square", radius=1,
-                configs=8192, order=10, base_seed=2026101401, tag=0xC2)
 
 
 def parse():
@@ -60,8 +60,11 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB device-to-device once (calibrates HBM byte counters)")
     ap.add_argument("--no-per-order", action="store_true", help="skip the per-order evals/s table")
-    ap.add_argument("--inflight", type=int, default=3,
-                    help="c2: batches in flight (plans with their own buffers on their own streams)")
+    ap.add_argument("--inflight", type=int, default=None,
+                    help="c2: batches in flight (plans with their own buffers on their own streams); "
+                         "default: the tuning table's, else 3")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="ignore the workload's tuning table (tuning/<workload>.json) and use built-in launch shapes")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
@@ -165,6 +168,22 @@ def errors_per_order(ctx, ctx_ulp, orders, verts, mode, dev, sptr, order_list):
             e["fp64_one_ulp_tables"] = bound_metrics(ul, ref, bound)
         out[str(n)] = e
     return out
+
+
+def apply_tuning(ctx, workload, args):
+    """Apply the autotuner's table for this workload (tune.py); returns the
+    line's record of it and the table's batches in flight."""
+    import tune
+    if args.no_tuning:
+        return {"table": None, "reason": "--no-tuning"}, None
+    tab = tune.load_table(workload)
+    if not tab:
+        return {"table": None, "reason": "no table"}, None
+    ctx.set_tuning(tab["tuning"])
+    return ({"table": os.path.relpath(tune.table_path(workload), ROOT), "device": tab.get("device"),
+             "inflight": tab.get("inflight"), "in_effect": ctx.tuning(),
+             "tuned_vs_default_rate_when_tuned": round(tab["tuned_rate"] / tab["default_rate"], 4)},
+            tab.get("inflight"))
 
 
 def cpu_info():
@@ -334,32 +353,6 @@ def cpu_baseline_configs(w, params_gpu, tables, configs, seconds, per_unit=1, un
             "sample": f"{n} all-subset evaluations of {what} over {dt:.1f} s wall on {cores} threads"}
 
 
-C3 = dict(N1=8, N2=8, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="origin_square", radius=2,
-          lookups=1 << 20, base_seed=2026101401, tag=0xC3, block=1024)
-
-
-def c3_queries():
-    """2^20 twisted-table lookups on the 8x8 lattice: block i of 1024 lookups
-    draws from mt19937_64(derive_mcmcmc_seed(2026101401, 0xC3, i)), per lookup
-    six U[0,1) numbers in the order site_i, site_j, replica_a, replica_b, tau,
-    tau' (site = floor(64 u) with x = site mod 8, y = site div 8; replica =
-    floor(2 u); tau = beta u)."""
-    import rng
-    w = C3
-    nb = w["lookups"] // w["block"]
-    eng = rng.MT19937_64([rng.derive_mcmcmc_seed(w["base_seed"], w["tag"], i) for i in range(nb)])
-    u = np.stack([eng.uniform() for _ in range(6 * w["block"])], axis=1).reshape(nb, w["block"], 6)
-    u = u.reshape(-1, 6)
-    S = w["N1"] * w["N2"]
-    si = np.minimum((S * u[:, 0]).astype(np.int32), S - 1)
-    sj = np.minimum((S * u[:, 1]).astype(np.int32), S - 1)
-    qi = np.zeros((len(u), 7), dtype=np.int32)
-    qi[:, 1] = np.minimum((2 * u[:, 2]).astype(np.int32), 1)
-    qi[:, 2] = np.minimum((2 * u[:, 3]).astype(np.int32), 1)
-    qi[:, 3], qi[:, 4] = si % w["N1"], si // w["N1"]
-    qi[:, 5], qi[:, 6] = sj % w["N1"], sj // w["N1"]
-    qt = np.stack([w["beta"] * u[:, 4], w["beta"] * u[:, 5]], axis=1)
-    return qi, qt
 
 
 def main_c3(args):
@@ -430,8 +423,6 @@ def main_c3(args):
     ctx.close()
 
 
-C4 = dict(N1=6, N2=6, beta=8.0, U=2.0, mu=1.0, alpha=1.0, nc=128, mask="left_columns", width=3,
-          chains=256, envelope=12, base_seed=2026101401)
 
 
 def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):
@@ -490,6 +481,7 @@ def main_c4(args):
     print(f"c4: tables built in {build_s:.1f} s", file=sys.stderr, flush=True)
     mode = "mfma64" if args.mode == "split32" else args.mode
     cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[mode])
+    tuning_rec, _ = apply_tuning(ctx, "c4", args)
     E = w["envelope"]
     # GPU batch alone: one envelope batch of all chains, graph replays
     import rng
@@ -556,6 +548,7 @@ def main_c4(args):
                    "mode": mode, "table_build_s": round(build_s, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tuning": tuning_rec,
         "envelope_evals_per_s": round(sweeps * w["chains"] / dt, 1),
         "gpu_batch_ms": round(batch_ms, 4),
         # odd orders vanish at half filling; order 4 is the first nonzero ratio
@@ -565,8 +558,6 @@ def main_c4(args):
     ctx.close()
 
 
-C5 = dict(N1=10, N2=10, beta=10.0, U=2.0, mu=1.0, alpha=1.0, nc=192, mask="left_columns", width=5,
-          configs=1024, order_min=2, order_max=14, base_seed=2026101401, tag=0xC5)
 
 
 def main_c5(args):
@@ -589,6 +580,7 @@ def main_c5(args):
     torch.cuda.synchronize()
     build_s = time.time() - t0
     print(f"c5: tables built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    tuning_rec, _ = apply_tuning(ctx, "c5", args)
     orders, verts = rng.synthetic_configs(w["base_seed"], w["tag"], w["configs"], w["order_min"], w["order_max"],
                                           squared=True)
     d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device=dev)
@@ -673,6 +665,7 @@ def main_c5(args):
                    "table_build_s": round(build_s, 2), "node_chunk": 7 * w["nc"] + 3},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tuning": tuning_rec,
         "exact_fp64_evals_per_s": round(w["configs"] / (res["exact"]["ms"] * 1e-3), 1),
         "stages_ms": {m: res[m]["stage_ms"] for m in res},
         "lookups_per_batch": res["mfma64"]["lookups"],
@@ -719,6 +712,7 @@ def main():
     torch.cuda.synchronize()
     build_s = time.time() - t0
 
+    tuning_rec, tuned_inflight = apply_tuning(ctx, "c2", args)
     orders, verts = rng.synthetic_configs(w["base_seed"], w["tag"], w["configs"])
     sel = [i for i in range(len(orders)) if orders[i] == w["order"]]
     batch = [verts[i] for i in sel]
@@ -736,7 +730,7 @@ def main():
     # batches are in flight at once, as a driver feeding the evaluator with
     # double-buffered batches runs them; the single-stream rate is reported
     # beside it.
-    inflight = max(1, args.inflight)
+    inflight = max(1, args.inflight if args.inflight is not None else (tuned_inflight or 3))
     plans = [plan] + [ctx.plan([w["order"]] * count, mode=args.mode) for _ in range(inflight - 1)]
     outs = [d_out] + [torch.empty(plan.out_size, dtype=torch.float64, device=dev) for _ in range(inflight - 1)]
     stats = [d_status] + [torch.zeros(count, dtype=torch.int32, device=dev) for _ in range(inflight - 1)]
@@ -836,6 +830,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batches_in_flight": inflight,
+        "tuning": tuning_rec,
         "single_stream_evals_per_s": round(count * args.steps / single_dt, 1) if single_dt else round(value, 1),
         "plain_fp64_evals_per_s": round(plain, 1),
         "exact_fp64_batched_evals_per_s": round(exact_value, 1) if exact_value else round(value, 1),

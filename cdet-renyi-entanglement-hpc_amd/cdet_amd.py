@@ -50,6 +50,46 @@ class CdetParams(ctypes.Structure):
     ]
 
 
+MAX_ORDER = 14
+
+
+class CdetTuning(ctypes.Structure):
+    """cdet_tuning (include/cdet_hip.h): launch shapes; 0 keeps a default."""
+
+    _fields_ = [
+        ("contract_wg_per_cu", ctypes.c_int32), ("contract_tile", ctypes.c_int32),
+        ("pair_min_per_cu", ctypes.c_int32), ("max_lanes", ctypes.c_int32),
+        ("minors_threads", ctypes.c_int32 * (MAX_ORDER + 1)),
+        ("group16_min_terms", ctypes.c_int32 * (MAX_ORDER + 1)),
+        ("group64_min_terms", ctypes.c_int32 * (MAX_ORDER + 1)),
+    ]
+
+    SCALARS = ("contract_wg_per_cu", "contract_tile", "pair_min_per_cu", "max_lanes")
+    PER_ORDER = ("minors_threads", "group16_min_terms", "group64_min_terms")
+
+    def to_dict(self):
+        d = {k: int(getattr(self, k)) for k in self.SCALARS}
+        for k in self.PER_ORDER:
+            d[k] = [int(x) for x in getattr(self, k)]
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k in cls.SCALARS:
+            setattr(t, k, int(d.get(k, 0)))
+        for k in cls.PER_ORDER:
+            vals = d.get(k, [])
+            arr = getattr(t, k)
+            if isinstance(vals, dict):  # {order: value}
+                for n, v in vals.items():
+                    arr[int(n)] = int(v)
+            else:
+                for n, v in enumerate(vals[:MAX_ORDER + 1]):
+                    arr[n] = int(v)
+        return t
+
+
 class CdetCounters(ctypes.Structure):
     _fields_ = [("integrand_calls", ctypes.c_uint64), ("fpm_fallbacks", ctypes.c_uint64),
                 ("green_pair_cache_hits", ctypes.c_uint64), ("green_pair_cache_misses", ctypes.c_uint64)]
@@ -136,6 +176,8 @@ def _signatures():
         "cdet_ctx_set_batch_mode": (i, [vp, i]),
         "cdet_seed_cond_batch_host": (i, [vp, vp, vp, P(ctypes.c_int32), i64, dp]),
         "cdet_get_counters": (i, [vp, P(CdetCounters)]),
+        "cdet_ctx_set_tuning": (i, [vp, P(CdetTuning)]),
+        "cdet_ctx_get_tuning": (i, [vp, P(CdetTuning)]),
         "cdet_reset_counters": (i, [vp]),
         # include/cdet_chunks.h
         "cdet_chunks_write_plan": (i, [ctypes.c_char_p, P(ChunkSpec)]),
@@ -311,6 +353,18 @@ class Context:
 
     def reset_counters(self):
         _check(lib().cdet_reset_counters(self._h))
+
+    # -- launch-shape tuning (include/cdet_hip.h cdet_tuning) ---------------
+    def set_tuning(self, tuning):
+        """Apply a tuning table (a dict as tune.py writes it, or a
+        CdetTuning); plans created afterwards use it."""
+        t = tuning if isinstance(tuning, CdetTuning) else CdetTuning.from_dict(tuning)
+        _check(lib().cdet_ctx_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = CdetTuning()
+        _check(lib().cdet_ctx_get_tuning(self._h, ctypes.byref(t)))
+        return t.to_dict()
 
     def plan(self, orders, mode="exact", single=False, kind=None):
         plan = BatchPlan(self, orders, mode, single, kind)

@@ -97,10 +97,12 @@ constexpr int kFoldWaves = 4;  // waves that can run grouped chains (fold buffer
 // one lane, so grouping shortens the chain about 3.5x (G = 16) with no more
 // instructions in total when a one-lane-per-mask layer would leave most lanes
 // of its wave idle.
-__device__ inline int layer_group(int M, int R, int waves) {
+// g16 / g64: the chain lengths from which the 16-lane and whole-wave forms
+// are taken (64 and 128 by default; tunable per order).
+__device__ inline int layer_group(int M, int R, int waves, int g16 = 64, int g64 = 128) {
     const int wf = waves < kFoldWaves ? waves : kFoldWaves;
-    if (M == 1 && R >= 128) return 64;
-    if (M <= 4 * wf && R >= 64) return 16;
+    if (M == 1 && R >= g64) return 64;
+    if (M <= 4 * wf && R >= g16) return 16;
     return 0;
 }
 
@@ -113,20 +115,17 @@ __device__ inline int layer_group(int M, int R, int waves) {
 // the reference's.  The one excluded subset (s = rest, the mask itself) is
 // the first term of the first step and enters as 0, which leaves acc and rhs
 // unchanged.  Waves without a mask return at once; the caller synchronises.
+// One round of group_chains: the wave's kPerWave groups take masks base ..
+// base + kPerWave - 1 (those below end); wtb is the wave's fold buffer.
 template <int G>
-__device__ __forceinline__ void group_chains(const double* a, double* dp, int beg, int end, int R, double thr,
-                                             bool filter, const uint16_t* masks_by_pop, double* rhs, bool single,
-                                             double* tb) {
-    static_assert(G == 16 || G == 32 || G == 64, "groups fold eight terms at a time");
-    constexpr int kPerWave = 64 / G;
+__device__ __forceinline__ void group_chain_round(const double* a, double* dp, int base, int end, int R, double thr,
+                                                  bool filter, const uint16_t* masks_by_pop, double* rhs, bool single,
+                                                  double* wtb) {
     constexpr int kLoBits = G == 64 ? 6 : (G == 32 ? 5 : (G == 16 ? 4 : (G == 8 ? 3 : (G == 4 ? 2 : 1))));
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int w = tid >> 6;
-    if (w >= kFoldWaves || beg + w * kPerWave >= end) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
     const int gi = lane / G;
     const int j = lane - gi * G;
-    const int it = beg + w * kPerWave + gi;
+    const int it = base + gi;
     const bool have = it < end;
     const uint32_t mask = have ? masks_by_pop[it] : 1u;
     const bool rec = have && (!single || (mask & 1u));
@@ -157,7 +156,6 @@ __device__ __forceinline__ void group_chains(const double* a, double* dp, int be
     double acc = rec ? a[mask] : 0.0;
     double rs = fabs(acc);
     const int steps = R / G;
-    double* wtb = tb + w * (kFoldSteps * 64);
     for (int st0 = 0; st0 < steps; st0 += kFoldSteps) {
         // every index below is a subset of the mask, so the loads are in
         // bounds for all lanes; invalid terms are replaced by 0
@@ -217,6 +215,21 @@ __device__ __forceinline__ void group_chains(const double* a, double* dp, int be
     }
 }
 
+template <int G>
+__device__ __forceinline__ void group_chains(const double* a, double* dp, int beg, int end, int R, double thr,
+                                             bool filter, const uint16_t* masks_by_pop, double* rhs, bool single,
+                                             double* tb, int wave0 = 0, int nwaves = kFoldWaves) {
+    static_assert(G == 16 || G == 32 || G == 64, "groups fold eight terms at a time");
+    constexpr int kPerWave = 64 / G;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6;
+    const int wl = w - wave0;
+    if (w >= kFoldWaves || wl < 0 || wl >= nwaves) return;  // uniform per wave
+    // rounds of nwaves x kPerWave masks
+    for (int base = beg + wl * kPerWave; base < end; base += nwaves * kPerWave)
+        group_chain_round<G>(a, dp, base, end, R, thr, filter, masks_by_pop, rhs, single, tb + w * (kFoldSteps * 64));
+}
+
 // Subset recursion over all popcount layers, workgroup-cooperative.
 //   all-subset (single = false; include/cdet/integrand_replica_randomly_sampled.h:292-347):
 //     every mask recurses with anchor = its lowest bit;
@@ -231,7 +244,7 @@ __device__ __forceinline__ void group_chains(const double* a, double* dp, int be
 // layer_group).  tb holds kFoldWaves x kFoldSteps x 64 doubles.
 __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, int n, double thr, bool filter,
                                          const uint16_t* masks_by_pop, const int* pop_off, double* rhs, bool single,
-                                         double* tb) {
+                                         double* tb, int g16 = 64, int g64 = 128) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
@@ -241,7 +254,7 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
         const int beg = pop_off[bits];
         const int end = pop_off[bits + 1];
         const int R = 1 << (bits - 1);
-        const int G = layer_group(end - beg, R, nt >> 6);
+        const int G = layer_group(end - beg, R, nt >> 6, g16, g64);
         if (G == 64) {
             group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, single, tb);
         } else if (G == 16) {
@@ -257,6 +270,61 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
                 double r;
                 dp[mask] = connected_chain(a, dp, mask, anchor, mask ^ anchor, R - 1, thr, filter, r);
                 if (rhs) rhs[mask] = r;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// All-subset recursion of both table classes of one configuration at once:
+// a, dp (and rhs, if non-null) hold class 0 at [0, 2^n) and class 1 at
+// [2^n, 2^(n+1)).  Each popcount layer runs the masks of both classes before
+// one barrier: one lane per (class, mask), or grouped chains with the two
+// classes' masks on different waves when a layer has a single long chain per
+// class, or the classes one after the other with all fold waves.  Every value
+// is formed by the same chain as in ztransform_layers, so the results are
+// identical to two separate runs.
+__device__ __forceinline__ void ztransform_layers_pair(const double* a, double* dp, int n, double thr, bool filter,
+                                                       const uint16_t* masks_by_pop, const int* pop_off, double* rhs,
+                                                       double* tb, int g16 = 64, int g64 = 128) {
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int waves = nt >> 6;
+    const uint32_t sc = 1u << n;
+    for (uint32_t m = tid; m < 2 * sc; m += nt) {
+        const uint32_t mm = m & (sc - 1);
+        dp[m] = (mm != 0 && (mm & (mm - 1)) == 0) ? a[m] : 0.0;
+    }
+    __syncthreads();
+    for (int bits = 2; bits <= n; ++bits) {
+        const int beg = pop_off[bits];
+        const int end = pop_off[bits + 1];
+        const int M = end - beg;
+        const int R = 1 << (bits - 1);
+        const int G = layer_group(M, R, waves, g16, g64);
+        if (G == 64 && waves >= 2) {
+            // one long chain per class: class c on wave c
+            group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb, 0, 1);
+            group_chains<64>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,
+                             tb, 1, 1);
+        } else if (G == 64) {
+            group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb);
+            group_chains<64>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,
+                             tb);
+        } else if (G == 16) {
+            group_chains<16>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb);
+            group_chains<16>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,
+                             tb);
+        } else {
+            for (int q = tid; q < 2 * M; q += nt) {
+                const int cls = q >= M ? 1 : 0;
+                const uint32_t mask = masks_by_pop[beg + q - cls * M];
+                const uint32_t anchor = mask & (~mask + 1u);
+                const double* ac = a + cls * sc;
+                double* dc = dp + cls * sc;
+                double r;
+                dc[mask] = connected_chain(ac, dc, mask, anchor, mask ^ anchor, R - 1, thr, filter, r);
+                if (rhs) rhs[cls * sc + mask] = r;
             }
         }
         __syncthreads();

@@ -770,7 +770,8 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
         if (mode == 0 || mode == 2) {
             // mode 2 writes the magnitude scale of each value instead of the value:
             // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
-            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, false, tb);
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, false, tb,
+                              ks.group16_R[n], ks.group64_R[n]);
             // Assembly (:833-877).
             // paired: twisted - 2 * ordinary (include/cdet/cdet_integrand_evaluator.h:348-350),
             // or for magnitude scales twisted + 2 * ordinary, formed in place;
@@ -795,7 +796,8 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
             }
             if (bad && status) atomicOr(&status[c], 1);
         } else {
-            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, nullptr, true, tb);
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, nullptr, true, tb, ks.group16_R[n],
+                              ks.group64_R[n]);
             const double cres = dp[sc - 1];
             if (tid == 0) {
                 const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
@@ -806,6 +808,113 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
         }
         __syncthreads();
       }
+    }
+}
+
+// Paired classes in one pass (orders up to kPairOrderMax): one workgroup per
+// configuration runs the subset recursions of the twisted and the ordinary
+// class layer by layer together (ztransform_layers_pair) and forms
+// twisted - 2 * ordinary (mode 0) or the magnitude scales twisted + 2 *
+// ordinary (mode 2) from LDS: the same values, bit for bit, as k_cdet's paired
+// form, which runs the classes one after the other and reads the twisted
+// values back from HBM.
+constexpr int kPairOrderMax = 10;
+struct CdetPairLds {
+    static __host__ __device__ inline size_t doubles(int n, int mode) {
+        const size_t sc = size_t(1) << n;
+        return (mode == 2 ? 6 : 4) * sc + 4 * static_cast<size_t>(n) + sc / 4 + 1;
+    }
+    static __host__ __device__ inline size_t bytes(int n, int mode) { return sizeof(double) * doubles(n, mode); }
+};
+
+__global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode, int items,
+                                                   const int32_t* __restrict__ cfg_list,
+                                                   const int64_t* __restrict__ cfg_pbase,
+                                                   const int64_t* __restrict__ cfg_obase,
+                                                   const double* __restrict__ g_up, const double* __restrict__ g_dn,
+                                                   const double* __restrict__ pm, double* __restrict__ out,
+                                                   int32_t* __restrict__ status,
+                                                   const uint16_t* __restrict__ masks_by_pop,
+                                                   const int* __restrict__ pop_off, int32_t* __restrict__ fb_count) {
+    extern __shared__ double lds_dyn[];
+    __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
+    __shared__ int s_popoff[kMaxOrder + 2];
+    if (fb_count && blockIdx.x == 0 && threadIdx.x == 0) *fb_count = 0;  // the bucket's fallback queue is consumed
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off[k];
+    const uint32_t sc = 1u << n;
+    const int n2 = n * n;
+    double* a = lds_dyn;
+    double* dp = a + 2 * sc;
+    double* rhs = dp + 2 * sc;  // mode 2 only
+    double* dU = rhs + (mode == 2 ? 2 * sc : 0);
+    double* dD = dU + 2 * n;
+    uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + 2 * n);
+    for (int li = blockIdx.x; li < items; li += gridDim.x) {
+        const int c = cfg_list[li];
+        // unscaled shifted diagonal entries for the singletons
+        // (include/cdet/integrand_replica_randomly_sampled.h:751-758)
+        for (int q = tid; q < 2 * n; q += nt) {
+            const int cls = q >= n ? 1 : 0;
+            const int i = q - cls * n;
+            const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
+            dU[q] = g_up[pb + i * n + i] + ks.dup;
+            dD[q] = g_dn[pb + i * n + i] + ks.ddn;
+        }
+        // a[class][mask] = pm_up * pm_down, a[class][0] = 0; four masks per
+        // thread in flight
+        {
+            constexpr int kPre = 4;
+            double vu[kPre], vd[kPre];
+            for (uint32_t m0 = tid; m0 < 2 * sc; m0 += kPre * nt) {
+#pragma unroll
+                for (int q = 0; q < kPre; ++q) {
+                    const uint32_t m = m0 + q * nt;
+                    if (m < 2 * sc) {
+                        const int cls = m >= sc ? 1 : 0;
+                        const uint32_t mm = m - cls * sc;
+                        const double* pu = pm + static_cast<size_t>(4 * li + 2 * cls) * sc;
+                        const double* pd = ks.same_spin[cls] ? pu : pu + sc;
+                        vu[q] = pu[mm];
+                        vd[q] = pd[mm];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kPre; ++q) {
+                    const uint32_t m = m0 + q * nt;
+                    if (m < 2 * sc) a[m] = (m & (sc - 1)) == 0 ? 0.0 : vu[q] * vd[q];
+                }
+            }
+        }
+        for (uint32_t m = tid; m < sc; m += nt) lmask[m] = masks_by_pop[m];
+        __syncthreads();
+        ztransform_layers_pair(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, tb,
+                               ks.group16_R[n], ks.group64_R[n]);
+        // assembly (:833-877) and twisted - 2 * ordinary
+        // (include/cdet/cdet_integrand_evaluator.h:348-350)
+        double* dst = out + cfg_obase[c];
+        bool bad = false;
+        for (uint32_t m = tid; m < sc; m += nt) {
+            double v[2];
+#pragma unroll
+            for (int cls = 0; cls < 2; ++cls) {
+                if (m == 0) {
+                    v[cls] = 0.0;
+                } else if ((m & (m - 1)) == 0) {
+                    const int i = __builtin_ctz(m);
+                    v[cls] = ks.c1[cls] * (dU[cls * n + i] * dD[cls * n + i] - ks.counter);
+                    if (mode == 2) v[cls] = fabs(v[cls]);
+                } else {
+                    const int k = __builtin_popcount(m);
+                    v[cls] = mode == 2 ? fabs(ks.pref[cls][k]) * rhs[cls * sc + m] : ks.pref[cls][k] * dp[cls * sc + m];
+                }
+                if (!isfinite(v[cls])) bad = true;
+            }
+            dst[m] = mode == 2 ? v[0] + 2.0 * v[1] : v[0] - 2.0 * v[1];
+        }
+        if (bad && status) atomicOr(&status[c], 1);
+        __syncthreads();
     }
 }
 
@@ -955,6 +1064,8 @@ struct cdet_ctx {
     std::mutex host_mu;
     // derived table copies (step order, split FP32), made once per table
     std::mutex derive_mu;
+    // launch-shape choices in effect (cdet_ctx_set_tuning)
+    cdet_tuning tuning{};
     // device address of this device's copy of `scratch` (no-allocation entry point)
     unsigned char* noalloc_scratch = nullptr;
     // host-buffer batch path (driver callback)
@@ -1130,6 +1241,10 @@ void refresh_scalars(cdet_ctx* ctx) {
         k.abssum[kind][0] = t.abssum[0];
         k.abssum[kind][1] = t.abssum[1];
     }
+    for (int order = 0; order <= kMaxOrder; ++order) {
+        k.group16_R[order] = ctx->tuning.group16_min_terms[order];
+        k.group64_R[order] = ctx->tuning.group64_min_terms[order];
+    }
     for (int cls = 0; cls < 2; ++cls) {
         const int ku = cls == 0 ? 0 : 2;
         k.same_spin[cls] = (k.coeff[ku][0] == k.coeff[ku + 1][0] && k.coeff[ku][1] == k.coeff[ku + 1][1] &&
@@ -1275,6 +1390,22 @@ static constexpr size_t kCdetMaxDyn = kMaxLds - sizeof(double) * kFoldWaves * kF
 // scratch), and 256 threads would leave each SIMD a single wave.  At order 12
 // (two workgroups per CU) the wider form measured slower.
 static constexpr int kWideOrder = 13;
+
+// Built-in launch shapes (cdet_tuning defaults).
+static cdet_tuning default_tuning() {
+    cdet_tuning t{};
+    t.contract_wg_per_cu = 8;
+    t.contract_tile = 0;
+    t.pair_min_per_cu = 2;
+    t.max_lanes = 4;
+    for (int n = 0; n <= kMaxOrder; ++n) {
+        t.minors_threads[n] = n >= kWideOrder ? 1024 : 256;
+        t.group16_min_terms[n] = 64;
+        t.group64_min_terms[n] = 128;
+    }
+    return t;
+}
+static bool wide_minors(const cdet_ctx* ctx, int n) { return ctx->tuning.minors_threads[n] >= 1024; }
 static bool cdet_gscr(int n, int mode) { return CdetLds::bytes(n, mode) > kCdetMaxDyn; }
 
 // Outputs and queues of one minors launch.
@@ -1353,7 +1484,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
             }
         } else if (fpm_gscr(n)) {
             const int grid = std::min(items, kGscrGrid);
-            if (n >= kWideOrder)
+            if (wide_minors(ctx, n))
                 hipLaunchKernelGGL((k_fpm<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, items, cfg_list,
                                    pbase, gup, gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
             else
@@ -1361,7 +1492,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
                                    pbase, gup, gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
             next += static_cast<size_t>(grid) * FpmLds::doubles(n);
         } else {
-            if (n >= kWideOrder)
+            if (wide_minors(ctx, n))
                 hipLaunchKernelGGL((k_fpm<false, 1024>), dim3(items), dim3(1024), FpmLds::bytes(n), stream, ctx->ks, n,
                                    items, cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
             else
@@ -1379,10 +1510,16 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         }
     }
     const int paired = (mode != 1 && o.paired) ? 1 : 0;
+    if (paired && n <= kPairOrderMax) {
+        hipLaunchKernelGGL(k_cdet_pair, dim3(count), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n, mode,
+                           count, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_count);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     const int items = paired ? count : 2 * count;
     if (cdet_gscr(n, mode)) {
         const int grid = std::min(items, kGscrGrid);
-        if (n >= kWideOrder)
+        if (wide_minors(ctx, n))
             hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, mode, items,
                                cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
                                popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired);
@@ -1391,7 +1528,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
                                cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
                                popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired);
     } else {
-        if (n >= kWideOrder)
+        if (wide_minors(ctx, n))
             hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ctx->ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
                                o.status, masks, popoff, nullptr, 0, o.fb_count, paired);
@@ -1426,8 +1563,9 @@ static bool use_f64_v2(const KScalars& ks) { return ks.nu % 16 == 0 && ks.nv % 1
 // lookups, so that more, smaller tiles spread the matrix-core work evenly
 // over the SIMDs (each extra tile re-reads its coefficient block, which the
 // large batches cannot afford).
-static int mfma_tile_queries(const KScalars& ks, int mode, int64_t lookups, int ncu) {
+static int mfma_tile_queries(const KScalars& ks, int mode, int64_t lookups, int ncu, int forced = 0) {
     if (mode != CDET_MODE_MFMA64 || !use_f64_v2(ks)) return kMfmaTile;
+    if (forced == 16 || forced == kF64Tile) return forced;
     const int64_t resident_waves = 16 * static_cast<int64_t>(ncu);
     return lookups / kF64Tile >= 3 * resident_waves ? kF64Tile : 16;
 }
@@ -1543,7 +1681,8 @@ static int contract_launch_mfma(const KScalars& ks, int mode, int grid, const in
 // Grid of a persistent contraction launch: one wave per tile up to 8
 // workgroups per CU, a multiple of the 8 XCDs.
 static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles) {
-    int64_t g = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), static_cast<int64_t>(ctx->ncu) * 8);
+    const int64_t per_cu = ctx->tuning.contract_wg_per_cu;
+    int64_t g = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), static_cast<int64_t>(ctx->ncu) * per_cu);
     g = ((g + 7) / 8) * 8;
     return static_cast<int>(g);
 }
@@ -1626,6 +1765,7 @@ static int ctx_init(cdet_ctx* ctx) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     rc = noalloc_scratch_of(ctx->device, &ctx->noalloc_scratch);
     if (rc) return rc;
+    ctx->tuning = default_tuning();
     refresh_scalars(ctx);
     return 0;
 }
@@ -1963,6 +2103,43 @@ int cdet_eval_all_subsets_noalloc(cdet_ctx* ctx, const cdet_vertex* d_vertices, 
     rc = launch_combine(0, 1, static_cast<int64_t>(size_t(1) << n), ctx->d_zero64, o, s);
     if (rc) return rc;
     ctx->calls += 1;
+    return 0;
+}
+
+int cdet_ctx_set_tuning(cdet_ctx* ctx, const cdet_tuning* in) {
+    if (!ctx || !in) return fail(CDET_ERR_ARGUMENT, "null argument");
+    cdet_tuning t = default_tuning();
+    if (in->contract_wg_per_cu > 0) {
+        if (in->contract_wg_per_cu > 8) return fail(CDET_ERR_ARGUMENT, "contract_wg_per_cu must be 1..8");
+        t.contract_wg_per_cu = in->contract_wg_per_cu;
+    }
+    if (in->contract_tile > 0) {
+        if (in->contract_tile != 16 && in->contract_tile != kF64Tile)
+            return fail(CDET_ERR_ARGUMENT, "contract_tile must be 16 or 64");
+        t.contract_tile = in->contract_tile;
+    }
+    if (in->pair_min_per_cu > 0) t.pair_min_per_cu = in->pair_min_per_cu;
+    if (in->max_lanes > 0) {
+        if (in->max_lanes > kMinorLanes) return fail(CDET_ERR_ARGUMENT, "max_lanes must be 1..4");
+        t.max_lanes = in->max_lanes;
+    }
+    for (int n = 0; n <= kMaxOrder; ++n) {
+        if (in->minors_threads[n] > 0) {
+            if (in->minors_threads[n] != 256 && in->minors_threads[n] != 1024)
+                return fail(CDET_ERR_ARGUMENT, "minors_threads must be 256 or 1024");
+            t.minors_threads[n] = in->minors_threads[n];
+        }
+        if (in->group16_min_terms[n] > 0) t.group16_min_terms[n] = std::max(16, in->group16_min_terms[n]);
+        if (in->group64_min_terms[n] > 0) t.group64_min_terms[n] = std::max(64, in->group64_min_terms[n]);
+    }
+    ctx->tuning = t;
+    refresh_scalars(ctx);  // plans made afterwards capture the new thresholds
+    return 0;
+}
+
+int cdet_ctx_get_tuning(cdet_ctx* ctx, cdet_tuning* out) {
+    if (!ctx || !out) return fail(CDET_ERR_ARGUMENT, "null argument");
+    *out = ctx->tuning;
     return 0;
 }
 

@@ -6,7 +6,6 @@
 // the sequence is captured once into a hipGraph and replayed.
 
 constexpr double kForkMinUs = 40.0;  // bucket work beyond the longest bucket worth a fork
-constexpr int kPairMin = 2;          // configurations per CU in every bucket for paired classes
 
 struct cdet_batch_plan {
     cdet_ctx* ctx = nullptr;
@@ -57,7 +56,7 @@ struct cdet_batch_plan {
     // and are shared by its plans.
     // Both classes of a configuration in one k_cdet workgroup, combined in
     // place (no k_combine_all pass over the outputs): when every order bucket
-    // holds at least kPairMin configurations per CU, so that the longer
+    // holds at least pair_min_per_cu (tuning) configurations per CU, so that the longer
     // workgroups still fill the device.
     bool paired = false;
     int lanes = 1;
@@ -307,7 +306,7 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
     auto* p = new cdet_batch_plan();
     p->ctx = ctx;
-    p->max_lanes = std::max(1, std::min(max_lanes, kMinorLanes));
+    p->max_lanes = std::max(1, std::min({max_lanes, kMinorLanes, ctx->tuning.max_lanes}));
     p->contract_mode = mode;
     if (eval_kind < 0 || eval_kind > 3) {
         delete p;
@@ -350,7 +349,7 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
         std::vector<int> fillpos(p->bucket_off.begin(), p->bucket_off.end());
         for (int64_t c = 0; c < count; ++c) lists[static_cast<size_t>(fillpos[static_cast<size_t>(orders[c])]++)] = static_cast<int32_t>(c);
     }
-    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks, mode, p->nq, ctx->ncu);
+    p->tile = (mode == CDET_MODE_EXACT) ? 64 : mfma_tile_queries(ctx->ks, mode, p->nq, ctx->ncu, ctx->tuning.contract_tile);
     p->nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
     const int64_t nonempty = std::min<int64_t>(p->nkeys, p->nq);
     p->max_tiles = p->nq / p->tile + nonempty + 1;
@@ -408,7 +407,8 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
         int min_cnt = std::numeric_limits<int>::max();
         for (int n = 1; n <= kMaxOrder; ++n)
             if (p->bucket_cnt[static_cast<size_t>(n)]) min_cnt = std::min(min_cnt, p->bucket_cnt[static_cast<size_t>(n)]);
-        p->paired = (p->eval_kind == 0 || p->eval_kind == 2) && min_cnt >= kPairMin * ctx->ncu;
+        p->paired = (p->eval_kind == 0 || p->eval_kind == 2) &&
+                    static_cast<int64_t>(min_cnt) >= static_cast<int64_t>(ctx->tuning.pair_min_per_cu) * ctx->ncu;
     }
     rc = plan_lanes(p, lane_streams);
     if (rc) {
@@ -520,7 +520,8 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     std::lock_guard<std::mutex> lock(ctx->query_mu);
     QueryScratch& q = ctx->query;
     const int32_t nkeys = static_cast<int32_t>(4 * ctx->ks.SP);
-    const int tile = mode == CDET_MODE_EXACT ? 64 : mfma_tile_queries(ctx->ks, mode, count, ctx->ncu);
+    const int tile =
+        mode == CDET_MODE_EXACT ? 64 : mfma_tile_queries(ctx->ks, mode, count, ctx->ncu, ctx->tuning.contract_tile);
     const int64_t max_tiles = count / tile + std::min<int64_t>(nkeys, count) + 1;
     if (q.cap < count || q.tiles_cap < max_tiles || q.nkeys != nkeys || !q.hist) {
         query_scratch_free(q);

@@ -16,6 +16,8 @@ struct KScalars {
     double pref_hi[2][kMaxOrder + 1];  // single-order prefactor, double-double
     double pref_lo[2][kMaxOrder + 1];
     int same_spin[2];                  // class up/down tables and shifts identical
+    int group16_R[kMaxOrder + 1];      // per order: chain length from which <= 16-mask layers group 16 lanes
+    int group64_R[kMaxOrder + 1];      // per order: chain length from which a single chain takes a wave
     const double* coeff[4][2];         // [kind][branch 0 = upper, 1 = lower]
     const double* corner[4][2];
     const double* steps[4][2];         // coeff in matrix-core step order (FP64 MFMA mode)

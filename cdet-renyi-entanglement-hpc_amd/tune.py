@@ -1,0 +1,224 @@
+"""Autotuner of the batched CDet integrand's launch shapes, per workload.
+
+Times, on the device it runs on, the choices of include/cdet_hip.h's
+cdet_tuning for one workload's batch and writes the fastest as a JSON table
+that bench.py (and any caller, through Context.set_tuning) applies:
+
+  contract_wg_per_cu   persistent contraction workgroups per CU (4..8): fewer
+                       leave wave slots to the other batches in flight
+  contract_tile        lookups per FP64 matrix-core tile (16 or 64)
+  pair_min_per_cu      configurations per CU from which both table classes run
+                       in one workgroup
+  max_lanes            streams a batch's order buckets spread over (1..4)
+  minors_threads[n]    per order >= 11: 256 or 1024 threads per workgroup of
+                       the minors kernels
+  group16/64_min_terms[n]  per order: chain lengths from which the subset
+                       recursion runs a popcount layer's chains on 16 lanes per
+                       mask / a whole wavefront
+  inflight             batches in flight (plans on their own streams); a
+                       property of the caller, recorded with the table
+
+The objective is the workload's throughput as bench.py measures it:
+configurations per second of graph-replayed batches, with `inflight` batches
+in flight for C2 (searched over 1..4) and one at a time for C4 (the batch of
+the 256 chain envelopes) and C5.  The search is coordinate descent from the built-in defaults,
+two passes, each candidate timed as the median of several runs and kept only
+for a gain above 2%.  Every choice
+gives bit-identical values; the tuner checks that the tuned batch reproduces
+the default batch's output bit for bit before writing the table.
+
+No counterpart in the reference, whose only work-split knob is its thread
+pool (include/cdet/integrand_parallel.h:34-154).
+
+Usage (on the GPU host):
+  python cdet-renyi-entanglement-hpc_amd/tune.py --workload c2 --out tuning/c2.json
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+import cdet_amd  # noqa: E402
+import workloads  # noqa: E402
+
+TABLE_DIR = os.path.join(HERE, "tuning")
+
+
+def table_path(workload):
+    """Committed tuning table of a workload (tuning/<workload>.json)."""
+    return os.path.join(TABLE_DIR, f"{workload}.json")
+
+
+def load_table(workload, path=None):
+    """The tuning table written by this tool for a workload, or None."""
+    p = path or table_path(workload)
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+class Batch:
+    """A workload's batch on the device, timed with `inflight` plans in
+    flight (graph replays), as bench.py runs it."""
+
+    def __init__(self, ctx, orders, verts, mode):
+        import torch
+        self.torch = torch
+        self.ctx = ctx
+        self.orders = [int(x) for x in orders]
+        self.mode = mode
+        self.d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device="cuda")
+        self.streams = [torch.cuda.current_stream()] + [torch.cuda.Stream() for _ in range(3)]
+
+    def run_once(self):
+        """Output of one eager run with the tuning in effect."""
+        torch = self.torch
+        plan = self.ctx.plan(self.orders, mode=self.mode)
+        out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+        plan.run(self.d_v, out, None, use_graph=False, stream=self.streams[0].cuda_stream)
+        torch.cuda.synchronize()
+        plan.close()
+        return out.cpu().numpy()
+
+    def rate(self, inflight, steps):
+        """Configurations per second over `steps` batches."""
+        torch = self.torch
+        plans = [self.ctx.plan(self.orders, mode=self.mode) for _ in range(inflight)]
+        outs = [torch.empty(plans[0].out_size, dtype=torch.float64, device="cuda") for _ in range(inflight)]
+        for k in range(inflight):
+            for _ in range(2):
+                plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k].cuda_stream)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(steps):
+            k = i % inflight
+            plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k].cuda_stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        for p in plans:
+            p.close()
+        return len(self.orders) * steps / dt
+
+
+def candidates(orders):
+    """(knob, values) pairs of the search; per-order knobs only for the orders
+    the batch holds."""
+    present = sorted(set(int(n) for n in orders))
+    out = [("contract_wg_per_cu", [8, 7, 6, 5, 4]),
+           ("contract_tile", [16, 64]),
+           ("pair_min_per_cu", [1, 2, 4, 1 << 20]),
+           ("max_lanes", [1, 2, 3, 4])]
+    for n in present:
+        if n >= 11:
+            out.append((("minors_threads", n), [256, 1024]))
+        if n >= 6:
+            out.append((("group16_min_terms", n), [32, 64, 128, 1 << 20]))
+        if n >= 8:
+            out.append((("group64_min_terms", n), [64, 128, 256, 1 << 20]))
+    return out
+
+
+def apply(table, knob, value):
+    t = json.loads(json.dumps(table))
+    if isinstance(knob, tuple):
+        name, n = knob
+        t[name][n] = value
+    else:
+        t[knob] = value
+    return t
+
+
+def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 3, 4), log=print):
+    import torch
+    torch.cuda.set_device(0)
+    ctx = workloads.context(workload)
+    orders, verts = workloads.configs(workload)
+    if workload == "c2":  # the headline batch: every order-10 configuration
+        sel = [i for i in range(len(orders)) if orders[i] == workloads.C2["order"]]
+        orders, verts = orders[sel], [verts[i] for i in sel]
+    batch = Batch(ctx, orders, verts, mode)
+    # C2 is timed as the benchmark driver times it (20-step runs); the median
+    # of `reps` runs is one measurement, and a change is kept only for a gain
+    # above `gain`, well outside the run-to-run spread
+    steps = steps or {"c2": 20, "c4": 40, "c5": 6}.get(workload, 20)
+    reps = {"c2": 7, "c4": 5, "c5": 3}.get(workload, 3)
+    gain = 1.02
+    base = ctx.tuning()
+    reference = batch.run_once()
+    inflight = 3 if workload == "c2" else 1  # as bench.py runs each workload
+
+    def measure(table, k_inflight):
+        ctx.set_tuning(table)
+        r = sorted(batch.rate(k_inflight, steps) for _ in range(reps))
+        return r[reps // 2]
+
+    best = dict(base)
+    best_rate = measure(best, inflight)
+    log(f"{workload}: defaults {best_rate:.4g} configurations/s with {inflight} in flight")
+    history = [{"tuning": "defaults", "inflight": inflight, "rate": best_rate}]
+    for ps in range(passes):
+        improved = False
+        if workload == "c2":
+            for k in inflight_choices:
+                if k == inflight:
+                    continue
+                r = measure(best, k)
+                history.append({"knob": "inflight", "value": k, "rate": r})
+                if r > best_rate * gain:
+                    best_rate, inflight, improved = r, k, True
+                    log(f"  inflight = {k}: {r:.4g}")
+        for knob, values in candidates(orders):
+            cur = best[knob[0]][knob[1]] if isinstance(knob, tuple) else best[knob]
+            for v in values:
+                if v == cur:
+                    continue
+                t = apply(best, knob, v)
+                r = measure(t, inflight)
+                history.append({"knob": knob if not isinstance(knob, tuple) else f"{knob[0]}[{knob[1]}]",
+                                "value": v, "rate": r})
+                if r > best_rate * gain:  # keep a change only for a clear gain
+                    best, best_rate, improved = t, r, True
+                    log(f"  {knob} = {v}: {r:.4g}")
+        if not improved:
+            break
+    ctx.set_tuning(best)
+    tuned = batch.run_once()
+    if not np.array_equal(tuned.view(np.int64), reference.view(np.int64)):
+        raise RuntimeError("tuned launch shapes changed the values; table not written")
+    ctx.close()
+    props = torch.cuda.get_device_properties(0)
+    return {"workload": workload, "mode": mode, "device": props.name, "cus": props.multi_processor_count,
+            "objective": "configurations/s" if workload != "c4" else "configurations/s of one 256-envelope batch",
+            "default_rate": history[0]["rate"], "tuned_rate": best_rate, "inflight": inflight,
+            "tuning": best, "history": history, "steps_per_measurement": steps,
+            "values_bit_identical_to_defaults": True}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"])
+    ap.add_argument("--mode", default="mfma64", choices=["exact", "mfma64"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--out", default=None, help="table path (default tuning/<workload>.json)")
+    args = ap.parse_args()
+    res = tune(args.workload, args.mode, args.steps, args.passes,
+               log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    out = args.out or table_path(args.workload)
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps({k: res[k] for k in ("workload", "default_rate", "tuned_rate", "inflight")}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

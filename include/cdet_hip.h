@@ -306,6 +306,33 @@ int cdet_eval_batch(cdet_ctx* ctx, const cdet_vertex* d_vertices, const int32_t*
 int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus,
                          int64_t count, double* out, int mode, void* stream);
 
+/*
+ * Launch-shape tuning of a context.  No counterpart in the reference, whose
+ * only work-split knob is its integrand thread pool
+ * (include/cdet/integrand_parallel.h:34-154).  The package's autotuner
+ * (tune.py) times these choices per workload on the device and writes a
+ * table that callers apply with cdet_ctx_set_tuning; plans read the values in
+ * effect when they are created.  Every choice gives bit-identical values.
+ * A field <= 0 keeps the built-in default (in parentheses).
+ */
+typedef struct cdet_tuning {
+    int32_t contract_wg_per_cu;   /* persistent contraction workgroups per CU, 1..8 (8) */
+    int32_t contract_tile;        /* lookups per FP64 matrix-core tile, 16 or 64 (by batch size) */
+    int32_t pair_min_per_cu;      /* both classes in one workgroup when every order bucket holds at
+                                     least this many configurations per CU (2); a large value turns
+                                     pairing off */
+    int32_t max_lanes;            /* streams a plan's order buckets may spread over, 1..4 (4) */
+    int32_t minors_threads[CDET_MAX_ORDER + 1];    /* per order: threads per workgroup of the general
+                                                      minors kernels, 256 or 1024 (1024 from order 13) */
+    int32_t group16_min_terms[CDET_MAX_ORDER + 1]; /* per order: a popcount layer of at most 16 masks
+                                                      whose chains have at least this many terms runs
+                                                      on 16 lanes per mask (64) */
+    int32_t group64_min_terms[CDET_MAX_ORDER + 1]; /* per order: a single chain of at least this many
+                                                      terms runs on a whole wavefront (128) */
+} cdet_tuning;
+int cdet_ctx_set_tuning(cdet_ctx* ctx, const cdet_tuning* tuning);
+int cdet_ctx_get_tuning(cdet_ctx* ctx, cdet_tuning* out); /* the values in effect */
+
 /* Runtime counters (include/cdet/runtime_counters.h:7-51). */
 int cdet_get_counters(cdet_ctx* ctx, cdet_counters* out);
 int cdet_reset_counters(cdet_ctx* ctx);

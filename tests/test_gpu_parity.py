@@ -645,3 +645,42 @@ def test_error_bound_covers_table_perturbations(gpu_available, name):
     # the bound is not vacuous: the perturbed evaluations reach a visible
     # fraction of it somewhere
     assert max(ratios["random"], ratios["coherent"]) > 1e-4
+
+
+def test_tuning_choices_bit_identical(gpu_available):
+    # Every launch-shape choice of cdet_tuning gives the same values bit for
+    # bit: contraction workgroups per CU and tile, paired classes, forked
+    # streams, minors workgroup size per order, grouped-chain thresholds.
+    import torch
+    ctx = gpu_ctx("c1")
+    default = ctx.tuning()
+    orders = [2, 3, 5, 6, 8, 9, 10, 11, 12, 12] * 4 + [6] * 600
+    configs = helpers.seeded_configs(9400, orders)
+    choices = [
+        {},
+        {"contract_wg_per_cu": 4, "contract_tile": 64},
+        {"contract_wg_per_cu": 5, "contract_tile": 16, "max_lanes": 1},
+        {"pair_min_per_cu": 1, "max_lanes": 2},
+        {"pair_min_per_cu": 1 << 20},
+        {"minors_threads": {11: 1024, 12: 1024}, "group16_min_terms": {n: 32 for n in range(15)},
+         "group64_min_terms": {n: 64 for n in range(15)}},
+        {"group16_min_terms": {n: 1 << 20 for n in range(15)}, "group64_min_terms": {n: 1 << 20 for n in range(15)}},
+    ]
+    ref = {}
+    try:
+        for mode in ("exact", "mfma64"):
+            for k, ch in enumerate(choices):
+                ctx.set_tuning(ch)
+                say(f"tuning choice {k} {mode}")
+                (out,), _, _ = run_batch(ctx, configs, mode)
+                if k == 0:
+                    ref[mode] = out
+                else:
+                    assert bits_equal(out, ref[mode]), (mode, ch)
+        for bad in ({"contract_wg_per_cu": 9}, {"contract_tile": 32}, {"max_lanes": 5},
+                    {"minors_threads": {12: 512}}):
+            with pytest.raises(RuntimeError):
+                ctx.set_tuning(bad)
+    finally:
+        ctx.set_tuning({})
+    assert ctx.tuning() == default

@@ -1658,14 +1658,21 @@ static int ensure_abssum_tables(cdet_ctx* ctx) {
 
 static int contract_launch_mfma(const KScalars& ks, int mode, int grid, const int32_t* ntiles, const int32_t* tkey,
                                 const int32_t* tstart, const int32_t* tlen, const double* su, const double* sv,
-                                const int32_t* sdest, double* gup, double* gdn, hipStream_t s) {
+                                const int32_t* sdest, double* gup, double* gdn, hipStream_t s, int tile) {
     if (mode == CDET_MODE_MFMA64 && use_f64_v2(ks)) {
-        if (ks.same_spin[0] && ks.same_spin[1])
-            hipLaunchKernelGGL(k_contract_f64<true>, dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen, su,
-                               sv, sdest, gup, gdn);
+        const bool same = ks.same_spin[0] && ks.same_spin[1];
+        if (tile <= 16 && same)
+            hipLaunchKernelGGL((k_contract_f64<true, 1>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen,
+                               su, sv, sdest, gup, gdn);
+        else if (tile <= 16)
+            hipLaunchKernelGGL((k_contract_f64<false, 1>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart,
+                               tlen, su, sv, sdest, gup, gdn);
+        else if (same)
+            hipLaunchKernelGGL((k_contract_f64<true, 4>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen,
+                               su, sv, sdest, gup, gdn);
         else
-            hipLaunchKernelGGL(k_contract_f64<false>, dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen, su,
-                               sv, sdest, gup, gdn);
+            hipLaunchKernelGGL((k_contract_f64<false, 4>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart,
+                               tlen, su, sv, sdest, gup, gdn);
     } else if (mode == CDET_MODE_MFMA64) {
         hipLaunchKernelGGL(k_contract_mfma<CDET_MODE_MFMA64>, dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart,
                            tlen, su, sv, sdest, gup, gdn);
@@ -1680,8 +1687,15 @@ static int contract_launch_mfma(const KScalars& ks, int mode, int grid, const in
 
 // Grid of a persistent contraction launch: one wave per tile up to 8
 // workgroups per CU, a multiple of the 8 XCDs.
-static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles) {
-    const int64_t per_cu = ctx->tuning.contract_wg_per_cu;
+static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles, int mode = CDET_MODE_EXACT, int tile = 64) {
+    int64_t per_cu = ctx->tuning.contract_wg_per_cu;
+    if (mode == CDET_MODE_MFMA64 && use_f64_v2(ctx->ks)) {
+        // no more workgroups than fit at once (256 threads = one wave per SIMD)
+        const bool same = ctx->ks.same_spin[0] && ctx->ks.same_spin[1];
+        const int fit = tile <= 16 ? (same ? contract_f64_waves<true, 1>() : contract_f64_waves<false, 1>())
+                                   : (same ? contract_f64_waves<true, 4>() : contract_f64_waves<false, 4>());
+        per_cu = std::min<int64_t>(per_cu, fit);
+    }
     int64_t g = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), static_cast<int64_t>(ctx->ncu) * per_cu);
     g = ((g + 7) / 8) * 8;
     return static_cast<int>(g);

@@ -385,8 +385,16 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
 // XCDs by xcd_tiles).  SAME selects the form for contexts whose up and down
 // tables are one table for both classes; it is held to 128 registers so that
 // four waves share each SIMD.
-template <bool SAME>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SAME ? 4 : 2))) void k_contract_f64(
+// MAXNT = 1 is the form for tiles of at most 16 queries (one N tile): it
+// needs a quarter of the accumulators, so that eight waves (six with separate
+// up and down tables) share each SIMD without spilling.
+template <bool SAME, int MAXNT>
+constexpr int contract_f64_waves() {
+    return MAXNT == 1 ? (SAME ? 8 : 6) : (SAME ? 4 : 2);
+}
+template <bool SAME, int MAXNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(contract_f64_waves<SAME, MAXNT>()))) void
+k_contract_f64(
     KScalars ks, const int32_t* __restrict__ n_tiles, const int32_t* __restrict__ tile_key,
     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_len, const double* __restrict__ s_u,
     const double* __restrict__ s_v, const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
@@ -406,7 +414,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SAME ? 4 : 
         bool same;
         ks.key_steps(key, cu, cd, same);
         const int nts = (len + 15) >> 4;
-        if (nts == 4)
+        if (MAXNT == 1)
+            contract_f64_tile<SAME, 1>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
+        else if (nts == 4)
             contract_f64_tile<SAME, 4>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
         else if (nts == 3)
             contract_f64_tile<SAME, 3>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);

@@ -287,7 +287,8 @@ static int contract_launch(cdet_batch_plan* p, hipStream_t s) {
         case CDET_MODE_MFMA64:
         case CDET_MODE_SPLIT32:
             return contract_launch_mfma(ctx->ks, p->contract_mode, p->contract_grid, p->d_ntiles, p->d_tkey,
-                                        p->d_tstart, p->d_tlen, p->d_su, p->d_sv, p->d_sdest, p->d_gup, p->d_gdn, s);
+                                        p->d_tstart, p->d_tlen, p->d_su, p->d_sv, p->d_sdest, p->d_gup, p->d_gdn, s,
+                                        p->tile);
         default:
             return fail(CDET_ERR_ARGUMENT, "unknown evaluation mode");
     }
@@ -400,7 +401,7 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
         plan_free(p);
         return fail(CDET_ERR_DEVICE, "plan upload failed");
     }
-    p->contract_grid = contract_grid(ctx, p->max_tiles);
+    p->contract_grid = contract_grid(ctx, p->max_tiles, mode, p->tile);
     // Global scratch (minors of every spin matrix, workspaces of large
     // orders), sized now so that runs never allocate and can be graph-captured.
     {
@@ -563,13 +564,13 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
                        q.tstart, q.tlen, q.ntiles);
     hipLaunchKernelGGL(k_scatter, dim3(gq), dim3(256), 0, s, count, q.qkey, q.qrank, q.qu, q.qv, q.keyoff, q.su, q.sv,
                        q.sdest);
-    const int grid = contract_grid(ctx, max_tiles);
+    const int grid = contract_grid(ctx, max_tiles, mode, tile);
     if (mode == CDET_MODE_EXACT) {
         hipLaunchKernelGGL(k_contract_exact, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, ctx->ks, q.ntiles,
                            q.tkey, q.tstart, q.tlen, q.su, q.sv, q.sdest, q.gup, q.gdn);
     } else {
         int rc = contract_launch_mfma(ctx->ks, mode, static_cast<int>(grid), q.ntiles, q.tkey, q.tstart, q.tlen, q.su,
-                                      q.sv, q.sdest, q.gup, q.gdn, s);
+                                      q.sv, q.sdest, q.gup, q.gdn, s, tile);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_interleave, dim3(gq), dim3(256), 0, s, count, q.gup, q.gdn, out);

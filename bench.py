@@ -65,6 +65,7 @@ def parse():
                          "default: the tuning table's, else 3")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore the workload's tuning table (tuning/<workload>.json) and use built-in launch shapes")
+    ap.add_argument("--no-split32", action="store_true", help="c2: skip the split-FP32 rate and error report")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
@@ -72,6 +73,8 @@ def parse():
                          "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice; "
                          "c5: 1024 configurations at orders 2..14 on the 10x10 lattice")
     ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
+    ap.add_argument("--no-exact-sweeps", action="store_true",
+                    help="c4: skip the same sweeps with the bit-identical FP64 form (the 2-sigma pass criterion)")
     ap.add_argument("--sweep-groups", type=int, default=1, choices=[1, 2],
                     help="c4/c5 sweeps: chain groups in flight (1: one batch per sweep; 2: two groups on the two "
                          "submit/wait slots)")
@@ -454,8 +457,10 @@ def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):
     sub = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p) if groups == 2 else None
     wt = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p) if groups == 2 else None
     t = time.perf_counter()
-    _, ratios, _ = mcmcmc.run_backend(c, fn, ctx.handle, envelopes=envelopes, submit_fn=sub, wait_fn=wt)
-    return time.perf_counter() - t, ratios, warm
+    results, ratios, _ = mcmcmc.run_backend(c, fn, ctx.handle, envelopes=envelopes, submit_fn=sub, wait_fn=wt)
+    dt = time.perf_counter() - t
+    run_sweeps.last_stats = mcmcmc.cross_chain_stats(results, 2, order_max)
+    return dt, ratios, warm
 
 
 def main_c4(args):
@@ -533,7 +538,29 @@ def main_c4(args):
         cpu = cpu_baseline_configs(w, params, tabs, cfgs, args.cpu_seconds, per_unit=len(cfgs), unit="sweeps/s",
                                    what=f"the {len(cfgs)} chain envelopes (orders max(3, E_c))")
     dt, ratios, warm = run_sweeps(ctx, w, orders, args.sweeps, E - 1, args.sweep_groups)
+    stats_mode = run_sweeps.last_stats
     sweeps = args.sweeps
+    # pass criterion: the same sweeps with the bit-identical FP64 form; the
+    # per-order cross-chain means (chain means as CLT samples) agree within 2
+    # sigma
+    criterion = None
+    if mode != "exact" and not args.no_exact_sweeps:
+        print("c4: the same sweeps with the bit-identical FP64 form", file=sys.stderr, flush=True)
+        cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES["exact"])
+        dt_exact, ratios_exact, _ = run_sweeps(ctx, w, orders, args.sweeps, E - 1, args.sweep_groups)
+        stats_exact = run_sweeps.last_stats
+        cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[mode])
+        agree = mcmcmc.agree_within(stats_mode, stats_exact, 2.0)
+        criterion = {
+            "rule": "per order |mean_mfma64 - mean_fp64| <= 2 sqrt(se_mfma64^2 + se_fp64^2), cross-chain means "
+                    "of the per-chain order-signal means (include/cdet/mcmcmc_sampler.h:1117-1131)",
+            "sweeps_each": sweeps, "passed": all(a for _, a in agree.values()),
+            "orders": {str(o): {"mean_mfma64": stats_mode[o][0], "se_mfma64": stats_mode[o][1],
+                                "mean_fp64": stats_exact[o][0], "se_fp64": stats_exact[o][1],
+                                "z": round(agree[o][0], 4)} for o in sorted(agree)},
+            "exact_fp64_sweeps_per_s": round(sweeps / dt_exact, 2),
+            "ratio_order4_over_order2_fp64": (float(ratios_exact[4]) if len(ratios_exact) > 4
+                                              and np.isfinite(ratios_exact[4]) else None)}
     line = {
         "metric": "C4 MCMCMC sweeps/s (6x6, beta=8, Nc=128, 256 chains, envelopes 2..12, lockstep batches)",
         "value": round(sweeps / dt, 2), "unit": "sweeps/s", "n_gpus": 1, "steps": sweeps, "warmup": warm,
@@ -553,6 +580,7 @@ def main_c4(args):
         "gpu_batch_ms": round(batch_ms, 4),
         # odd orders vanish at half filling; order 4 is the first nonzero ratio
         "ratio_order4_over_order2": (float(ratios[4]) if len(ratios) > 4 and np.isfinite(ratios[4]) else None),
+        "pass_criterion_vs_fp64": criterion,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
@@ -762,6 +790,14 @@ def main():
         exact_value = count * args.steps / timed(inflight, eplans)
         for ep in eplans:
             ep.close()
+    # the split-FP32 matrix-core form (hi/lo FP32 coefficients and basis, three
+    # FP32 products per K step, FP64 accumulation), batched the same way
+    split_value = None
+    if args.mode != "split32" and not args.no_split32:
+        splans = [ctx.plan([w["order"]] * count, mode="split32") for _ in range(inflight)]
+        split_value = count * args.steps / timed(inflight, splans)
+        for sp in splans:
+            sp.close()
     for extra in plans[1:]:
         extra.close()
     status_bad = int(np.count_nonzero(st & 1))
@@ -794,6 +830,10 @@ def main():
                                      range(2, w["order"] + 1) if not args.no_per_order else [w["order"]])
     if ctx_ulp is not None:
         ctx_ulp.close()
+    split_err = None
+    if args.mode != "split32" and not args.no_split32:
+        split_err = errors_per_order(ctx, None, orders, verts, "split32", dev, sptr,
+                                     range(2, w["order"] + 1) if not args.no_per_order else [w["order"]])
     ref = run_values(ctx, d_verts, [w["order"]] * count, "exact", 0, dev, sptr)
     bound = run_values(ctx, d_verts, [w["order"]] * count, "exact", 3, dev, sptr)
     errors = bound_metrics(got, ref, bound)
@@ -839,6 +879,12 @@ def main():
         "exact_fp64_batched_speedup_vs_plain": round((exact_value or value) / plain, 2),
         "errors_vs_fp64_exact": errors,
         "errors_per_order": per_order_err,
+        "split32": None if split_value is None else {
+            "evals_per_s": round(split_value, 1),
+            "note": "FP32 hi/lo coefficients and basis, three FP32 MFMA products per K step added in FP64; the "
+                    "FP32 matrix peak (157.3 TF/s) is only twice the FP64 one, and FP32 products leave errors far "
+                    "above the FP64 first-order bound (errors_per_order)",
+            "errors_per_order": split_err},
         "nonfinite_configs": status_bad,
         "fpm_fallback_configs": fallbacks,
     }

@@ -320,6 +320,41 @@ def integrand_scalars_from_params(params):
     return scale, float(a_dn / U), float(a_up / U)
 
 
+def cross_chain_stats(results, order_min, order_max):
+    """Cross-chain statistics of each order's signal: every chain's mean is one
+    independent CLT sample (include/cdet/mcmcmc_sampler.h:1117-1131, with the
+    running mean and M2 of RunningStatsLongDouble, :31-78).  Returns
+    {order: (mean, stderr, chains)}; stderr is NaN below two chains."""
+    out = {}
+    for order in range(order_min, order_max + 1):
+        count, mean, m2 = 0, 0.0, 0.0
+        for r in results:
+            if r.order_signal_count[order] > 0:
+                count += 1
+                x = float(r.order_signal_mean[order])
+                delta = x - mean
+                mean += delta / count
+                m2 += delta * (x - mean)
+        se = float(np.sqrt(m2 / (count - 1) / count)) if count >= 2 else float("nan")
+        out[order] = (mean, se, count)
+    return out
+
+
+def agree_within(stats_a, stats_b, sigmas=2.0):
+    """Per order: |mean_a - mean_b| <= sigmas * sqrt(se_a^2 + se_b^2) (orders
+    whose signal is identically zero in both count as agreeing).  Returns
+    {order: (difference / combined stderr, agrees)}."""
+    out = {}
+    for order in stats_a:
+        ma, sa, _ = stats_a[order]
+        mb, sb, _ = stats_b[order]
+        se = float(np.hypot(sa, sb))
+        d = abs(ma - mb)
+        z = d / se if se > 0 else (0.0 if d == 0 else float("inf"))
+        out[order] = (z, bool(z <= sigmas))
+    return out
+
+
 def run(config, eval_fn, eval_user, trace_steps=0):
     """Run the lockstep driver; eval_fn is a ctypes function pointer (or an
     address) of cdet_batch_eval_fn type."""

@@ -239,3 +239,29 @@ def test_driver_two_groups_forked_slot_plans(gpu_available, tmp_path):
     assert np.array_equal(ts, ta)
     assert np.array_equal(rats, rata, equal_nan=True)
     ctx.close()
+
+
+def test_c4_sweeps_mfma64_agree_with_fp64_within_two_sigma(gpu_available):
+    # The C4 pass criterion: 10^4 lockstep sweeps of the 256 chains of the 6x6
+    # workload (envelopes max(3, 2 + floor(11 u^2))) with the FP64 matrix-core
+    # contraction and with the bit-identical FP64 form; per order, the
+    # cross-chain means of the chains' order-signal means (the reference's CLT
+    # samples, include/cdet/mcmcmc_sampler.h:1117-1131) agree within 2 sigma.
+    import bench
+    import workloads
+    w = workloads.C4
+    orders, _ = workloads.configs("c4")
+    print("c4: building tables", flush=True)
+    ctx = workloads.context("c4")
+    stats = {}
+    for mode in ("mfma64", "exact"):
+        print(f"c4: 10^4 sweeps, {mode}", flush=True)
+        cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[mode])
+        bench.run_sweeps(ctx, w, orders, 10000, w["envelope"] - 1)
+        stats[mode] = bench.run_sweeps.last_stats
+    ag = mcmcmc.agree_within(stats["mfma64"], stats["exact"], 2.0)
+    print("c4: z per order", {o: round(z, 3) for o, (z, _) in ag.items()}, flush=True)
+    assert all(a for _, a in ag.values()), ag
+    # the even orders carry signal (odd orders vanish at half filling)
+    assert stats["exact"][2][0] != 0.0 and stats["exact"][4][0] != 0.0
+    ctx.close()

@@ -250,3 +250,25 @@ def test_load_data_dispatch_without_gpu():
         mcmcmc.load_data_from_params(Ctx(), dict(base, MCMCMC_TABLE_BACKEND=["disk_lru"]), environ={})
     with pytest.raises(RuntimeError, match="can only run with LOAD_DATA_WORKFLOW=resident or load_chunks"):
         mcmcmc.load_data_from_params(Ctx(), dict(base, LOAD_DATA_WORKFLOW=["sample"]), environ={})
+
+
+def test_cross_chain_stats_are_chain_means_as_clt_samples():
+    # include/cdet/mcmcmc_sampler.h:1117-1131: one sample per chain with
+    # signal at the order; running mean and M2 as RunningStatsLongDouble
+    r = np.random.default_rng(3)
+    res = (mcmcmc.ChainResult * 5)()
+    means = r.normal(size=(5, 4))
+    for c in range(5):
+        for o in range(2, 6):
+            res[c].order_signal_mean[o] = means[c, o - 2]
+            res[c].order_signal_count[o] = 0 if (c == 4 and o == 5) else 10
+    st = mcmcmc.cross_chain_stats(list(res), 2, 5)
+    for o in range(2, 6):
+        x = means[:, o - 2] if o != 5 else means[:4, 3]
+        m, se, n = st[o]
+        assert n == len(x) and abs(m - x.mean()) < 1e-14
+        assert abs(se - x.std(ddof=1) / np.sqrt(len(x))) < 1e-14
+    ag = mcmcmc.agree_within(st, {o: (st[o][0] + 1.5 * st[o][1], st[o][1], st[o][2]) for o in st})
+    assert all(a for _, a in ag.values())  # 1.5 se apart, combined se = sqrt(2) se: z ~ 1.06
+    ag = mcmcmc.agree_within(st, {o: (st[o][0] + 3.0 * st[o][1], st[o][1], st[o][2]) for o in st})
+    assert not any(a for _, a in ag.values())  # z ~ 2.12

@@ -73,6 +73,8 @@ def parse():
                          "c4: batched MCMCMC sweeps of 256 chains on the 6x6 lattice; "
                          "c5: 1024 configurations at orders 2..14 on the 10x10 lattice")
     ap.add_argument("--sweeps", type=int, default=10000, help="c4: lockstep sweeps (warmup + production)")
+    ap.add_argument("--no-pair-reuse", action="store_true",
+                    help="c4/c5 sweeps: evaluate every Green pair each step (no reuse of retained vertices' pairs)")
     ap.add_argument("--no-exact-sweeps", action="store_true",
                     help="c4: skip the same sweeps with the bit-identical FP64 form (the 2-sigma pass criterion)")
     ap.add_argument("--sweep-groups", type=int, default=1, choices=[1, 2],
@@ -453,6 +455,8 @@ def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):
     c.production_steps, c.chain_count, c.base_seed = sweeps - warm, len(envelopes), w["base_seed"]
     c.seed_mode, c.checkpoint_enabled, c.resume_mode = 0, 0, 0
     L = cdet_amd.lib()
+    L.cdet_ctx_set_pair_reuse(ctx.handle, 0 if getattr(run_sweeps, "no_reuse", False) else 1)
+    ctx.reset_counters()
     fn = ctypes.cast(L.cdet_eval_batch_host, ctypes.c_void_p)
     sub = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p) if groups == 2 else None
     wt = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p) if groups == 2 else None
@@ -460,6 +464,11 @@ def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):
     results, ratios, _ = mcmcmc.run_backend(c, fn, ctx.handle, envelopes=envelopes, submit_fn=sub, wait_fn=wt)
     dt = time.perf_counter() - t
     run_sweeps.last_stats = mcmcmc.cross_chain_stats(results, 2, order_max)
+    cnt = ctx.counters()
+    looked = cnt["green_pair_cache_hits"] + cnt["green_pair_cache_misses"]
+    run_sweeps.last_reuse = {"reused_pairs": int(cnt["green_pair_cache_hits"]),
+                             "looked_up_pairs": int(cnt["green_pair_cache_misses"]),
+                             "reused_fraction": round(cnt["green_pair_cache_hits"] / looked, 4) if looked else 0.0}
     return dt, ratios, warm
 
 
@@ -537,8 +546,10 @@ def main_c4(args):
         tabs = block_view_tables(ctx, cfgs)
         cpu = cpu_baseline_configs(w, params, tabs, cfgs, args.cpu_seconds, per_unit=len(cfgs), unit="sweeps/s",
                                    what=f"the {len(cfgs)} chain envelopes (orders max(3, E_c))")
+    run_sweeps.no_reuse = args.no_pair_reuse
     dt, ratios, warm = run_sweeps(ctx, w, orders, args.sweeps, E - 1, args.sweep_groups)
     stats_mode = run_sweeps.last_stats
+    reuse_rec = run_sweeps.last_reuse
     sweeps = args.sweeps
     # pass criterion: the same sweeps with the bit-identical FP64 form; the
     # per-order cross-chain means (chain means as CLT samples) agree within 2
@@ -581,6 +592,7 @@ def main_c4(args):
         # odd orders vanish at half filling; order 4 is the first nonzero ratio
         "ratio_order4_over_order2": (float(ratios[4]) if len(ratios) > 4 and np.isfinite(ratios[4]) else None),
         "pass_criterion_vs_fp64": criterion,
+        "green_pair_reuse": reuse_rec,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
@@ -674,11 +686,13 @@ def main_c5(args):
     # lockstep sweeps of 1024 chains at envelopes E_c (orders 1..E_c-1)
     ctx_mode = "mfma64"
     cdet_amd.lib().cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES[ctx_mode])
-    sweep_s, sweep_ratios = None, None
+    sweep_s, sweep_ratios, c5_reuse = None, None, None
     if args.c5_sweeps > 0:
         print(f"c5: {args.c5_sweeps} sweeps of {w['configs']} chains", file=sys.stderr, flush=True)
+        run_sweeps.no_reuse = args.no_pair_reuse
         sweep_s, sweep_ratios, _ = run_sweeps(ctx, w, orders, args.c5_sweeps, w["order_max"] - 1,
                                                  args.sweep_groups)
+        c5_reuse = run_sweeps.last_reuse
     # one lookup reads one nu x nv coefficient block per table (tables are
     # shared by the two spins here); upper bound on the contraction's bytes
     block = w["nc"] * w["nc"] * 8
@@ -702,6 +716,7 @@ def main_c5(args):
         "errors_per_order": per_order,
         "sweeps_per_s": round(args.c5_sweeps / sweep_s, 2) if sweep_s else None, "sweeps": args.c5_sweeps,
         "sweep_groups": args.sweep_groups,
+        "sweep_green_pair_reuse": c5_reuse,
         "sweep_ratio_order4_over_order2": (float(sweep_ratios[4]) if sweep_ratios is not None
                                            and np.isfinite(sweep_ratios[4]) else None),
     }

@@ -167,6 +167,8 @@ def _signatures():
         "cdet_batch_query_count": (i64, [vp]),
         "cdet_batch_plan_streams": (i, [vp]),
         "cdet_batch_run": (i, [vp, vp, vp, vp, i, vp]),
+        "cdet_batch_plan_reuse_pairs": (i, [vp, i]),
+        "cdet_ctx_set_pair_reuse": (i, [vp, i]),
         "cdet_batch_profile": (i, [vp, vp, vp, vp, vp, P(ctypes.c_float)]),
         "cdet_eval_batch": (i, [vp, vp, P(ctypes.c_int32), i64, vp, vp, i, vp]),
         "cdet_green_pair_eval": (i, [vp, vp, vp, i64, vp, i, vp]),
@@ -419,6 +421,10 @@ class BatchPlan:
         _check(lib().cdet_batch_profile(self._h, self._ptr(d_vertices), self._ptr(d_out), self._ptr(d_status),
                                         self._ptr(stream), ms))
         return tuple(ms)
+
+    def reuse_pairs(self, enable=True):
+        """Green-pair reuse across this plan's runs (cdet_batch_plan_reuse_pairs)."""
+        _check(lib().cdet_batch_plan_reuse_pairs(self._h, 1 if enable else 0))
 
     def run(self, d_vertices, d_out, d_status=None, use_graph=False, stream=None):
         _check(lib().cdet_batch_run(self._h, self._ptr(d_vertices), self._ptr(d_out), self._ptr(d_status),

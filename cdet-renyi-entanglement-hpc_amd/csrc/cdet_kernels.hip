@@ -125,6 +125,17 @@ __global__ __launch_bounds__(64) void k_pairs_plain(KScalars ks, const double* _
 // Sub-counters per coefficient block in the lookup histogram.
 constexpr int kKeySub = 16;
 
+// Green-pair reuse (reuse != 0): prev_verts holds the configuration's
+// vertices of the plan's previous run (valid where prev_ok[c] is set), and
+// g_up / g_dn still hold that run's Green values.  As the reference's pair
+// cache (include/cdet/integrand_replica_randomly_sampled.h:57-86, :664-696),
+// a vertex equal to a previous one (all four coordinates, bit for bit) takes
+// its previous index, and a pair of two such vertices copies its previous
+// value instead of being looked up: the value is the one the lookup computes,
+// since every evaluation form gives a lookup the same result whatever its
+// tile-mates.  cache[0] / cache[1] count reused and looked-up pairs.
+constexpr int kPlanPairsPerThread = (2 * kMaxOrder * kMaxOrder + 255) / 256;
+
 __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restrict__ verts,
                                               const int32_t* __restrict__ cfg_order,
                                               const int64_t* __restrict__ cfg_vbase,
@@ -133,8 +144,12 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
                                               int32_t* __restrict__ q_rank, double* __restrict__ q_u,
                                               double* __restrict__ q_v, int32_t* __restrict__ hist,
                                               int32_t* __restrict__ status, double* __restrict__ e_up,
-                                              double* __restrict__ e_dn, double err_scale) {
+                                              double* __restrict__ e_dn, double err_scale, int reuse,
+                                              double* __restrict__ prev_verts, int32_t* __restrict__ prev_ok,
+                                              unsigned long long* __restrict__ cache) {
     __shared__ Vtx vs[2][kMaxOrder];
+    __shared__ int pidx[kMaxOrder];
+    __shared__ int s_hits, s_misses;
     const int c = blockIdx.x;
     const int n = cfg_order[c];
     if (status && threadIdx.x == 0) status[c] = 0;
@@ -146,9 +161,56 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
         const int v = threadIdx.x - cls * n;
         vs[cls][v] = decode_vertex(verts + 4 * (vb + v), ks.beta, ks.N1, ks.N2, cls == 0 ? 2 : 1);
     }
+    const bool have_prev = reuse && prev_ok[c] != 0;
+    if (reuse && threadIdx.x < n) {
+        int found = -1;
+        if (have_prev) {
+            const double* cur = verts + 4 * (vb + threadIdx.x);
+            for (int k = 0; k < n && found < 0; ++k) {
+                const double* pv = prev_verts + 4 * (vb + k);
+                if (__double_as_longlong(cur[0]) == __double_as_longlong(pv[0]) &&
+                    __double_as_longlong(cur[1]) == __double_as_longlong(pv[1]) &&
+                    __double_as_longlong(cur[2]) == __double_as_longlong(pv[2]) &&
+                    __double_as_longlong(cur[3]) == __double_as_longlong(pv[3]))
+                    found = k;
+            }
+        }
+        pidx[threadIdx.x] = found;
+    }
+    if (reuse && threadIdx.x == 0) {
+        s_hits = 0;
+        s_misses = 0;
+    }
     __syncthreads();
     const int n2 = n * n;
-    for (int t = threadIdx.x; t < 2 * n2; t += blockDim.x) {
+    // previous values of the reusable pairs, read before any pair is written
+    double keep_up[kPlanPairsPerThread], keep_dn[kPlanPairsPerThread];
+    bool keep[kPlanPairsPerThread];
+    if (reuse) {
+#pragma unroll
+        for (int r = 0; r < kPlanPairsPerThread; ++r) {
+            const int t = threadIdx.x + r * 256;
+            keep[r] = false;
+            if (t < 2 * n2) {
+                const int cls = t / n2;
+                const int idx = t - cls * n2;
+                const int i = idx / n;
+                const int j = idx - i * n;
+                if (pidx[i] >= 0 && pidx[j] >= 0) {
+                    const int64_t src = qb + static_cast<int64_t>(cls) * n2 + pidx[i] * n + pidx[j];
+                    keep[r] = true;
+                    keep_up[r] = g_up[src];
+                    keep_dn[r] = g_dn[src];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    int hits = 0, misses = 0;
+#pragma unroll
+    for (int r = 0; r < kPlanPairsPerThread; ++r) {
+        const int t = threadIdx.x + r * 256;
+        if (t >= 2 * n2) break;
         const int cls = t / n2;
         const int idx = t - cls * n2;
         const int i = idx / n;
@@ -160,6 +222,16 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
         const int ku = cls == 0 ? CDET_TABLE_TWISTED_UP : CDET_TABLE_ORDINARY_UP;
         const int br = upper ? 0 : 1;
         const int64_t q = qb + t;
+        if (reuse) {
+            if (keep[r]) {
+                g_up[q] = keep_up[r];
+                g_dn[q] = keep_dn[r];
+                q_key[q] = -1;
+                ++hits;
+                continue;
+            }
+            if (!d.corner) ++misses;
+        }
         if (d.corner) {
             g_up[q] = ks.corner[ku][br][d.sitepair];
             g_dn[q] = ks.corner[ku + 1][br][d.sitepair];
@@ -180,6 +252,21 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
         q_rank[q] = atomicAdd(&hist[slot], 1);
         q_u[q] = d.u;
         q_v[q] = d.v;
+    }
+    if (reuse) {
+        // this run's vertices become the next run's previous ones (every
+        // read of prev_verts above precedes the barrier)
+        if (hits) atomicAdd(&s_hits, hits);
+        if (misses) atomicAdd(&s_misses, misses);
+        for (int k = threadIdx.x; k < 4 * n; k += blockDim.x) prev_verts[4 * vb + k] = verts[4 * vb + k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            prev_ok[c] = 1;
+            if (cache) {
+                atomicAdd(&cache[0], static_cast<unsigned long long>(s_hits));
+                atomicAdd(&cache[1], static_cast<unsigned long long>(s_misses));
+            }
+        }
     }
 }
 
@@ -1050,6 +1137,7 @@ struct cdet_ctx {
     uint16_t* d_masks = nullptr;  // masks by popcount for every order, (kMaxOrder+1) * 2^kMaxOrder
     int* d_popoff = nullptr;      // (kMaxOrder+1) * (kMaxOrder+2)
     uint64_t* d_fallbacks = nullptr;
+    unsigned long long* d_pair_cache = nullptr;  // Green-pair reuse: [0] reused, [1] looked up
     std::atomic<uint64_t> calls{0};  // counted from every calling thread
     int32_t* d_one = nullptr;   // cfg list {0}
     int64_t* d_zero64 = nullptr;  // pbase/obase {0}
@@ -1066,6 +1154,8 @@ struct cdet_ctx {
     std::mutex derive_mu;
     // launch-shape choices in effect (cdet_ctx_set_tuning)
     cdet_tuning tuning{};
+    // Green-pair reuse of the driver's batch plans (cdet_ctx_set_pair_reuse)
+    int pair_reuse = 1;
     // device address of this device's copy of `scratch` (no-allocation entry point)
     unsigned char* noalloc_scratch = nullptr;
     // host-buffer batch path (driver callback)
@@ -1756,6 +1846,8 @@ static int ctx_init(cdet_ctx* ctx) {
     HIP_TRY(hipMemcpy(ctx->d_popoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&ctx->d_fallbacks, sizeof(uint64_t)));
     HIP_TRY(hipMemset(ctx->d_fallbacks, 0, sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&ctx->d_pair_cache, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(ctx->d_pair_cache, 0, 2 * sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&ctx->d_one, sizeof(int32_t)));
     HIP_TRY(hipMalloc(&ctx->d_zero64, sizeof(int64_t)));
     HIP_TRY(hipMemset(ctx->d_one, 0, sizeof(int32_t)));
@@ -1845,6 +1937,7 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
     (void)hipFree(ctx->d_masks);
     (void)hipFree(ctx->d_popoff);
     (void)hipFree(ctx->d_fallbacks);
+    (void)hipFree(ctx->d_pair_cache);
     (void)hipFree(ctx->d_one);
     (void)hipFree(ctx->d_zero64);
     for (PlainWork* w : ctx->plain_all) plain_work_free(w);
@@ -2160,12 +2253,14 @@ int cdet_ctx_get_tuning(cdet_ctx* ctx, cdet_tuning* out) {
 int cdet_get_counters(cdet_ctx* ctx, cdet_counters* out) {
     if (!ctx || !out) return fail(CDET_ERR_ARGUMENT, "null argument");
     uint64_t fb = 0;
+    unsigned long long pc[2] = {0, 0};
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(&fb, ctx->d_fallbacks, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pc, ctx->d_pair_cache, sizeof(pc), hipMemcpyDeviceToHost));
     out->integrand_calls = ctx->calls;
     out->fpm_fallbacks = fb;
-    out->green_pair_cache_hits = 0;
-    out->green_pair_cache_misses = 0;
+    out->green_pair_cache_hits = pc[0];
+    out->green_pair_cache_misses = pc[1];
     return 0;
 }
 
@@ -2173,6 +2268,7 @@ int cdet_reset_counters(cdet_ctx* ctx) {
     if (!ctx) return fail(CDET_ERR_ARGUMENT, "null argument");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemset(ctx->d_fallbacks, 0, sizeof(uint64_t)));
+    HIP_TRY(hipMemset(ctx->d_pair_cache, 0, 2 * sizeof(unsigned long long)));
     ctx->calls = 0;
     return 0;
 }

@@ -46,6 +46,11 @@ struct cdet_batch_plan {
     int32_t* d_fb_count = nullptr;  // one queue length per order
     double* d_eup = nullptr;        // error-bound kind: table-value error scale per Green pair
     double* d_edn = nullptr;
+    // Green-pair reuse across runs (cdet_batch_plan_reuse_pairs)
+    int reuse = 0;
+    double* d_prev_verts = nullptr;  // vertices of the previous run, same packing
+    int32_t* d_prev_ok = nullptr;    // per configuration: previous vertices and values valid
+    uint64_t reuse_tables = 0;       // table generation the previous values belong to
     Scratch scratch;                // minors global scratch of lane 0, sized at creation
     int contract_grid = 0;
     // Order buckets spread over streams: bucket n runs on lane lane_of[n]
@@ -113,7 +118,8 @@ static void plan_free(cdet_batch_plan* p) {
     void* bufs[] = {p->d_order, p->d_vbase, p->d_pbase, p->d_obase, p->d_lists, p->d_gup, p->d_gdn, p->d_qkey,
                     p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
                     p->d_ntiles, p->d_su, p->d_sv, p->d_sdest, p->d_out_ord, p->d_single, p->d_fb_list,
-                    p->d_fb_count, p->d_wgtot, p->scratch.ptr, p->d_eup, p->d_edn};
+                    p->d_fb_count, p->d_wgtot, p->scratch.ptr, p->d_eup, p->d_edn, p->d_prev_verts,
+                    p->d_prev_ok};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -227,7 +233,8 @@ static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, int32_
     hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
                        reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
                        p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist, d_status, p->d_eup, p->d_edn,
-                       static_cast<double>(ctx->ks.nu + ctx->ks.nv) * kUnitRoundoff);
+                       static_cast<double>(ctx->ks.nu + ctx->ks.nv) * kUnitRoundoff, p->reuse, p->d_prev_verts,
+                       p->d_prev_ok, ctx->d_pair_cache);
     const unsigned kwg = static_cast<unsigned>((p->nkeys + kScanKeys - 1) / kScanKeys);
     hipLaunchKernelGGL(k_key_totals, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot);
     hipLaunchKernelGGL(k_scan_keys, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot,
@@ -244,6 +251,17 @@ static int plan_launch_sequence(cdet_batch_plan* p, const cdet_vertex* d_vertice
     rc = plan_minors(p, d_out, d_status, s);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Green-pair reuse: values of an earlier table generation are not reused
+// (a fill kernel on the run's stream clears the per-configuration flags).
+static int plan_reuse_guard(cdet_batch_plan* p, hipStream_t s) {
+    if (!p->reuse || p->reuse_tables == p->ctx->tables_gen) return 0;
+    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((p->count + 255) / 256)), dim3(256), 0, s, p->d_prev_ok,
+                       p->count, 0);
+    HIP_TRY(hipGetLastError());
+    p->reuse_tables = p->ctx->tables_gen;
     return 0;
 }
 
@@ -436,6 +454,28 @@ int cdet_batch_plan_destroy(cdet_batch_plan* plan) {
     return 0;
 }
 
+int cdet_batch_plan_reuse_pairs(cdet_batch_plan* p, int enable) {
+    if (!p) return fail(CDET_ERR_ARGUMENT, "cdet_batch_plan_reuse_pairs: null plan");
+    enable = enable ? 1 : 0;
+    if (p->eval_kind == 3) enable = 0;  // the error bounds need every pair's own error scale
+    if (enable == p->reuse) return 0;
+    if (p->ran) (void)hipEventSynchronize(p->done);
+    if (enable && !p->d_prev_verts) {
+        int rc = dev_alloc(&p->d_prev_verts, 4 * p->vert_total);
+        rc |= dev_alloc(&p->d_prev_ok, p->count);
+        if (rc) return CDET_ERR_DEVICE;
+    }
+    if (enable) HIP_TRY(hipMemset(p->d_prev_ok, 0, sizeof(int32_t) * static_cast<size_t>(p->count)));
+    p->reuse = enable;
+    p->reuse_tables = p->ctx->tables_gen;
+    // the captured graph holds the planning kernel's arguments: capture again
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    return 0;
+}
+
 int64_t cdet_batch_out_size(const cdet_batch_plan* plan) { return plan ? plan->out_total : -1; }
 int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan) { return plan ? plan->vert_total : -1; }
 int64_t cdet_batch_query_count(const cdet_batch_plan* plan) { return plan ? plan->nq : -1; }
@@ -447,6 +487,8 @@ int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_
     int trc = plan_tables(p);
     if (trc) return trc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
+    trc = plan_reuse_guard(p, s);
+    if (trc) return trc;
     if (!use_graph) {
         int rc = plan_launch_sequence(p, d_vertices, d_out, d_status, s);
         if (!rc) rc = plan_mark_done(p, s);
@@ -468,6 +510,8 @@ int cdet_batch_profile(cdet_batch_plan* p, const cdet_vertex* d_vertices, double
     int trc = plan_tables(p);
     if (trc) return trc;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->ctx->stream;
+    trc = plan_reuse_guard(p, s);
+    if (trc) return trc;
     hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], s));
@@ -586,6 +630,22 @@ extern "C" int cdet_tables_build(cdet_ctx* ctx, const cdet_lattice* lattice) {
 // ---------------------------------------------------------------------------
 // Host-buffer batch evaluation for the driver (cdet_batch_eval_fn).
 // ---------------------------------------------------------------------------
+extern "C" int cdet_ctx_set_pair_reuse(cdet_ctx* ctx, int enable) {
+    if (!ctx) return fail(CDET_ERR_ARGUMENT, "cdet_ctx_set_pair_reuse: null context");
+    ctx->pair_reuse = enable ? 1 : 0;
+    if (ctx->host_plan) {
+        int rc = cdet_batch_plan_reuse_pairs(ctx->host_plan, ctx->pair_reuse);
+        if (rc) return rc;
+    }
+    for (auto& sl : ctx->slots) {
+        if (!sl.plan) continue;
+        if (sl.pending) return fail(CDET_ERR_ARGUMENT, "cdet_ctx_set_pair_reuse: a submitted batch is pending");
+        int rc = cdet_batch_plan_reuse_pairs(sl.plan, ctx->pair_reuse);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 extern "C" int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode) {
     if (!ctx || mode < 0 || mode > 2) return fail(CDET_ERR_ARGUMENT, "cdet_ctx_set_batch_mode: bad argument");
     if (ctx->batch_mode != mode && ctx->host_plan) {
@@ -628,6 +688,10 @@ extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, c
         if (ctx->host_plan) cdet_batch_plan_destroy(ctx->host_plan);
         ctx->host_plan = nullptr;
         int rc = cdet_batch_plan_create(ctx, h_orders, count, ctx->batch_mode, 0, &ctx->host_plan);
+        if (rc) return rc;
+        // consecutive calls are steps of the same chains: reuse the Green
+        // pairs of retained vertices, as the reference's pair cache does
+        rc = cdet_batch_plan_reuse_pairs(ctx->host_plan, ctx->pair_reuse);
         if (rc) return rc;
         ctx->host_plan_orders.assign(h_orders, h_orders + count);
     }
@@ -717,6 +781,8 @@ extern "C" int cdet_eval_batch_submit(void* user, int slot, const cdet_vertex* h
         sl.plan = nullptr;
         int rc = plan_create(ctx, h_orders, count, ctx->batch_mode, 0, kSlotLanes, sl.lanes, &sl.plan);
         if (rc) return rc;
+        rc = cdet_batch_plan_reuse_pairs(sl.plan, ctx->pair_reuse);
+        if (rc) return rc;
         sl.orders.assign(h_orders, h_orders + count);
     }
     cdet_batch_plan* p = sl.plan;
@@ -733,6 +799,8 @@ extern "C" int cdet_eval_batch_submit(void* user, int slot, const cdet_vertex* h
     if (!rc) rc = grow_pinned(&sl.h_v, sl.h_v_cap, vbytes);
     if (!rc) rc = grow_pinned(&sl.h_out, sl.h_out_cap, obytes);
     if (!rc) rc = grow_pinned(&sl.h_status, sl.h_status_cap, sbytes);
+    if (rc) return rc;
+    rc = plan_reuse_guard(p, sl.stream);
     if (rc) return rc;
     // the graph is captured here, on the submitting thread, so that the slot
     // threads make only stream-ordered calls (copies, graph launches)

@@ -273,6 +273,19 @@ int64_t cdet_batch_query_count(const cdet_batch_plan* plan);  /* Green-pair look
 int cdet_batch_plan_streams(const cdet_batch_plan* plan);
 
 /*
+ * Green-pair reuse across the plan's runs (enable != 0; off by default):
+ * each configuration's vertices are compared, bit for bit, with those of the
+ * plan's previous run, and a Green pair of two vertices that were present
+ * (at any position) copies its previous value instead of being looked up
+ * again, as the reference's GreenPairMatrixCache does for consecutive calls
+ * on one thread (include/cdet/integrand_replica_randomly_sampled.h:57-86,
+ * :664-696).  The values are those a lookup computes, in every mode.  Values
+ * of an earlier table generation are never reused.  Ignored for the
+ * error-bound kind.  Counted in cdet_counters.green_pair_cache_hits/misses.
+ */
+int cdet_batch_plan_reuse_pairs(cdet_batch_plan* plan, int enable);
+
+/*
  * Run a plan on DEVICE buffers: d_vertices (cdet_vertex[vertex_count]),
  * d_out (double[out_size]), d_status (int32[count] or NULL: bit 0 = a
  * non-finite value, bit 1 = FPM fallback taken).  Asynchronous on `stream`

@@ -250,6 +250,12 @@ int cdet_eval_batch_submit(void* ctx, int slot, const cdet_vertex* h_vertices, c
                            int64_t count);
 int cdet_eval_batch_wait(void* ctx, int slot, double* h_out);
 int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode);
+/* Green-pair reuse of those plans (cdet_batch_plan_reuse_pairs), on by
+ * default: consecutive calls are steps of the same chains, and a chain's
+ * retained vertices keep their Green pairs, as the reference's pair cache
+ * does (include/cdet/integrand_replica_randomly_sampled.h:664-696).  Values
+ * are the same either way. */
+int cdet_ctx_set_pair_reuse(cdet_ctx* ctx, int enable);
 
 /* GPU implementation of cdet_batch_cond_fn over a context (user = cdet_ctx*):
  * one workgroup per envelope walks the popcount layers of the subset

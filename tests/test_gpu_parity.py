@@ -684,3 +684,80 @@ def test_tuning_choices_bit_identical(gpu_available):
     finally:
         ctx.set_tuning({})
     assert ctx.tuning() == default
+
+
+def test_green_pair_reuse_bit_identical(gpu_available):
+    # A plan with Green-pair reuse: the second run's configurations keep some
+    # of the first run's vertices (at other positions) and draw the rest anew;
+    # its values equal a fresh plan's and the oracle's bit for bit, the reused
+    # pairs are counted, and after a table change nothing stale is reused.
+    import torch
+    from oracle import oracle as O
+    name = "c1"
+    ctx = gpu_ctx(name)
+    orc = helpers.oracle_of(name)
+    orders = [7, 5, 8, 6, 9, 4] * 12
+    first = helpers.seeded_configs(9500, orders)
+    fresh = helpers.seeded_configs(9501, orders)
+    r = np.random.default_rng(95)
+    second = []
+    for a, b in zip(first, fresh):
+        keep = r.random(len(a)) < 0.6
+        v = np.where(keep[:, None], a, b)
+        second.append(v[r.permutation(len(v))].copy())
+    sptr = torch.cuda.current_stream().cuda_stream
+    for graph in (False, True):
+        plan = ctx.plan(orders, "mfma64" if graph else "exact")
+        plan.reuse_pairs(True)
+        out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+        ctx.reset_counters()
+        for cfgs in (first, second):
+            d_v = torch.tensor(np.concatenate(cfgs), dtype=torch.float64, device="cuda")
+            plan.run(d_v, out, None, use_graph=graph, stream=sptr)
+            torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        counters = ctx.counters()
+        say(f"pair reuse (graph={graph}): {counters['green_pair_cache_hits']} reused, "
+            f"{counters['green_pair_cache_misses']} looked up")
+        assert counters["green_pair_cache_hits"] > 0
+        (want,), ob, _ = run_batch(ctx, second, "mfma64" if graph else "exact")
+        assert bits_equal(got, want)
+        if not graph:
+            for k in range(0, len(second), 7):
+                assert bits_equal(got[ob[k]:ob[k + 1]], orc.all_subsets(second[k])), k
+        plan.close()
+    # table change between runs: the reusing plan reads the new tables only
+    s = helpers.SETS[name]
+    op = helpers.params_of(name)
+    c2 = cdet_amd.Context(cdet_amd.make_params(s["N1"], s["N2"], s["nc"], s["beta"], s["U"], s["mu"],
+                                               s["alpha_up"], s["alpha_down"],
+                                               derived=(op.scale, op.diag_up_shift, op.diag_down_shift)))
+    base = helpers.tables_of(name)
+    c2.upload_tables(base)
+    plan = c2.plan(orders, "exact")
+    plan.reuse_pairs(True)
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    d_v = torch.tensor(np.concatenate(first), dtype=torch.float64, device="cuda")
+    plan.run(d_v, out, None, use_graph=True, stream=sptr)
+    torch.cuda.synchronize()
+    made = {}
+    bent = []
+    for t in base:
+        if id(t) not in made:
+            made[id(t)] = O.Table(t.upper * (1.0 + 1e-3), t.lower * (1.0 - 1e-3), t.ucorner, t.lcorner)
+        bent.append(made[id(t)])
+    # the aliasing kinds take their own copies first (a kind another kind
+    # aliases cannot be uploaded), then the sources, then the aliases again
+    for k in (1, 3, 0, 2):
+        c2.upload_table(k, bent[k].upper, bent[k].lower, bent[k].ucorner, bent[k].lcorner)
+    c2.alias_table(1, 0)
+    c2.alias_table(3, 2)
+    plan.run(d_v, out, None, use_graph=True, stream=sptr)
+    torch.cuda.synchronize()
+    orc2 = O.Oracle(op, bent)
+    got = out.cpu().numpy()
+    _, ob = cdet_amd.packed_offsets(orders, False)
+    for k in range(0, len(first), 5):
+        assert bits_equal(got[ob[k]:ob[k + 1]], orc2.all_subsets(first[k])), k
+    plan.close()
+    c2.close()

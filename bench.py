@@ -191,6 +191,18 @@ def apply_tuning(ctx, workload, args):
             tab.get("inflight"))
 
 
+def plan_tile(plan):
+    """Lookups per contraction tile of a plan (16 or 64) as the library picks
+    it: 64 only when the batch has about three 64-lookup tiles per resident
+    wave or the tuning forces it."""
+    t = plan.ctx.tuning().get("contract_tile", 0)
+    if t:
+        return t
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return 64 if plan.query_count // 64 >= 3 * 16 * cus else 16
+
+
 def cpu_info():
     """(usable host cores, CPU model name)."""
     try:
@@ -527,7 +539,7 @@ def main_c4(args):
     blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"])
     alg_bytes = blocks * 8.0 * w["nc"] * w["nc"]
     achieved = alg_bytes / (stage[1] * 1e-3) / 1e9
-    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true>"}[mode]
+    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true, 1>"}[mode]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, "c4"),
                 "kernel": kname, "algorithmic_bytes_per_launch": alg_bytes, "distinct_blocks": blocks,
@@ -673,8 +685,8 @@ def main_c5(args):
     achieved = alg_bytes / (cms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles("cdet::k_contract_f64<true>", "c5"),
-                "kernel": "cdet::k_contract_f64<true>", "algorithmic_bytes_per_launch": alg_bytes,
+                "traffic": traffic_from_profiles("cdet::k_contract_f64<true, 4>", "c5"),
+                "kernel": "cdet::k_contract_f64<true, 4>", "algorithmic_bytes_per_launch": alg_bytes,
                 "distinct_blocks": blocks, "kernel_ms": round(cms, 4)}
     cpu = None
     if not args.no_cpu:
@@ -829,7 +841,8 @@ def main():
     flops = queries * (2.0 * nc * nc + 2.0 * nc)
     contract_s = stage[1] * 1e-3
     achieved = flops / contract_s / 1e12
-    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true>", "split32": "cdet::k_contract_mfma<2>"}[args.mode]
+    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true, " + ("1>" if plan_tile(plan) <= 16 else "4>"),
+             "split32": "cdet::k_contract_mfma<2>"}[args.mode]
     roofline = {"bound": "mfma" if args.mode != "exact" else "fp64-valu", "achieved": round(achieved, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                 "traffic": traffic_from_profiles(kname), "kernel": kname,

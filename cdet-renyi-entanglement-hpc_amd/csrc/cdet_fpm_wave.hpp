@@ -197,6 +197,11 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
         }
         wave_sync_lds();
         const int nitems = sets * nms;
+        // the last level (the lanes' level-L matrices) is stored element-major,
+        // element e of matrix s at e 2^L + s, so that each lane's matrix load
+        // below is one contiguous wave read per element (matrix-major, the
+        // lanes' loads would be 8 M^2 bytes apart and hit the same LDS banks)
+        const bool last = l == L - 1;
         for (int it = lane; it < nitems; it += 64) {
             const int s = it / nms;
             const int rem = it - s * nms;
@@ -205,8 +210,14 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
             const int src = s * oms;
             const double f = oldl[src + (r + 1) * os] * inv[s];
             const double base = oldl[src + (r + 1) * os + (cc + 1)];
-            newl[s * nms + rem] = base;
-            newl[(s + sets) * nms + rem] = __builtin_fma(-oldl[src + 1 + cc], f, base);
+            const double with = __builtin_fma(-oldl[src + 1 + cc], f, base);
+            if (last) {
+                newl[rem * (2 * sets) + s] = base;
+                newl[rem * (2 * sets) + s + sets] = with;
+            } else {
+                newl[s * nms + rem] = base;
+                newl[(s + sets) * nms + rem] = with;
+            }
         }
         wave_sync_lds();
         double* t = oldl;
@@ -218,7 +229,7 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
     double X[M * M];
     const int s = lane < T ? lane : 0;
 #pragma unroll
-    for (int i = 0; i < M * M; ++i) X[i] = oldl[s * M * M + i];
+    for (int i = 0; i < M * M; ++i) X[i] = oldl[i * T + s];
     fpm_lane_subtree<M>(X, p, fb_eps, piv, bad);
     if (lane >= T) bad = false;
     // running product over the low bits of s (final[s], masks < 2^L)

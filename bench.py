@@ -789,18 +789,20 @@ def main():
     plans = [plan] + [ctx.plan([w["order"]] * count, mode=args.mode) for _ in range(inflight - 1)]
     outs = [d_out] + [torch.empty(plan.out_size, dtype=torch.float64, device=dev) for _ in range(inflight - 1)]
     stats = [d_status] + [torch.zeros(count, dtype=torch.int32, device=dev) for _ in range(inflight - 1)]
-    streams = [stream] + [torch.cuda.Stream() for _ in range(inflight - 1)]
+    # one stream per batch in flight, each on a hardware queue of its own
+    # (ordinary streams share the device's few queues and then serialise)
+    streams = [ctx.stream_create() for _ in range(inflight)]
 
     def timed(k_inflight, plan_set=None):
         ps = plans if plan_set is None else plan_set
         for i in range(args.warmup):
             k = i % k_inflight
-            ps[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k].cuda_stream)
+            ps[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k])
         torch.cuda.synchronize()
         t = time.perf_counter()
         for i in range(args.steps):
             k = i % k_inflight
-            ps[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k].cuda_stream)
+            ps[k].run(d_verts, outs[k], stats[k], use_graph=True, stream=streams[k])
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
@@ -827,6 +829,9 @@ def main():
             sp.close()
     for extra in plans[1:]:
         extra.close()
+    torch.cuda.synchronize()
+    for s_ in streams:
+        ctx.stream_destroy(s_)
     status_bad = int(np.count_nonzero(st & 1))
     fallbacks = int(np.count_nonzero(st & 2))
 

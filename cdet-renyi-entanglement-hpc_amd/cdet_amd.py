@@ -152,6 +152,8 @@ def _signatures():
         "cdet_ctx_create": (i, [P(CdetParams), P(vp)]),
         "cdet_ctx_destroy": (i, [vp]),
         "cdet_ctx_stream": (vp, [vp]),
+        "cdet_stream_create": (i, [vp, P(vp)]),
+        "cdet_stream_destroy": (i, [vp, vp]),
         "cdet_tables_upload": (i, [vp, i, dp, dp, dp, dp]),
         "cdet_tables_alias": (i, [vp, i, i]),
         "cdet_tables_build": (i, [vp, P(CdetLattice)]),
@@ -267,6 +269,18 @@ class Context:
     @property
     def stream(self):
         return lib().cdet_ctx_stream(self._h)
+
+    def stream_create(self):
+        """A hipStream_t (int) on a hardware queue of its own over every CU
+        (include/cdet_hip.h cdet_stream_create): batches run concurrently
+        overlap only on such streams.  Destroy with stream_destroy before the
+        context closes."""
+        s = ctypes.c_void_p()
+        _check(lib().cdet_stream_create(self._h, ctypes.byref(s)))
+        return int(s.value)
+
+    def stream_destroy(self, stream):
+        _check(lib().cdet_stream_destroy(self._h, ctypes.c_void_p(int(stream))))
 
     # -- tables ------------------------------------------------------------
     def upload_table(self, kind, upper, lower, ucorner, lcorner):

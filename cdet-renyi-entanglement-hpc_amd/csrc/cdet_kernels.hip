@@ -1794,6 +1794,21 @@ static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles, int mode = CDET
 static int alloc_table(cdet_ctx* ctx, int kind);
 static void invalidate_split(DevTable& t);
 
+// A stream on a hardware queue of its own, over every CU of the device.  HIP
+// multiplexes ordinary streams over a small pool of hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default), so two streams of a process can share a
+// queue and then run their kernels one after the other; a stream made with a
+// CU mask is given a queue of its own.  Batches in flight and the order lanes
+// of a plan overlap only on such streams.  The stream synchronizes with the
+// null stream like a stream made by hipStreamCreate.
+static int queue_stream_create(cdet_ctx* ctx, hipStream_t* out) {
+    const int words = (ctx->ncu + 31) / 32;
+    std::vector<uint32_t> mask(static_cast<size_t>(words), 0u);
+    for (int cu = 0; cu < ctx->ncu; ++cu) mask[static_cast<size_t>(cu / 32)] |= 1u << (cu % 32);
+    HIP_TRY(hipExtStreamCreateWithCUMask(out, static_cast<uint32_t>(words), mask.data()));
+    return 0;
+}
+
 #include "cdet_producer.hpp"
 #include "cdet_plan.hpp"
 #include "cdet_chunks.hpp"
@@ -1836,7 +1851,10 @@ static int ctx_init(cdet_ctx* ctx) {
     if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
         ctx->ncu <= 0)
         ctx->ncu = 256;
-    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    {
+        const int rc = queue_stream_create(ctx, &ctx->stream);
+        if (rc) return rc;
+    }
     std::vector<uint16_t> masks;
     std::vector<int> off;
     masks_by_popcount(masks, off);
@@ -1973,6 +1991,22 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
 }
 
 void* cdet_ctx_stream(cdet_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int cdet_stream_create(cdet_ctx* ctx, void** out) {
+    if (!ctx || !out) return fail(CDET_ERR_ARGUMENT, "cdet_stream_create: bad argument");
+    *out = nullptr;
+    hipStream_t s = nullptr;
+    const int rc = queue_stream_create(ctx, &s);
+    if (rc) return rc;
+    *out = s;
+    return 0;
+}
+
+int cdet_stream_destroy(cdet_ctx* ctx, void* stream) {
+    if (!ctx || !stream) return fail(CDET_ERR_ARGUMENT, "cdet_stream_destroy: bad argument");
+    HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    return 0;
+}
 
 static int table_sizes(const cdet_ctx* ctx, size_t& ncoef, size_t& ncorner) {
     const size_t S = static_cast<size_t>(ctx->p.N1) * ctx->p.N2;

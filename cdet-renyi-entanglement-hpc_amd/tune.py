@@ -77,17 +77,24 @@ class Batch:
         self.orders = [int(x) for x in orders]
         self.mode = mode
         self.d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device="cuda")
-        self.streams = [torch.cuda.current_stream()] + [torch.cuda.Stream() for _ in range(3)]
+        # batches in flight run on streams of hardware queues of their own
+        self.streams = [ctx.stream_create() for _ in range(4)]
 
     def run_once(self):
         """Output of one eager run with the tuning in effect."""
         torch = self.torch
         plan = self.ctx.plan(self.orders, mode=self.mode)
         out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
-        plan.run(self.d_v, out, None, use_graph=False, stream=self.streams[0].cuda_stream)
+        plan.run(self.d_v, out, None, use_graph=False, stream=self.streams[0])
         torch.cuda.synchronize()
         plan.close()
         return out.cpu().numpy()
+
+    def close(self):
+        self.torch.cuda.synchronize()
+        for s in self.streams:
+            self.ctx.stream_destroy(s)
+        self.streams = []
 
     def rate(self, inflight, steps):
         """Configurations per second over `steps` batches."""
@@ -96,12 +103,12 @@ class Batch:
         outs = [torch.empty(plans[0].out_size, dtype=torch.float64, device="cuda") for _ in range(inflight)]
         for k in range(inflight):
             for _ in range(2):
-                plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k].cuda_stream)
+                plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k])
         torch.cuda.synchronize()
         t = time.perf_counter()
         for i in range(steps):
             k = i % inflight
-            plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k].cuda_stream)
+            plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
         for p in plans:
@@ -194,6 +201,7 @@ def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 
     tuned = batch.run_once()
     if not np.array_equal(tuned.view(np.int64), reference.view(np.int64)):
         raise RuntimeError("tuned launch shapes changed the values; table not written")
+    batch.close()
     ctx.close()
     props = torch.cuda.get_device_properties(0)
     return {"workload": workload, "mode": mode, "device": props.name, "cus": props.multi_processor_count,

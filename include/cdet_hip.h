@@ -353,6 +353,19 @@ int cdet_reset_counters(cdet_ctx* ctx);
 /* Context stream (hipStream_t) on which the library launches by default. */
 void* cdet_ctx_stream(cdet_ctx* ctx);
 
+/* A stream (hipStream_t) on a hardware queue of its own, over every CU of the
+ * context's device (call with that device current).  HIP multiplexes ordinary
+ * streams over a pool of GPU_MAX_HW_QUEUES hardware queues (4 by default), so
+ * two batches run on two ordinary streams can still execute one after the
+ * other; plans run concurrently overlap only on streams of their own queues.
+ * The context stream, the order lanes of a plan and the host slots' streams
+ * are made this way.  Like a hipStreamCreate stream it synchronizes with the
+ * null stream.  Destroy it with cdet_stream_destroy before the context.  No
+ * counterpart in the reference, whose chains are host threads
+ * (include/cdet/mcmcmc_sampler.h:1104-1110). */
+int cdet_stream_create(cdet_ctx* ctx, void** out_stream);
+int cdet_stream_destroy(cdet_ctx* ctx, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -185,6 +185,53 @@ def test_forked_order_buckets_bit_identical(gpu_available):
         single.close()
 
 
+def test_batches_in_flight_on_queue_streams(gpu_available):
+    # Three plans replayed round-robin on three streams of hardware queues of
+    # their own (cdet_stream_create), so their launches overlap on the
+    # device: every replay gives the values of the same batch run alone on
+    # the context stream, bit for bit, in both FP64 modes.
+    import torch
+    ctx = gpu_ctx("c1")
+    sets = [helpers.seeded_configs(9300 + k, orders)
+            for k, orders in enumerate([[8] * 40, [6, 7, 8, 9] * 10, [10] * 12])]
+    streams = [ctx.stream_create() for _ in sets]
+    assert len(set(streams)) == len(sets) and all(streams)
+    try:
+        for mode in ("exact", "mfma64"):
+            plans, verts, outs, alone = [], [], [], []
+            for configs in sets:
+                v = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
+                p = ctx.plan([len(c) for c in configs], mode)
+                o = torch.empty(p.out_size, dtype=torch.float64, device="cuda")
+                p.run(v, o, None, use_graph=False, stream=ctx.stream)
+                torch.cuda.synchronize()
+                alone.append(o.cpu().numpy().copy())
+                plans.append(p)
+                verts.append(v)
+                outs.append(o)
+            for r in range(3):
+                say(f"in flight on queue streams: {mode} round {r}")
+                for o in outs:
+                    o.fill_(float("nan"))
+                torch.cuda.synchronize()
+                for k, p in enumerate(plans):
+                    p.run(verts[k], outs[k], None, use_graph=True, stream=streams[k])
+                torch.cuda.synchronize()
+                for k in range(len(plans)):
+                    assert bits_equal(outs[k].cpu().numpy(), alone[k]), (mode, r, k)
+            for p in plans:
+                p.close()
+            if mode == "exact":
+                orc = helpers.oracle_of("c1")
+                assert bits_equal(alone[2].reshape(len(sets[2]), -1)[0], orc.all_subsets(sets[2][0]))
+    finally:
+        torch.cuda.synchronize()
+        for s in streams:
+            ctx.stream_destroy(s)
+    with pytest.raises(RuntimeError):
+        ctx.stream_destroy(0)
+
+
 def test_paired_classes_bit_identical(gpu_available):
     # A single-order batch with at least two configurations per CU runs both
     # classes of a configuration in one workgroup and combines them in place;

@@ -1798,9 +1798,13 @@ static void invalidate_split(DevTable& t);
 // multiplexes ordinary streams over a small pool of hardware queues
 // (GPU_MAX_HW_QUEUES, 4 by default), so two streams of a process can share a
 // queue and then run their kernels one after the other; a stream made with a
-// CU mask is given a queue of its own.  Batches in flight and the order lanes
-// of a plan overlap only on such streams.  The stream synchronizes with the
-// null stream like a stream made by hipStreamCreate.
+// CU mask is given a queue of its own.  Batches in flight overlap only on such
+// streams (order-10 batches of 3360 configurations, 4 in flight: 7.2 M
+// configurations/s against 5.5 M on ordinary streams).  The stream
+// synchronizes with the null stream like a stream made by hipStreamCreate.
+// Only the caller's in-flight streams are made this way: the context stream,
+// the host slots and the order lanes stay ordinary non-blocking streams,
+// which measured faster for the host-slot sweeps and the lane fork/join.
 static int queue_stream_create(cdet_ctx* ctx, hipStream_t* out) {
     const int words = (ctx->ncu + 31) / 32;
     std::vector<uint32_t> mask(static_cast<size_t>(words), 0u);
@@ -1851,10 +1855,7 @@ static int ctx_init(cdet_ctx* ctx) {
     if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
         ctx->ncu <= 0)
         ctx->ncu = 256;
-    {
-        const int rc = queue_stream_create(ctx, &ctx->stream);
-        if (rc) return rc;
-    }
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     std::vector<uint16_t> masks;
     std::vector<int> off;
     masks_by_popcount(masks, off);

@@ -20,7 +20,7 @@ that bench.py (and any caller, through Context.set_tuning) applies:
 
 The objective is the workload's throughput as bench.py measures it:
 configurations per second of graph-replayed batches, with `inflight` batches
-in flight for C2 (searched over 1..4) and one at a time for C4 (the batch of
+in flight for C2 (searched over 1..6) and one at a time for C4 (the batch of
 the 256 chain envelopes) and C5.  The search is coordinate descent from the built-in defaults,
 two passes, each candidate timed as the median of several runs and kept only
 for a gain above 2%.  Every choice
@@ -70,7 +70,7 @@ class Batch:
     """A workload's batch on the device, timed with `inflight` plans in
     flight (graph replays), as bench.py runs it."""
 
-    def __init__(self, ctx, orders, verts, mode):
+    def __init__(self, ctx, orders, verts, mode, max_inflight=6):
         import torch
         self.torch = torch
         self.ctx = ctx
@@ -78,7 +78,7 @@ class Batch:
         self.mode = mode
         self.d_v = torch.tensor(np.concatenate(verts), dtype=torch.float64, device="cuda")
         # batches in flight run on streams of hardware queues of their own
-        self.streams = [ctx.stream_create() for _ in range(4)]
+        self.streams = [ctx.stream_create() for _ in range(max_inflight)]
 
     def run_once(self):
         """Output of one eager run with the tuning in effect."""
@@ -96,24 +96,28 @@ class Batch:
             self.ctx.stream_destroy(s)
         self.streams = []
 
-    def rate(self, inflight, steps):
-        """Configurations per second over `steps` batches."""
+    def rate(self, inflight, steps, reps, warmup=5):
+        """Median over `reps` runs of configurations per second over `steps`
+        batches, each run after `warmup` untimed batches, on the same plans
+        (as bench.py times its steps)."""
         torch = self.torch
         plans = [self.ctx.plan(self.orders, mode=self.mode) for _ in range(inflight)]
         outs = [torch.empty(plans[0].out_size, dtype=torch.float64, device="cuda") for _ in range(inflight)]
-        for k in range(inflight):
-            for _ in range(2):
+        rates = []
+        for _ in range(reps):
+            for i in range(warmup):
+                k = i % inflight
                 plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k])
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for i in range(steps):
-            k = i % inflight
-            plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k])
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for i in range(steps):
+                k = i % inflight
+                plans[k].run(self.d_v, outs[k], None, use_graph=True, stream=self.streams[k])
+            torch.cuda.synchronize()
+            rates.append(len(self.orders) * steps / (time.perf_counter() - t))
         for p in plans:
             p.close()
-        return len(self.orders) * steps / dt
+        return sorted(rates)[reps // 2]
 
 
 def candidates(orders):
@@ -144,7 +148,7 @@ def apply(table, knob, value):
     return t
 
 
-def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 3, 4), log=print):
+def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 3, 4, 5, 6), log=print):
     import torch
     torch.cuda.set_device(0)
     ctx = workloads.context(workload)
@@ -152,12 +156,12 @@ def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 
     if workload == "c2":  # the headline batch: every order-10 configuration
         sel = [i for i in range(len(orders)) if orders[i] == workloads.C2["order"]]
         orders, verts = orders[sel], [verts[i] for i in sel]
-    batch = Batch(ctx, orders, verts, mode)
+    batch = Batch(ctx, orders, verts, mode, max(inflight_choices) if workload == "c2" else 1)
     # C2 is timed as the benchmark driver times it (20-step runs); the median
     # of `reps` runs is one measurement, and a change is kept only for a gain
     # above `gain`, well outside the run-to-run spread
     steps = steps or {"c2": 20, "c4": 40, "c5": 6}.get(workload, 20)
-    reps = {"c2": 7, "c4": 5, "c5": 3}.get(workload, 3)
+    reps = {"c2": 15, "c4": 5, "c5": 3}.get(workload, 3)
     gain = 1.02
     base = ctx.tuning()
     reference = batch.run_once()
@@ -165,8 +169,7 @@ def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 
 
     def measure(table, k_inflight):
         ctx.set_tuning(table)
-        r = sorted(batch.rate(k_inflight, steps) for _ in range(reps))
-        return r[reps // 2]
+        return batch.rate(k_inflight, steps, reps)
 
     best = dict(base)
     best_rate = measure(best, inflight)

@@ -358,8 +358,8 @@ void* cdet_ctx_stream(cdet_ctx* ctx);
  * streams over a pool of GPU_MAX_HW_QUEUES hardware queues (4 by default), so
  * two batches run on two ordinary streams can still execute one after the
  * other; plans run concurrently overlap only on streams of their own queues.
- * The context stream, the order lanes of a plan and the host slots' streams
- * are made this way.  Like a hipStreamCreate stream it synchronizes with the
+ * The library's own streams (context stream, host slots, order lanes) are
+ * ordinary non-blocking streams, which measured faster for what they carry.  Like a hipStreamCreate stream it synchronizes with the
  * null stream.  Destroy it with cdet_stream_destroy before the context.  No
  * counterpart in the reference, whose chains are host threads
  * (include/cdet/mcmcmc_sampler.h:1104-1110). */

@@ -203,6 +203,16 @@ def plan_tile(plan):
     return 64 if plan.query_count // 64 >= 3 * 16 * cus else 16
 
 
+def contraction_kernel(plan, nu):
+    """Name of the FP64 matrix-core contraction kernel a plan launches (tables
+    shared by both spins): tiles of at most 16 lookups run the two-slab-chunk
+    form when nu / 16 is even, else the slab-major form; 64-lookup tiles the
+    four-N-tile form."""
+    if plan_tile(plan) > 16:
+        return "cdet::k_contract_f64<true, 4>"
+    return "cdet::k_contract_f64_slabs<true, 2>" if (nu // 16) % 2 == 0 else "cdet::k_contract_f64<true, 1>"
+
+
 def cpu_info():
     """(usable host cores, CPU model name)."""
     try:
@@ -529,6 +539,7 @@ def main_c4(args):
         stage += np.array(plan.profile(d_v, d_o, None, stream=sptr))
     stage /= 5
     lookups = plan.query_count
+    kname = {"exact": "k_contract_exact", "mfma64": contraction_kernel(plan, w["nc"])}[mode]
     plan.close()
     print(f"c4: GPU batch of {w['chains']} envelopes (orders 2..{E}) {batch_ms:.3f} ms", file=sys.stderr,
           flush=True)
@@ -539,7 +550,6 @@ def main_c4(args):
     blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"])
     alg_bytes = blocks * 8.0 * w["nc"] * w["nc"]
     achieved = alg_bytes / (stage[1] * 1e-3) / 1e9
-    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true, 1>"}[mode]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, "c4"),
                 "kernel": kname, "algorithmic_bytes_per_launch": alg_bytes, "distinct_blocks": blocks,
@@ -884,7 +894,7 @@ def main():
     flops = queries * (2.0 * nc * nc + 2.0 * nc)
     contract_s = stage[1] * 1e-3
     achieved = flops / contract_s / 1e12
-    kname = {"exact": "k_contract_exact", "mfma64": "cdet::k_contract_f64<true, " + ("1>" if plan_tile(plan) <= 16 else "4>"),
+    kname = {"exact": "k_contract_exact", "mfma64": contraction_kernel(plan, w["nc"]),
              "split32": "cdet::k_contract_mfma<2>"}[args.mode]
     roofline = {"bound": "mfma" if args.mode != "exact" else "fp64-valu", "achieved": round(achieved, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),

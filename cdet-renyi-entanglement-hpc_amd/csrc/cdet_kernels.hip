@@ -1751,7 +1751,14 @@ static int contract_launch_mfma(const KScalars& ks, int mode, int grid, const in
                                 const int32_t* sdest, double* gup, double* gdn, hipStream_t s, int tile) {
     if (mode == CDET_MODE_MFMA64 && use_f64_v2(ks)) {
         const bool same = ks.same_spin[0] && ks.same_spin[1];
-        if (tile <= 16 && same)
+        const int chunk = tile <= 16 ? contract_f64_slab_chunk(ks.nu >> 4, same) : 0;
+        if (chunk && same)
+            hipLaunchKernelGGL((k_contract_f64_slabs<true, 2>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart,
+                               tlen, su, sv, sdest, gup, gdn);
+        else if (chunk)
+            hipLaunchKernelGGL((k_contract_f64_slabs<false, 2>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart,
+                               tlen, su, sv, sdest, gup, gdn);
+        else if (tile <= 16 && same)
             hipLaunchKernelGGL((k_contract_f64<true, 1>), dim3(grid), dim3(256), 0, s, ks, ntiles, tkey, tstart, tlen,
                                su, sv, sdest, gup, gdn);
         else if (tile <= 16)
@@ -1782,8 +1789,10 @@ static int contract_grid(const cdet_ctx* ctx, int64_t max_tiles, int mode = CDET
     if (mode == CDET_MODE_MFMA64 && use_f64_v2(ctx->ks)) {
         // no more workgroups than fit at once (256 threads = one wave per SIMD)
         const bool same = ctx->ks.same_spin[0] && ctx->ks.same_spin[1];
-        const int fit = tile <= 16 ? (same ? contract_f64_waves<true, 1>() : contract_f64_waves<false, 1>())
-                                   : (same ? contract_f64_waves<true, 4>() : contract_f64_waves<false, 4>());
+        const int chunk = tile <= 16 ? contract_f64_slab_chunk(ctx->ks.nu >> 4, same) : 0;
+        int fit = tile <= 16 ? (same ? contract_f64_waves<true, 1>() : contract_f64_waves<false, 1>())
+                             : (same ? contract_f64_waves<true, 4>() : contract_f64_waves<false, 4>());
+        if (chunk) fit = same ? contract_f64_slab_waves<true, 2>() : contract_f64_slab_waves<false, 2>();
         per_cu = std::min<int64_t>(per_cu, fit);
     }
     int64_t g = std::min<int64_t>(std::max<int64_t>((max_tiles + 3) / 4, 1), static_cast<int64_t>(ctx->ncu) * per_cu);

@@ -262,6 +262,13 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 
 constexpr int kF64Tile = 64;  // queries per wave in k_contract_f64
 
+// Coefficient steps in flight per wave in contract_f64_tile (a third set
+// measured no faster).
+template <int NT>
+constexpr int contract_f64_prefetch() {
+    return 2;
+}
+
 // One tile of k_contract_f64 with NT live 16-wide N tiles (16 (NT-1) < len <=
 // 16 NT).  K is visited in groups of 16; MFMA step r of group t uses
 // k = 16 t + 4 r + g on both operands, so each lane's K basis is one stride-4
@@ -349,16 +356,18 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
             }
         }
     };
-    // Two register sets: step k + 1 is loaded while step k runs.
-    double a0[4], b0[4], a1[4], b1[4];
-    load_next(a0, b0);
-    for (int k = 0; k < total; k += 2) {
-        load_next(a1, b1);
-        __builtin_amdgcn_sched_barrier(0);
-        compute(a0, b0);
-        load_next(a0, b0);
-        __builtin_amdgcn_sched_barrier(0);
-        if (k + 1 < total) compute(a1, b1);
+    // PF register sets: step k + PF - 1 is loaded while step k runs.
+    constexpr int PF = contract_f64_prefetch<NT>();
+    double a[PF][4], b[PF][4];
+#pragma unroll
+    for (int d = 0; d + 1 < PF; ++d) load_next(a[d], b[d]);
+    for (int k = 0; k < total; k += PF) {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) {
+            load_next(a[(d + PF - 1) % PF], b[(d + PF - 1) % PF]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + d < total) compute(a[d], b[d]);
+        }
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -377,6 +386,151 @@ __device__ inline void contract_f64_tile(const KScalars& ks, int g, int j, const
             g_up[d] = x;
             g_dn[d] = y;
         }
+    }
+}
+
+// One tile of at most 16 queries with the K groups outside the row slabs: the
+// slabs are taken S at a time (S live accumulators) and, within such a chunk,
+// the basis values T_k(v) of a K group (k = 16 t + 4 r + g) are formed once
+// and feed the products of all S slabs.  FP64 vector instructions hold up the
+// FP64 matrix core of their SIMD (one FP64 FMA per 16x16x4 product costs
+// about 15% of the product rate), so the basis work falls from one recurrence
+// step per product (contract_f64_tile<SAME, 1>) to one per S products.  The
+// products of a slab and their order of accumulation are those of
+// contract_f64_tile<SAME, 1> (slab-major over the step-order table, K groups
+// in order), and so are the basis values and the row fold: the values are the
+// same bit for bit.  Coefficient offsets are wave-uniform, so the loads take
+// a scalar base and a fixed per-lane offset; the step PF - 1 ahead of the one
+// computed is loaded into a rotating register set.
+template <bool SAME, int S>
+__device__ inline void contract_f64_tile_slabs(const KScalars& ks, int g, int j, const double* __restrict__ cu,
+                                               const double* __restrict__ cd, int32_t start, int32_t len,
+                                               const double* __restrict__ s_u, const double* __restrict__ s_v,
+                                               const int32_t* __restrict__ s_dest, double* __restrict__ g_up,
+                                               double* __restrict__ g_dn) {
+    constexpr int PF = 2;
+    static_assert(S % PF == 0, "the rotating register sets must divide the chunk");
+    const int groups = ks.nv >> 4;
+    const int nchunk = (ks.nu >> 4) / S;
+    const int p = start + (j < len ? j : 0);
+    const double v = s_v[p];
+    BasisK Au;
+    Au.init(s_u[p], g);
+    double G = 0.0, Gd = 0.0;
+    const int lane_off = 16 * g + j;
+    double a[PF][4], b[PF][4];
+    auto load = [&](double* x, double* y, int off) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            x[r] = cu[off + lane_off + 64 * r];
+            if (!SAME) y[r] = cd[off + lane_off + 64 * r];
+        }
+    };
+    const int slab_stride = groups * 256;  // doubles between the steps of one K group in two slabs
+#pragma unroll
+    for (int d = 0; d + 1 < PF; ++d) load(a[d], b[d], d * slab_stride);  // steps (chunk 0, group 0, slab d)
+    for (int c = 0; c < nchunk; ++c) {
+        d4 acc[S], accd[S];
+#pragma unroll
+        for (int mm = 0; mm < S; ++mm) acc[mm] = accd[mm] = d4{0.0, 0.0, 0.0, 0.0};
+        BasisK Bv;
+        Bv.init(v, g);
+        for (int t = 0; t < groups; ++t) {
+            // T_k(v) for k = 16 t + 4 r + g, by the recurrence of contract_f64_tile
+            double bv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double& rd = (r & 1) ? Bv.prev : Bv.cur;
+                double& wr = (r & 1) ? Bv.cur : Bv.prev;
+                bv[r] = rd;
+                wr = __builtin_fma(Bv.w, rd, -wr);
+            }
+#pragma unroll
+            for (int mm = 0; mm < S; ++mm) {
+                // the step PF - 1 after (c, t, mm); past the last step the last
+                // step is read again
+                int lm = mm + PF - 1, lt = t, lc = c;
+                if (lm >= S) {
+                    lm -= S;
+                    if (++lt == groups) {
+                        lt = 0;
+                        ++lc;
+                    }
+                }
+                if (lc == nchunk) {
+                    lc = nchunk - 1;
+                    lt = groups - 1;
+                    lm = S - 1;
+                }
+                load(a[(mm + PF - 1) % PF], b[(mm + PF - 1) % PF], (lc * S + lm) * slab_stride + lt * 256);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    acc[mm] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mm % PF][r], bv[r], acc[mm], 0, 0, 0);
+                    if (!SAME)
+                        accd[mm] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[mm % PF][r], bv[r], accd[mm], 0, 0, 0);
+                }
+            }
+        }
+        // rows 16 (S c + mm) + 4 r + g of the accumulators, slabs in order
+#pragma unroll
+        for (int mm = 0; mm < S; ++mm) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                G = __builtin_fma(Au.cur, acc[mm][r], G);
+                if (!SAME) Gd = __builtin_fma(Au.cur, accd[mm][r], Gd);
+                Au.step();
+            }
+        }
+    }
+    G += __shfl_xor(G, 16);
+    G += __shfl_xor(G, 32);
+    double y = G;
+    if (!SAME) {
+        y = Gd;
+        y += __shfl_xor(y, 16);
+        y += __shfl_xor(y, 32);
+    }
+    if (g == 0 && j < len) {
+        const int32_t d = s_dest[start + j];
+        g_up[d] = G;
+        g_dn[d] = y;
+    }
+}
+
+// Slabs per chunk of contract_f64_tile_slabs for nu = 16 mslabs: 2 where
+// mslabs is even, else 0 (the slab-major form contract_f64_tile<SAME, 1>
+// runs).  Order-10 batches of the 96x96 tables: 65.4 us per contraction with
+// chunks of 2 slabs (66 registers, seven waves per SIMD), 67.4 us with chunks
+// of 6 (106 registers, four waves), 70.0 us slab-major.
+__host__ __device__ constexpr int contract_f64_slab_chunk(int mslabs, bool same) {
+    return (void)same, mslabs % 2 == 0 ? 2 : 0;
+}
+template <bool SAME, int S>
+constexpr int contract_f64_slab_waves() {
+    return SAME ? 7 : 4;
+}
+template <bool SAME, int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(contract_f64_slab_waves<SAME, S>()))) void
+k_contract_f64_slabs(KScalars ks, const int32_t* __restrict__ n_tiles, const int32_t* __restrict__ tile_key,
+                     const int32_t* __restrict__ tile_start, const int32_t* __restrict__ tile_len,
+                     const double* __restrict__ s_u, const double* __restrict__ s_v,
+                     const int32_t* __restrict__ s_dest, double* __restrict__ g_up, double* __restrict__ g_dn) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const TileRange tr = xcd_tiles(*n_tiles, wave, blockDim.x >> 6);
+    for (int t0 = tr.first; t0 < tr.end; t0 += tr.stride) {
+        const int tile = __builtin_amdgcn_readfirstlane(t0);
+        const int32_t key = __builtin_amdgcn_readfirstlane(tile_key[tile]);
+        const int32_t start = __builtin_amdgcn_readfirstlane(tile_start[tile]);
+        const int32_t len = __builtin_amdgcn_readfirstlane(tile_len[tile]);
+        const double* cu;
+        const double* cd;
+        bool same;
+        ks.key_steps(key, cu, cd, same);
+        contract_f64_tile_slabs<SAME, S>(ks, g, j, cu, cd, start, len, s_u, s_v, s_dest, g_up, g_dn);
     }
 }
 

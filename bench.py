@@ -663,7 +663,8 @@ def main_c5(args):
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
         stage = np.array(plan.profile(d_v, d_o, None, stream=sptr))
-        res[mode] = dict(ms=ms, stage_ms=[round(float(x), 4) for x in stage], lookups=int(plan.query_count))
+        res[mode] = dict(ms=ms, stage_ms=[round(float(x), 4) for x in stage], lookups=int(plan.query_count),
+                         kernel=contraction_kernel(plan, w["nc"]) if mode == "mfma64" else "k_contract_exact")
         outs[mode] = d_o.cpu().numpy()
         print(f"c5: {mode} {ms:.3f} ms per batch, stages {stage}", file=sys.stderr, flush=True)
         plan.close()
@@ -695,8 +696,8 @@ def main_c5(args):
     achieved = alg_bytes / (cms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles("cdet::k_contract_f64<true, 4>", "c5"),
-                "kernel": "cdet::k_contract_f64<true, 4>", "algorithmic_bytes_per_launch": alg_bytes,
+                "traffic": traffic_from_profiles(res["mfma64"]["kernel"], "c5"),
+                "kernel": res["mfma64"]["kernel"], "algorithmic_bytes_per_launch": alg_bytes,
                 "distinct_blocks": blocks, "kernel_ms": round(cms, 4)}
     cpu = None
     if not args.no_cpu:

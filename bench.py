@@ -65,6 +65,9 @@ def parse():
                          "default: the tuning table's, else 3")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore the workload's tuning table (tuning/<workload>.json) and use built-in launch shapes")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY[.ORDER]=VALUE",
+                    help="override one launch-shape knob of the tuning table (cdet_tuning), e.g. "
+                         "fused_minors_max_order=10 or group16_min_terms.10=1048576; repeatable")
     ap.add_argument("--no-split32", action="store_true", help="c2: skip the split-FP32 rate and error report")
     ap.add_argument("--no-ulp", action="store_true",
                     help="skip the FP64 one-ulp table perturbation reference for the error report")
@@ -179,14 +182,32 @@ def apply_tuning(ctx, workload, args):
     """Apply the autotuner's table for this workload (tune.py); returns the
     line's record of it and the table's batches in flight."""
     import tune
-    if args.no_tuning:
-        return {"table": None, "reason": "--no-tuning"}, None
-    tab = tune.load_table(workload)
+    over = {}
+    for kv in args.tune:
+        k, _, v = kv.partition("=")
+        k, _, order = k.strip().partition(".")
+        if order:  # per-order knob: KEY.ORDER=VALUE
+            over.setdefault(k, {})[int(order)] = int(v)
+        else:
+            over[k] = int(v)
+    tab = None if args.no_tuning else tune.load_table(workload)
     if not tab:
-        return {"table": None, "reason": "no table"}, None
-    ctx.set_tuning(tab["tuning"])
+        if over:
+            ctx.set_tuning(over)
+        return {"table": None, "reason": "--no-tuning" if args.no_tuning else "no table",
+                "overrides": over or None, "in_effect": ctx.tuning()}, None
+    merged = dict(tab["tuning"])
+    for k, v in over.items():
+        if isinstance(v, dict):  # per-order values on top of the table's list
+            vals = list(merged.get(k, [0] * 15))
+            for n, x in v.items():
+                vals[n] = x
+            merged[k] = vals
+        else:
+            merged[k] = v
+    ctx.set_tuning(merged)
     return ({"table": os.path.relpath(tune.table_path(workload), ROOT), "device": tab.get("device"),
-             "inflight": tab.get("inflight"), "in_effect": ctx.tuning(),
+             "inflight": tab.get("inflight"), "overrides": over or None, "in_effect": ctx.tuning(),
              "tuned_vs_default_rate_when_tuned": round(tab["tuned_rate"] / tab["default_rate"], 4)},
             tab.get("inflight"))
 
@@ -295,11 +316,13 @@ def unit_index(coordinate, extent):
     return min(max(int(np.floor(extent * coordinate)), 0), extent - 1)
 
 
-def contraction_blocks(configs, N1, N2, beta, branch_eps=1e-13):
+def contraction_blocks(configs, N1, N2, beta, branch_eps=1e-13, skip_diagonal=False):
     """Distinct (class, branch, sitepair) coefficient blocks the table
     contraction of a batch reads (corner lookups read none), by the lookup
     rules of include/cdet/g_funcs.h:609-653 and
-    include/cdet/integrand_replica_randomly_sampled.h:729."""
+    include/cdet/integrand_replica_randomly_sampled.h:729.  skip_diagonal:
+    the FP64 matrix-core mode, whose contraction takes no diagonal pair (those
+    are evaluated from row sums while planning)."""
     keys = set()
     for v in configs:
         n = len(v)
@@ -309,6 +332,8 @@ def contraction_blocks(configs, N1, N2, beta, branch_eps=1e-13):
             for i in range(n):
                 xi, yi, ri, ti, ui = dec[i]
                 for j in range(n):
+                    if skip_diagonal and i == j:
+                        continue
                     xj, yj, rj, tj, uj = dec[j]
                     upper = i != j and ui > uj
                     if (ti if upper else tj) < branch_eps:
@@ -538,7 +563,7 @@ def main_c4(args):
     for _ in range(5):
         stage += np.array(plan.profile(d_v, d_o, None, stream=sptr))
     stage /= 5
-    lookups = plan.query_count
+    lookups, _ = plan.contracted()  # lookups of the contraction (the profiled run)
     kname = {"exact": "k_contract_exact", "mfma64": contraction_kernel(plan, w["nc"])}[mode]
     plan.close()
     print(f"c4: GPU batch of {w['chains']} envelopes (orders 2..{E}) {batch_ms:.3f} ms", file=sys.stderr,
@@ -547,7 +572,7 @@ def main_c4(args):
     # blocks (about 1.5 lookups per 131 KB block), so it is bound by the block
     # reads: algorithmic bytes = distinct blocks x 8 Nc^2, against HBM
     flops = lookups * (2.0 * w["nc"] * w["nc"] + 2.0 * w["nc"])
-    blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"])
+    blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"], skip_diagonal=mode == "mfma64")
     alg_bytes = blocks * 8.0 * w["nc"] * w["nc"]
     achieved = alg_bytes / (stage[1] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -663,7 +688,7 @@ def main_c5(args):
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
         stage = np.array(plan.profile(d_v, d_o, None, stream=sptr))
-        res[mode] = dict(ms=ms, stage_ms=[round(float(x), 4) for x in stage], lookups=int(plan.query_count),
+        res[mode] = dict(ms=ms, stage_ms=[round(float(x), 4) for x in stage], lookups=int(plan.contracted()[0]),
                          kernel=contraction_kernel(plan, w["nc"]) if mode == "mfma64" else "k_contract_exact")
         outs[mode] = d_o.cpu().numpy()
         print(f"c5: {mode} {ms:.3f} ms per batch, stages {stage}", file=sys.stderr, flush=True)
@@ -690,7 +715,7 @@ def main_c5(args):
                                                      bnd[sl].reshape(len(idx), -1))}
     # contraction roofline: HBM-bound at this size (each lookup reads a
     # 295 KB coefficient block); algorithmic bytes = distinct blocks x 8 Nc^2
-    blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"])
+    blocks = contraction_blocks(verts, w["N1"], w["N2"], w["beta"], skip_diagonal=True)
     alg_bytes = blocks * 8.0 * w["nc"] * w["nc"]
     cms = res["mfma64"]["stage_ms"][1]
     achieved = alg_bytes / (cms * 1e-3) / 1e9
@@ -890,9 +915,13 @@ def main():
     for _ in range(reps):
         stage += np.array(plan.profile(d_verts, d_out, d_status, stream=sptr))
     stage /= reps
-    queries = plan.query_count  # lookups per launch (corners included, ~0 at these times)
+    queries = plan.query_count  # Green-pair slots per step: 2 classes x n^2 per configuration
+    # lookups the contraction itself processed in the profiled run (no corner
+    # pairs; in the FP64 matrix-core mode no diagonal pairs, which are
+    # evaluated from row sums while planning), and its tiles
+    contracted, tiles = plan.contracted()
     nc = w["nc"]
-    flops = queries * (2.0 * nc * nc + 2.0 * nc)
+    flops = contracted * (2.0 * nc * nc + 2.0 * nc)
     contract_s = stage[1] * 1e-3
     achieved = flops / contract_s / 1e12
     kname = {"exact": "k_contract_exact", "mfma64": contraction_kernel(plan, w["nc"]),
@@ -900,7 +929,8 @@ def main():
     roofline = {"bound": "mfma" if args.mode != "exact" else "fp64-valu", "achieved": round(achieved, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                 "traffic": traffic_from_profiles(kname), "kernel": kname,
-                "flops_per_launch": flops, "kernel_ms": round(stage[1], 4),
+                "flops_per_launch": flops, "lookups_per_launch": int(contracted), "tiles_per_launch": int(tiles),
+                "kernel_ms": round(stage[1], 4),
                 "stage_ms": {"plan_group": round(stage[0], 4), "contract": round(stage[1], 4),
                              "fpm_cdet": round(stage[2], 4)}}
 

@@ -62,9 +62,10 @@ class CdetTuning(ctypes.Structure):
         ("minors_threads", ctypes.c_int32 * (MAX_ORDER + 1)),
         ("group16_min_terms", ctypes.c_int32 * (MAX_ORDER + 1)),
         ("group64_min_terms", ctypes.c_int32 * (MAX_ORDER + 1)),
+        ("fused_minors_max_order", ctypes.c_int32),
     ]
 
-    SCALARS = ("contract_wg_per_cu", "contract_tile", "pair_min_per_cu", "max_lanes")
+    SCALARS = ("contract_wg_per_cu", "contract_tile", "pair_min_per_cu", "max_lanes", "fused_minors_max_order")
     PER_ORDER = ("minors_threads", "group16_min_terms", "group64_min_terms")
 
     def to_dict(self):
@@ -167,6 +168,7 @@ def _signatures():
         "cdet_batch_out_size": (i64, [vp]),
         "cdet_batch_vertex_count": (i64, [vp]),
         "cdet_batch_query_count": (i64, [vp]),
+        "cdet_batch_contracted": (i, [vp, P(ctypes.c_int64), P(ctypes.c_int64)]),
         "cdet_batch_plan_streams": (i, [vp]),
         "cdet_batch_run": (i, [vp, vp, vp, vp, i, vp]),
         "cdet_batch_plan_reuse_pairs": (i, [vp, i]),
@@ -408,6 +410,12 @@ class BatchPlan:
         self.vertex_count = lib().cdet_batch_vertex_count(h)
         self.query_count = lib().cdet_batch_query_count(h)
         self.streams = lib().cdet_batch_plan_streams(h)
+
+    def contracted(self):
+        """(lookups, tiles) the last run sent to the table contraction."""
+        lk, nt = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().cdet_batch_contracted(self._h, ctypes.byref(lk), ctypes.byref(nt)))
+        return lk.value, nt.value
 
     def close(self):
         if getattr(self, "_h", None):

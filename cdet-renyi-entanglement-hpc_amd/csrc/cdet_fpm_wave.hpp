@@ -33,20 +33,27 @@ constexpr int kWaveFpmL = 6;  // level at which lanes take one matrix each (2^6 
 
 __host__ __device__ constexpr int fpm_wave_L(int n) { return n - 1 < kWaveFpmL ? n - 1 : kWaveFpmL; }
 
-// LDS doubles of k_fpm_wave: two level lists (even / odd levels up to L),
-// the inverse pivots of a level (<= 2^(L-1)) and the pivots of masks < 2^L.
-__host__ __device__ inline int fpm_wave_lds_doubles(int n) {
-    const int L = fpm_wave_L(n);
-    int e0 = 0, e1 = 0;
-    for (int l = 0; l <= L; ++l) {
+// LDS of the wave FPM: two level lists (even / odd levels up to L), the
+// inverse pivots of a level (<= 2^(L-1)) and the pivots of masks < 2^L.
+// fpm_wave_e0: doubles of the even-level list; fpm_wave_e1: the odd-level
+// list with the 32 inverse pivots and 64 low-mask pivots behind it.
+__host__ __device__ constexpr int fpm_wave_e0(int n) {
+    int e = 0;
+    for (int l = 0; l <= fpm_wave_L(n); l += 2) {
         const int sz = (1 << l) * (n - l) * (n - l);
-        if (l & 1)
-            e1 = sz > e1 ? sz : e1;
-        else
-            e0 = sz > e0 ? sz : e0;
+        e = sz > e ? sz : e;
     }
-    return e0 + e1 + 32 + 64;
+    return e;
 }
+__host__ __device__ constexpr int fpm_wave_e1(int n) {
+    int e = 0;
+    for (int l = 1; l <= fpm_wave_L(n); l += 2) {
+        const int sz = (1 << l) * (n - l) * (n - l);
+        e = sz > e ? sz : e;
+    }
+    return e + 32 + 64;
+}
+__host__ __device__ constexpr int fpm_wave_lds_doubles(int n) { return fpm_wave_e0(n) + fpm_wave_e1(n); }
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -109,47 +116,28 @@ __device__ __forceinline__ void fpm_lane_subtree(const double (&X)[M * M], doubl
     }
 }
 
-// One spin matrix per 64-thread workgroup.  Work item w as k_fpm: 2 (li,
-// class) pairs x 2 spins, or x 1 when both classes have identical spin tables.
-// Writes pm[(4 li + 2 class + spin) 2^n + mask], or queues the matrix for the
-// pivoted-determinant fallback exactly as k_fpm does.
+// Outcome of fpm_wave_minors.
+constexpr int kFpmWaveOk = 0;
+constexpr int kFpmWaveFallback = 1;  // a pivot tripped the reference's fallback test
+
+// Principal minors of one spin matrix on the calling wavefront.  g holds the
+// n x n Green values (row-major) and shift the spin's diagonal shift; the
+// vertex matrix is (g + shift I) scale
+// (include/cdet/integrand_replica_randomly_sampled.h:751-761).  list0
+// (fpm_wave_e0(n) doubles) and list1 (fpm_wave_e1(n) doubles) are LDS used by
+// this wave alone; the wave's last LDS access precedes the return.  On
+// kFpmWaveOk, lane s < 2^L holds b[h] = det of the principal submatrix of mask
+// 2^L h + s (h < 2^M, M = n - L); an all-zero matrix gives 1 for mask 0 and 0
+// elsewhere, as mat2pm does.  kFpmWaveFallback is wave-uniform; b is then
+// undefined.  Nothing is written outside scr.
 template <int M>
-__global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, const int32_t* __restrict__ cfg_list,
-                                                 const int64_t* __restrict__ cfg_pbase,
-                                                 const double* __restrict__ g_up, const double* __restrict__ g_dn,
-                                                 double* __restrict__ pm, int32_t* __restrict__ fb_list,
-                                                 int32_t* __restrict__ fb_count) {
-    extern __shared__ double lds_dyn[];
-    const int lane = threadIdx.x;
-    const int w = blockIdx.x;
-    if (w >= items) return;
-    const bool both_same = ks.same_spin[0] && ks.same_spin[1];
-    const int li = both_same ? (w >> 1) : (w >> 2);
-    const int cls = both_same ? (w & 1) : ((w >> 1) & 1);
-    const int spin = both_same ? 0 : (w & 1);
-    const bool same = ks.same_spin[cls] != 0;
-    if (spin == 1 && same) return;  // the up minors serve both spins
-    const int L = fpm_wave_L(n);    // = n - M
+__device__ __forceinline__ int fpm_wave_minors(const KScalars& ks, int n, const double* __restrict__ g, double shift,
+                                               double* list0, double* list1, int lane, double (&b)[1 << M]) {
+    const int L = fpm_wave_L(n);  // = n - M
     const int T = 1 << L;
     const int n2 = n * n;
-    int e0 = 0;
-    for (int l = 0; l <= L; l += 2) {
-        const int sz = (1 << l) * (n - l) * (n - l);
-        e0 = sz > e0 ? sz : e0;
-    }
-    int e1 = 0;
-    for (int l = 1; l <= L; l += 2) {
-        const int sz = (1 << l) * (n - l) * (n - l);
-        e1 = sz > e1 ? sz : e1;
-    }
-    double* list0 = lds_dyn;
-    double* list1 = list0 + e0;
-    double* inv = list1 + e1;
+    double* inv = list1 + (fpm_wave_e1(n) - 96);
     double* outlow = inv + 32;
-    const int c = cfg_list[li];
-    const double* g = (spin ? g_dn : g_up) + 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
-    const double shift = spin ? ks.ddn : ks.dup;
-    // vertex matrix (include/cdet/integrand_replica_randomly_sampled.h:751-761)
     double lmax = 0.0;
     for (int idx = lane; idx < n2; idx += 64) {
         const int i = idx / n;
@@ -168,11 +156,10 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
     const double max_abs = lmax;
     const bool explicit_shift = !(ks.shift < 0.0);
     const double p = explicit_shift ? ks.shift : max_abs / static_cast<double>(n2);
-    const uint32_t sc = 1u << n;
-    double* dst = pm + static_cast<size_t>(4 * li + 2 * cls + spin) * sc;
     if (max_abs == 0.0 && p == 0.0) {
-        for (uint32_t m = lane; m < sc; m += 64) dst[m] = (m == 0) ? 1.0 : 0.0;
-        return;
+#pragma unroll
+        for (int h = 0; h < (1 << M); ++h) b[h] = (h == 0 && lane == 0) ? 1.0 : 0.0;
+        return kFpmWaveOk;
     }
     const double fb_eps = (ks.tol > 0.0 && max_abs > 0.0) ? ks.tol * max_abs : 0.0;
     wave_sync_lds();
@@ -202,11 +189,18 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
         // below is one contiguous wave read per element (matrix-major, the
         // lanes' loads would be 8 M^2 bytes apart and hit the same LDS banks)
         const bool last = l == L - 1;
+        // item it = s nms + rem, rem = r ns + cc: divided once for the lane's
+        // first item, then stepped by the wave stride 64 = ds nms + drem,
+        // drem = dr ns + dcc, with carries
+        const int ds = 64 / nms;
+        const int drem = 64 - ds * nms;
+        const int dr = drem / ns;
+        const int dcc = drem - dr * ns;
+        int s = lane / nms;
+        int rem = lane - s * nms;
+        int r = rem / ns;
+        int cc = rem - r * ns;
         for (int it = lane; it < nitems; it += 64) {
-            const int s = it / nms;
-            const int rem = it - s * nms;
-            const int r = rem / ns;
-            const int cc = rem - r * ns;
             const int src = s * oms;
             const double f = oldl[src + (r + 1) * os] * inv[s];
             const double base = oldl[src + (r + 1) * os + (cc + 1)];
@@ -218,6 +212,15 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
                 newl[s * nms + rem] = base;
                 newl[(s + sets) * nms + rem] = with;
             }
+            cc += dcc;
+            const int c2 = cc >= ns ? 1 : 0;
+            cc -= c2 * ns;
+            r += dr + c2;
+            const int c1 = r >= ns ? 1 : 0;
+            r -= c1 * ns;
+            rem += drem;
+            rem -= c1 * nms;
+            s += ds + c1;
         }
         wave_sync_lds();
         double* t = oldl;
@@ -239,21 +242,14 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
         uint32_t rest = static_cast<uint32_t>(s) ^ pre;
         fin = outlow[pre];
         while (rest) {
-            const uint32_t b = rest & (~rest + 1u);
-            rest ^= b;
-            pre |= b;
+            const uint32_t bb = rest & (~rest + 1u);
+            rest ^= bb;
+            pre |= bb;
             fin = outlow[pre] * fin;
         }
     }
-    if (__any(bad)) {  // the reference switches to pivoted determinants
-        if (lane == 0) {
-            const int slot = atomicAdd(fb_count, 1);
-            fb_list[slot] = (li << 3) | (cls << 2) | (same ? 3 : (1 << spin));
-        }
-        return;
-    }
+    if (__any(bad)) return kFpmWaveFallback;  // the reference switches to pivoted determinants
     // prefix products over the high bits: b[h] = final[2^L h + s]
-    double b[1 << M];
     b[0] = fin;
 #pragma unroll
     for (int h = 1; h < (1 << M); ++h) {
@@ -278,6 +274,42 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
             if (set) b[h] = __builtin_fma(-p, o, b[h]);
         }
     }
+    return kFpmWaveOk;
+}
+
+// One spin matrix per 64-thread workgroup.  Work item w as k_fpm: 2 (li,
+// class) pairs x 2 spins, or x 1 when both classes have identical spin tables.
+// Writes pm[(4 li + 2 class + spin) 2^n + mask], or queues the matrix for the
+// pivoted-determinant fallback exactly as k_fpm does.
+template <int M>
+__global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, const int32_t* __restrict__ cfg_list,
+                                                 const int64_t* __restrict__ cfg_pbase,
+                                                 const double* __restrict__ g_up, const double* __restrict__ g_dn,
+                                                 double* __restrict__ pm, int32_t* __restrict__ fb_list,
+                                                 int32_t* __restrict__ fb_count) {
+    extern __shared__ double lds_dyn[];
+    const int lane = threadIdx.x;
+    const int w = blockIdx.x;
+    if (w >= items) return;
+    const bool both_same = ks.same_spin[0] && ks.same_spin[1];
+    const int li = both_same ? (w >> 1) : (w >> 2);
+    const int cls = both_same ? (w & 1) : ((w >> 1) & 1);
+    const int spin = both_same ? 0 : (w & 1);
+    const bool same = ks.same_spin[cls] != 0;
+    if (spin == 1 && same) return;  // the up minors serve both spins
+    const int T = 1 << fpm_wave_L(n);
+    const int c = cfg_list[li];
+    const double* g = (spin ? g_dn : g_up) + 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n * n;
+    double b[1 << M];
+    if (fpm_wave_minors<M>(ks, n, g, spin ? ks.ddn : ks.dup, lds_dyn, lds_dyn + fpm_wave_e0(n), lane, b) ==
+        kFpmWaveFallback) {
+        if (lane == 0) {
+            const int slot = atomicAdd(fb_count, 1);
+            fb_list[slot] = (li << 3) | (cls << 2) | (same ? 3 : (1 << spin));
+        }
+        return;
+    }
+    double* dst = pm + static_cast<size_t>(4 * li + 2 * cls + spin) * (size_t(1) << n);
     if (lane < T) {
 #pragma unroll
         for (int h = 0; h < (1 << M); ++h) dst[static_cast<size_t>(h) * T + lane] = b[h];

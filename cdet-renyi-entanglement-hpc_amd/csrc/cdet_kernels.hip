@@ -66,6 +66,7 @@ struct DevTable {
     float* split[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [branch][hi, lo]
     double* steps[2] = {nullptr, nullptr};  // [branch] coefficients in matrix-core step order
     double* abssum[2] = {nullptr, nullptr};  // [branch] sum of |c_ab| per coefficient block
+    double* rowsum = nullptr;  // lower branch: per block and row a, sum over b of c_ab (lookups at v = 1)
     int alias = -1;  // kind whose storage this kind shares
 };
 
@@ -134,7 +135,15 @@ constexpr int kKeySub = 16;
 // value instead of being looked up: the value is the one the lookup computes,
 // since every evaluation form gives a lookup the same result whatever its
 // tile-mates.  cache[0] / cache[1] count reused and looked-up pairs.
+// With rows_at_v1 (FP64 matrix-core plans) the diagonal pairs (i, i) are not
+// looked up: they are lower-branch lookups at v = (2 tau) / tau - 1 = 1
+// exactly, where T_b(1) = 1 for every b, so G = sum_a T_a(u) (sum_b c_ab), a
+// Clenshaw over u of the block's row sums (ks.rowsum).  The rows of the
+// configuration's diagonal pairs are staged in LDS by coalesced workgroup
+// loads, kPlanRowDoubles doubles per pass, and one lane per pair runs the
+// Clenshaw from there.
 constexpr int kPlanPairsPerThread = (2 * kMaxOrder * kMaxOrder + 255) / 256;
+constexpr int kPlanRowDoubles = 1024;
 
 __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restrict__ verts,
                                               const int32_t* __restrict__ cfg_order,
@@ -146,12 +155,18 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
                                               int32_t* __restrict__ status, double* __restrict__ e_up,
                                               double* __restrict__ e_dn, double err_scale, int reuse,
                                               double* __restrict__ prev_verts, int32_t* __restrict__ prev_ok,
-                                              unsigned long long* __restrict__ cache) {
+                                              unsigned long long* __restrict__ cache, int rows_at_v1) {
     __shared__ Vtx vs[2][kMaxOrder];
     __shared__ int pidx[kMaxOrder];
     __shared__ int s_hits, s_misses;
+    __shared__ double s_du[2 * kMaxOrder];     // diagonal pair k = class n + i: u
+    __shared__ int64_t s_dsp[2 * kMaxOrder];   // its sitepair
+    __shared__ int s_dku[2 * kMaxOrder];       // its up-spin table kind
+    __shared__ int s_dok[2 * kMaxOrder];       // evaluated from row sums in this run
+    __shared__ double rows[kPlanRowDoubles];
     const int c = blockIdx.x;
     const int n = cfg_order[c];
+    if (threadIdx.x < 2 * kMaxOrder) s_dok[threadIdx.x] = 0;
     if (status && threadIdx.x == 0) status[c] = 0;
     const int sub = c & (kKeySub - 1);
     const int64_t vb = cfg_vbase[c];
@@ -246,12 +261,52 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
             e_up[q] = err_scale * ks.abssum[ku][br][d.sitepair];
             e_dn[q] = err_scale * ks.abssum[ku + 1][br][d.sitepair];
         }
+        if (rows_at_v1 && i == j && d.v == 1.0) {  // from the row sums below
+            const int k = cls * n + i;
+            s_du[k] = d.u;
+            s_dsp[k] = d.sitepair;
+            s_dku[k] = ku;
+            s_dok[k] = 1;
+            q_key[q] = -1;
+            continue;
+        }
         const int32_t key = static_cast<int32_t>((static_cast<int64_t>(cls) * 2 + br) * ks.SP + d.sitepair);
         const int32_t slot = key * kKeySub + sub;
         q_key[q] = slot;
         q_rank[q] = atomicAdd(&hist[slot], 1);
         q_u[q] = d.u;
         q_v[q] = d.v;
+    }
+    if (rows_at_v1) {
+        __syncthreads();
+        const int nu = ks.nu;
+        const bool dn_rows = ks.rowsum[0] != ks.rowsum[1] || ks.rowsum[2] != ks.rowsum[3];
+        const int per_pass = kPlanRowDoubles / (nu * (dn_rows ? 2 : 1));
+        for (int k0 = 0; k0 < 2 * n; k0 += per_pass) {
+            const int kc = min(per_pass, 2 * n - k0);
+            for (int idx = threadIdx.x; idx < kc * nu; idx += blockDim.x) {
+                const int kk = idx / nu;
+                const int a = idx - kk * nu;
+                const int k = k0 + kk;
+                if (s_dok[k]) {
+                    rows[kk * nu + a] = ks.rowsum[s_dku[k]][s_dsp[k] * nu + a];
+                    if (dn_rows) rows[(per_pass + kk) * nu + a] = ks.rowsum[s_dku[k] + 1][s_dsp[k] * nu + a];
+                }
+            }
+            __syncthreads();
+            if (static_cast<int>(threadIdx.x) < kc) {
+                const int k = k0 + threadIdx.x;
+                if (s_dok[k]) {
+                    const int cls = k >= n ? 1 : 0;
+                    const int i = k - cls * n;
+                    const int64_t q = qb + static_cast<int64_t>(cls) * n2 + i * n + i;
+                    const double gu = clenshaw_row(rows + threadIdx.x * nu, nu - 1, s_du[k]);
+                    g_up[q] = gu;
+                    g_dn[q] = dn_rows ? clenshaw_row(rows + (per_pass + threadIdx.x) * nu, nu - 1, s_du[k]) : gu;
+                }
+            }
+            __syncthreads();
+        }
     }
     if (reuse) {
         // this run's vertices become the next run's previous ones (every
@@ -713,9 +768,12 @@ __global__ __launch_bounds__(NT) void k_fpm(KScalars ks, int n, int items, const
 // Pivoted determinants of every principal submatrix of the queued fallback
 // spin matrices (include/cdet/fast_principle_minors.h:39-115, :180-186),
 // spread over a fixed grid: work item = (queue entry, spin, chunk of 64
-// masks), one mask per thread, the thread's k x k working matrix interleaved
-// in LDS.  Counts the fallbacks (include/cdet/runtime_counters.h:27) and
-// flags the configuration's status.
+// masks), one mask per thread, the thread's k x k working matrix in registers
+// up to order 10 (det_minor_regs) and interleaved in LDS above
+// (det_minor_brute).  Counts the fallbacks
+// (include/cdet/runtime_counters.h:27) and flags the configuration's status.
+// (Working matrices in LDS at every order, in 16-thread workgroups, measured
+// slower: the order-10 minors stage 94 -> 120 us.)
 constexpr int kDetThreads = 64;
 constexpr int kDetRegMax = 10;  // orders whose fallback determinants run in registers
 template <bool REG>
@@ -914,6 +972,66 @@ struct CdetPairLds {
     static __host__ __device__ inline size_t bytes(int n, int mode) { return sizeof(double) * doubles(n, mode); }
 };
 
+// Recursion and assembly of one configuration's two classes once a[class][mask]
+// = pm_up * pm_down is in LDS (a[class][0] = 0) together with the unscaled
+// shifted diagonals dU, dD (class 1 at offset n): the connected values
+// (include/cdet/integrand_replica_randomly_sampled.h:292-347), the singletons
+// and prefactors (:833-877) and twisted - 2 * ordinary
+// (include/cdet/cdet_integrand_evaluator.h:348-350) into dst; mode 2 writes
+// the magnitude scales twisted + 2 * ordinary instead.  Non-finite values set
+// status bit 0.  Ends with a workgroup barrier.
+__device__ __forceinline__ void pair_finish(const KScalars& ks, int n, int mode, const double* dU, const double* dD,
+                                            double* a, double* dp, double* rhs, const uint16_t* lmask,
+                                            const int* s_popoff, double* tb, double* __restrict__ dst,
+                                            int32_t* status_c) {
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const uint32_t sc = 1u << n;
+    ztransform_layers_pair(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, tb,
+                           ks.group16_R[n], ks.group64_R[n]);
+    bool bad = false;
+    for (uint32_t m = tid; m < sc; m += nt) {
+        double v[2];
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+            if (m == 0) {
+                v[cls] = 0.0;
+            } else if ((m & (m - 1)) == 0) {
+                const int i = __builtin_ctz(m);
+                v[cls] = ks.c1[cls] * (dU[cls * n + i] * dD[cls * n + i] - ks.counter);
+                if (mode == 2) v[cls] = fabs(v[cls]);
+            } else {
+                const int k = __builtin_popcount(m);
+                v[cls] = mode == 2 ? fabs(ks.pref[cls][k]) * rhs[cls * sc + m] : ks.pref[cls][k] * dp[cls * sc + m];
+            }
+            if (!isfinite(v[cls])) bad = true;
+        }
+        dst[m] = mode == 2 ? v[0] + 2.0 * v[1] : v[0] - 2.0 * v[1];
+    }
+    if (bad && status_c) atomicOr(status_c, 1);
+    __syncthreads();
+}
+
+// Unscaled shifted diagonal entries of both classes
+// (include/cdet/integrand_replica_randomly_sampled.h:751-758), thread q0
+// onward in steps of stride.
+__device__ __forceinline__ void pair_diagonals(const KScalars& ks, int n, int64_t pb, const double* __restrict__ g_up,
+                                               const double* __restrict__ g_dn, double* dU, double* dD, int q0,
+                                               int stride) {
+    const int n2 = n * n;
+    for (int q = q0; q < 2 * n; q += stride) {
+        const int cls = q >= n ? 1 : 0;
+        const int i = q - cls * n;
+        const int64_t at = pb + static_cast<int64_t>(cls) * n2 + i * n + i;
+        dU[q] = g_up[at] + ks.dup;
+        dD[q] = g_dn[at] + ks.ddn;
+    }
+}
+
+// With defer_count non-null the work items are the defer_count[0]
+// configurations of defer_list (the fused kernel's deferred ones, below)
+// instead of cfg_list[0 .. items); the grid's last workgroup to finish
+// resets both queue words (defer_count[1] counts finished workgroups).
 __global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode, int items,
                                                    const int32_t* __restrict__ cfg_list,
                                                    const int64_t* __restrict__ cfg_pbase,
@@ -922,7 +1040,9 @@ __global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode,
                                                    const double* __restrict__ pm, double* __restrict__ out,
                                                    int32_t* __restrict__ status,
                                                    const uint16_t* __restrict__ masks_by_pop,
-                                                   const int* __restrict__ pop_off, int32_t* __restrict__ fb_count) {
+                                                   const int* __restrict__ pop_off, int32_t* __restrict__ fb_count,
+                                                   const int32_t* __restrict__ defer_list,
+                                                   int32_t* __restrict__ defer_count) {
     extern __shared__ double lds_dyn[];
     __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
     __shared__ int s_popoff[kMaxOrder + 2];
@@ -931,24 +1051,18 @@ __global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode,
     const int nt = blockDim.x;
     for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off[k];
     const uint32_t sc = 1u << n;
-    const int n2 = n * n;
     double* a = lds_dyn;
     double* dp = a + 2 * sc;
     double* rhs = dp + 2 * sc;  // mode 2 only
     double* dU = rhs + (mode == 2 ? 2 * sc : 0);
     double* dD = dU + 2 * n;
     uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + 2 * n);
-    for (int li = blockIdx.x; li < items; li += gridDim.x) {
+    for (uint32_t m = tid; m < sc; m += nt) lmask[m] = masks_by_pop[m];
+    if (defer_count) items = __atomic_load_n(defer_count, __ATOMIC_RELAXED);
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        const int li = defer_count ? defer_list[w] : w;
         const int c = cfg_list[li];
-        // unscaled shifted diagonal entries for the singletons
-        // (include/cdet/integrand_replica_randomly_sampled.h:751-758)
-        for (int q = tid; q < 2 * n; q += nt) {
-            const int cls = q >= n ? 1 : 0;
-            const int i = q - cls * n;
-            const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
-            dU[q] = g_up[pb + i * n + i] + ks.dup;
-            dD[q] = g_dn[pb + i * n + i] + ks.ddn;
-        }
+        pair_diagonals(ks, n, 2 * cfg_pbase[c], g_up, g_dn, dU, dD, tid, nt);
         // a[class][mask] = pm_up * pm_down, a[class][0] = 0; four masks per
         // thread in flight
         {
@@ -974,34 +1088,144 @@ __global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode,
                 }
             }
         }
-        for (uint32_t m = tid; m < sc; m += nt) lmask[m] = masks_by_pop[m];
         __syncthreads();
-        ztransform_layers_pair(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, tb,
-                               ks.group16_R[n], ks.group64_R[n]);
-        // assembly (:833-877) and twisted - 2 * ordinary
-        // (include/cdet/cdet_integrand_evaluator.h:348-350)
-        double* dst = out + cfg_obase[c];
-        bool bad = false;
-        for (uint32_t m = tid; m < sc; m += nt) {
-            double v[2];
-#pragma unroll
-            for (int cls = 0; cls < 2; ++cls) {
-                if (m == 0) {
-                    v[cls] = 0.0;
-                } else if ((m & (m - 1)) == 0) {
-                    const int i = __builtin_ctz(m);
-                    v[cls] = ks.c1[cls] * (dU[cls * n + i] * dD[cls * n + i] - ks.counter);
-                    if (mode == 2) v[cls] = fabs(v[cls]);
-                } else {
-                    const int k = __builtin_popcount(m);
-                    v[cls] = mode == 2 ? fabs(ks.pref[cls][k]) * rhs[cls * sc + m] : ks.pref[cls][k] * dp[cls * sc + m];
-                }
-                if (!isfinite(v[cls])) bad = true;
+        pair_finish(ks, n, mode, dU, dD, a, dp, rhs, lmask, s_popoff, tb, out + cfg_obase[c],
+                    status ? status + c : nullptr);
+    }
+    if (defer_count) {
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            if (atomicAdd(defer_count + 1, 1) == static_cast<int>(gridDim.x) - 1) {
+                defer_count[0] = 0;
+                defer_count[1] = 0;
             }
-            dst[m] = mode == 2 ? v[0] + 2.0 * v[1] : v[0] - 2.0 * v[1];
         }
-        if (bad && status) atomicOr(&status[c], 1);
+    }
+}
+
+// Principal minors and paired subset recursion of one configuration in one
+// workgroup (orders 2 .. kPairOrderMax, identical up and down tables in both
+// classes, modes 0 and 2).  Waves 0 and 1 run the principal minors of the
+// twisted and the ordinary vertex matrix (fpm_wave_minors) while waves 2 and
+// 3 load the diagonals.  Each FPM wave writes its minors into its own class's
+// array a[class] as soon as it is done: its level lists sit in its own
+// a[class] and dp[class] (order 10) or beyond the recursion's arrays (lower
+// orders, whose lists outgrow 2^n doubles), never in the other class's a.
+// After one barrier a[class][mask] becomes pm * pm (a[class][0] = 0) and
+// pair_finish runs the recursion and assembly.  The minors never leave the
+// workgroup: the same values, bit for bit, as k_fpm_wave + k_cdet_pair, with
+// two launches and the minors' HBM round trip fewer.
+// A configuration whose twisted or ordinary matrix trips the reference's FPM
+// fallback (include/cdet/fast_principle_minors.h:180-186) is deferred: the
+// minors of its other class go to pm, the flagged matrix to the fallback
+// queue (fb_list / fb_count, as k_fpm_wave queues it) and the configuration
+// to defer_list / defer_count; k_fallback_dets then writes the flagged
+// matrix's pivoted determinants and k_cdet_pair's deferred form finishes the
+// configuration.  (Pivoted determinants in registers here would need ~200
+// VGPRs and cut the occupancy of every workgroup for the rare fallback.)
+struct MinorsPairLds {
+    // the FPM level lists fit in a[class] and dp[class]
+    static __host__ __device__ constexpr bool lists_in_arrays(int n) {
+        return fpm_wave_e0(n) <= (1 << n) && fpm_wave_e1(n) <= (1 << n);
+    }
+    static __host__ __device__ inline size_t arrays(int n, int mode) {
+        return (mode == 2 ? 6 : 4) * (size_t(1) << n);
+    }
+    static __host__ __device__ inline size_t extra(int n) {
+        return lists_in_arrays(n) ? 0 : 2 * static_cast<size_t>(fpm_wave_lds_doubles(n));
+    }
+    static __host__ __device__ inline size_t doubles(int n, int mode) {
+        return arrays(n, mode) + 4 * static_cast<size_t>(n) + (size_t(1) << n) / 4 + 1 + extra(n);
+    }
+    static __host__ __device__ inline size_t bytes(int n, int mode) { return sizeof(double) * doubles(n, mode); }
+};
+
+// Four workgroups per CU (their LDS fills it) need at most 128 VGPRs each.
+template <int N>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_minors_pair(
+    KScalars ks, int n_rt, int mode, int items,
+                                                     const int32_t* __restrict__ cfg_list,
+                                                     const int64_t* __restrict__ cfg_pbase,
+                                                     const int64_t* __restrict__ cfg_obase,
+                                                     const double* __restrict__ g_up, const double* __restrict__ g_dn,
+                                                     double* __restrict__ pm, double* __restrict__ out,
+                                                     int32_t* __restrict__ status,
+                                                     const uint16_t* __restrict__ masks_by_pop,
+                                                     const int* __restrict__ pop_off, int32_t* __restrict__ fb_list,
+                                                     int32_t* __restrict__ fb_count, int32_t* __restrict__ defer_list,
+                                                     int32_t* __restrict__ defer_count) {
+    constexpr int L = fpm_wave_L(N);
+    constexpr int M = N - L;
+    constexpr int T = 1 << L;
+    constexpr uint32_t sc = 1u << N;
+    constexpr int n2 = N * N;
+    constexpr int E0 = fpm_wave_e0(N);
+    constexpr bool kInArrays = MinorsPairLds::lists_in_arrays(N);
+    extern __shared__ double lds_dyn[];
+    __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
+    __shared__ int s_popoff[kMaxOrder + 2];
+    __shared__ int s_bad[2];
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    double* a = lds_dyn;
+    double* dp = a + 2 * sc;
+    double* rhs = dp + 2 * sc;  // mode 2 only
+    double* dU = lds_dyn + MinorsPairLds::arrays(N, mode);
+    double* dD = dU + 2 * N;
+    uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + 2 * N);
+    double* xtra = dD + 2 * N + sc / 4 + 1;
+    // this wave's FPM level lists (waves 0 and 1)
+    const int wc = wave & 1;
+    double* list0 = kInArrays ? a + wc * sc : xtra + wc * fpm_wave_lds_doubles(N);
+    double* list1 = kInArrays ? dp + wc * sc : list0 + E0;
+    for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off[k];
+    for (uint32_t m = tid; m < sc; m += nt) lmask[m] = masks_by_pop[m];
+    for (int li = blockIdx.x; li < items; li += gridDim.x) {
+        const int c = cfg_list[li];
+        const int64_t pb = 2 * cfg_pbase[c];
+        if (wave < 2) {
+            double b[1 << M];
+            // the order at run time (n_rt = N): with N folded in, the level loops
+            // unroll completely and the kernel needs ~170 VGPRs instead of ~100
+            const int r = fpm_wave_minors<M>(ks, n_rt, g_up + pb + static_cast<int64_t>(wave) * n2, ks.dup, list0,
+                                             list1, lane, b);
+            if (lane == 0) s_bad[wave] = r;
+            if (r == kFpmWaveOk) {
+                wave_sync_lds();  // the wave's list reads precede its writes of a[class]
+                if (lane < T) {
+                    double* ac = a + wave * sc;
+#pragma unroll
+                    for (int h = 0; h < (1 << M); ++h) ac[h * T + lane] = b[h];
+                }
+            }
+        } else {
+            pair_diagonals(ks, N, pb, g_up, g_dn, dU, dD, tid - 128, nt - 128);
+        }
         __syncthreads();
+        if ((s_bad[0] | s_bad[1]) != 0) {  // uniform
+            if (wave < 2) {
+                if (s_bad[wave] == kFpmWaveOk) {
+                    double* dst = pm + static_cast<size_t>(4 * li + 2 * wave) * sc;
+                    for (uint32_t m = lane; m < sc; m += 64) dst[m] = a[wave * sc + m];
+                } else if (lane == 0) {
+                    fb_list[atomicAdd(fb_count, 1)] = (li << 3) | (wave << 2) | 3;
+                }
+            }
+            if (tid == 0) defer_list[atomicAdd(defer_count, 1)] = li;
+            __syncthreads();  // the next configuration reuses LDS
+            continue;
+        }
+        // a = pm_up * pm_down with identical spins, a[class][0] = 0
+        for (uint32_t m = tid; m < 2 * sc; m += nt) {
+            const double v = a[m];
+            a[m] = (m & (sc - 1)) == 0 ? 0.0 : v * v;
+        }
+        __syncthreads();
+        pair_finish(ks, n_rt, mode, dU, dD, a, dp, rhs, lmask, s_popoff, tb, out + cfg_obase[c],
+                    status ? status + c : nullptr);
     }
 }
 
@@ -1072,6 +1296,22 @@ __global__ __launch_bounds__(256) void k_block_abssum(const double* __restrict__
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) out[b] = s;
+}
+
+// out[r] = sum over b of c[r nv + b] for every coefficient row r (block-major
+// rows a of each block): one wave per row, lanes strided over b, then a fixed
+// butterfly (row sums of the lower-branch tables, for lookups at v = 1).
+__global__ __launch_bounds__(256) void k_row_sums(const double* __restrict__ c, int64_t rows, int nv,
+                                                  double* __restrict__ out) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
+    const double* p = c + r * nv;
+    double s = 0.0;
+    for (int i = lane; i < nv; i += 64) s += p[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) out[r] = s;
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -1330,6 +1570,7 @@ void refresh_scalars(cdet_ctx* ctx) {
         k.split_lo[kind][1] = t.split[1][1];
         k.abssum[kind][0] = t.abssum[0];
         k.abssum[kind][1] = t.abssum[1];
+        k.rowsum[kind] = t.rowsum;
     }
     for (int order = 0; order <= kMaxOrder; ++order) {
         k.group16_R[order] = ctx->tuning.group16_min_terms[order];
@@ -1470,7 +1711,13 @@ static int noalloc_scratch_of(int device, unsigned char** out) {
 }
 
 static constexpr size_t kMaxLds = 160 * 1024 - 2048;
-static constexpr int kDetGrid = 512;
+// Workgroups of k_fallback_dets: its launches are mostly empty (fallbacks are
+// rare) and each workgroup of the register form needs a quarter of a SIMD's
+// registers, so a small grid that loops over the queue is dispatched sooner
+// among other batches' kernels (32 against 512: +1.8% configurations/s with
+// four batches in flight).
+static constexpr int kDetGrid = 32;
+static constexpr int kDeferGrid = 16;  // workgroups of the deferred k_cdet_pair pass
 static constexpr int kGscrGrid = 1024;  // workgroups of the global-scratch forms
 static bool fpm_gscr(int n) { return FpmLds::bytes(n) > kMaxLds; }
 // k_cdet's static LDS (wave-wide chain terms) comes out of its dynamic budget.
@@ -1493,6 +1740,7 @@ static cdet_tuning default_tuning() {
         t.group16_min_terms[n] = 64;
         t.group64_min_terms[n] = 128;
     }
+    t.fused_minors_max_order = 1;
     return t;
 }
 static bool wide_minors(const cdet_ctx* ctx, int n) { return ctx->tuning.minors_threads[n] >= 1024; }
@@ -1509,6 +1757,8 @@ struct MinorsOut {
     bool paired = false;  // modes 0, 2: k_cdet takes both classes of a configuration
     const double* err_up = nullptr;  // mode 3: table-value error scale per Green pair
     const double* err_dn = nullptr;
+    int32_t* defer_list = nullptr;   // paired plans: k_minors_pair's deferred configurations (capacity count)
+    int32_t* defer_count = nullptr;  // two words, zero between runs
 };
 
 // Global scratch of one order bucket: the principal minors of every spin
@@ -1549,6 +1799,43 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     double* next = pm + 4 * static_cast<size_t>(count) * sc;
     const uint16_t* masks = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
     const int* popoff = ctx->d_popoff + n * (kMaxOrder + 2);
+    const bool fused = o.paired && o.defer_count && (mode == 0 || mode == 2) && n >= 2 &&
+                       n <= std::min(kPairOrderMax, ctx->tuning.fused_minors_max_order) && ctx->ks.same_spin[0] &&
+                       ctx->ks.same_spin[1];
+    if (fused) {
+        // minors and recursion in one workgroup per configuration; the
+        // configurations whose minors need the pivoted fallback finish in
+        // k_fallback_dets and the deferred k_cdet_pair
+        const size_t lds = MinorsPairLds::bytes(n, mode);
+#define CDET_MINORS_PAIR(NN)                                                                                          \
+    case NN:                                                                                                          \
+        hipLaunchKernelGGL(k_minors_pair<NN>, dim3(count), dim3(256), lds, stream, ctx->ks, NN, mode, count,         \
+                           cfg_list,                                                                                  \
+                           pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_list, o.fb_count,         \
+                           o.defer_list, o.defer_count);                                                              \
+        break;
+        switch (n) {
+            CDET_MINORS_PAIR(2)
+            CDET_MINORS_PAIR(3)
+            CDET_MINORS_PAIR(4)
+            CDET_MINORS_PAIR(5)
+            CDET_MINORS_PAIR(6)
+            CDET_MINORS_PAIR(7)
+            CDET_MINORS_PAIR(8)
+            CDET_MINORS_PAIR(9)
+            default:
+                CDET_MINORS_PAIR(10)
+        }
+#undef CDET_MINORS_PAIR
+        const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
+        hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
+                           cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+        hipLaunchKernelGGL(k_cdet_pair, dim3(kDeferGrid), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n,
+                           mode, 0, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_count,
+                           o.defer_list, o.defer_count);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     if (n >= 2) {
         const int items = (ctx->ks.same_spin[0] && ctx->ks.same_spin[1] ? 2 : 4) * count;
         if (n <= kWaveFpmMaxOrder) {
@@ -1602,7 +1889,8 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     const int paired = (mode != 1 && o.paired) ? 1 : 0;
     if (paired && n <= kPairOrderMax) {
         hipLaunchKernelGGL(k_cdet_pair, dim3(count), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n, mode,
-                           count, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_count);
+                           count, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_count,
+                           nullptr, nullptr);
         HIP_TRY(hipGetLastError());
         return 0;
     }
@@ -1738,6 +2026,33 @@ static int ensure_abssum_tables(cdet_ctx* ctx) {
             HIP_TRY(hipGetLastError());
             made = true;
         }
+    }
+    if (made) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        refresh_scalars(ctx);
+    }
+    return 0;
+}
+
+// Row sums of the lower-branch coefficient blocks of every resident table,
+// built once per table for the FP64 matrix-core mode: a lookup at v = 1
+// (every diagonal pair, i = j, and any pair of equal times) has T_b(v) = 1 for
+// all b, so its value is sum_a T_a(u) sum_b c_ab, a Clenshaw over one row of
+// sums instead of a block contraction.
+static int ensure_rowsum_tables(cdet_ctx* ctx) {
+    std::lock_guard<std::mutex> lock(ctx->derive_mu);
+    size_t nc, nk;
+    table_sizes(ctx, nc, nk);
+    const int64_t rows = static_cast<int64_t>(nc / static_cast<size_t>(ctx->p.cheb_nv));
+    bool made = false;
+    for (int kind = 0; kind < 4; ++kind) {
+        DevTable& t = ctx->tab[kind];
+        if (t.alias >= 0 || !t.lower || t.rowsum) continue;
+        HIP_TRY(hipMalloc(&t.rowsum, static_cast<size_t>(rows) * sizeof(double)));
+        hipLaunchKernelGGL(k_row_sums, dim3(static_cast<unsigned>((rows + 3) / 4)), dim3(256), 0, ctx->stream, t.lower,
+                           rows, ctx->p.cheb_nv, t.rowsum);
+        HIP_TRY(hipGetLastError());
+        made = true;
     }
     if (made) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2041,6 +2356,8 @@ static void invalidate_split(DevTable& t) {
         (void)hipFree(d);
         d = nullptr;
     }
+    (void)hipFree(t.rowsum);
+    t.rowsum = nullptr;
 }
 
 static int alloc_table(cdet_ctx* ctx, int kind) {
@@ -2282,6 +2599,11 @@ int cdet_ctx_set_tuning(cdet_ctx* ctx, const cdet_tuning* in) {
         }
         if (in->group16_min_terms[n] > 0) t.group16_min_terms[n] = std::max(16, in->group16_min_terms[n]);
         if (in->group64_min_terms[n] > 0) t.group64_min_terms[n] = std::max(64, in->group64_min_terms[n]);
+    }
+    if (in->fused_minors_max_order > 0) {
+        if (in->fused_minors_max_order > kPairOrderMax)
+            return fail(CDET_ERR_ARGUMENT, "fused_minors_max_order must be 1..10");
+        t.fused_minors_max_order = in->fused_minors_max_order;
     }
     ctx->tuning = t;
     refresh_scalars(ctx);  // plans made afterwards capture the new thresholds

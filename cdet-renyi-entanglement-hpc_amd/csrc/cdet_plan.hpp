@@ -44,6 +44,8 @@ struct cdet_batch_plan {
     double* d_single = nullptr;
     int32_t* d_fb_list = nullptr;
     int32_t* d_fb_count = nullptr;  // one queue length per order
+    int32_t* d_defer_list = nullptr;   // configurations k_minors_pair defers (bucket offsets as d_lists)
+    int32_t* d_defer_count = nullptr;  // two words per order
     double* d_eup = nullptr;        // error-bound kind: table-value error scale per Green pair
     double* d_edn = nullptr;
     // Green-pair reuse across runs (cdet_batch_plan_reuse_pairs)
@@ -99,7 +101,10 @@ static int plan_tables(cdet_batch_plan* p) {
         if (rc) return rc;
     }
     if (p->contract_mode == CDET_MODE_SPLIT32) return ensure_split_tables(p->ctx);
-    if (p->contract_mode == CDET_MODE_MFMA64) return ensure_step_tables(p->ctx);
+    if (p->contract_mode == CDET_MODE_MFMA64) {
+        const int rc = ensure_rowsum_tables(p->ctx);
+        return rc ? rc : ensure_step_tables(p->ctx);
+    }
     return 0;
 }
 
@@ -119,7 +124,7 @@ static void plan_free(cdet_batch_plan* p) {
                     p->d_qrank, p->d_qu, p->d_qv, p->d_hist, p->d_keyoff, p->d_tkey, p->d_tstart, p->d_tlen,
                     p->d_ntiles, p->d_su, p->d_sv, p->d_sdest, p->d_out_ord, p->d_single, p->d_fb_list,
                     p->d_fb_count, p->d_wgtot, p->scratch.ptr, p->d_eup, p->d_edn, p->d_prev_verts,
-                    p->d_prev_ok};
+                    p->d_prev_ok, p->d_defer_list, p->d_defer_count};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -214,6 +219,8 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
         MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n, p->paired};
         o.err_up = p->d_eup;
         o.err_dn = p->d_edn;
+        o.defer_list = p->d_defer_list + off;
+        o.defer_count = p->d_defer_count + 2 * n;
         int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off, cnt,
                                p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s);
         if (rc) return rc;
@@ -238,7 +245,7 @@ static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, int32_
                        reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
                        p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist, d_status, p->d_eup, p->d_edn,
                        static_cast<double>(ctx->ks.nu + ctx->ks.nv) * kUnitRoundoff, p->reuse, p->d_prev_verts,
-                       p->d_prev_ok, ctx->d_pair_cache);
+                       p->d_prev_ok, ctx->d_pair_cache, p->contract_mode == CDET_MODE_MFMA64 ? 1 : 0);
     const unsigned kwg = static_cast<unsigned>((p->nkeys + kScanKeys - 1) / kScanKeys);
     hipLaunchKernelGGL(k_key_totals, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot);
     hipLaunchKernelGGL(k_scan_keys, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot,
@@ -402,12 +409,15 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     rc |= dev_alloc(&p->d_single, p->eval_kind == 1 ? 4 * count : 1);
     rc |= dev_alloc(&p->d_fb_list, 4 * count);
     rc |= dev_alloc(&p->d_fb_count, kMaxOrder + 1);
+    rc |= dev_alloc(&p->d_defer_list, count);
+    rc |= dev_alloc(&p->d_defer_count, 2 * (kMaxOrder + 1));
     if (p->eval_kind == 3) {
         rc |= dev_alloc(&p->d_eup, p->nq);
         rc |= dev_alloc(&p->d_edn, p->nq);
     }
     if (!rc && (hipMemset(p->d_hist, 0, sizeof(int32_t) * static_cast<size_t>(p->nkeys) * kKeySub) != hipSuccess ||
-                hipMemset(p->d_fb_count, 0, sizeof(int32_t) * (kMaxOrder + 1)) != hipSuccess))
+                hipMemset(p->d_fb_count, 0, sizeof(int32_t) * (kMaxOrder + 1)) != hipSuccess ||
+                hipMemset(p->d_defer_count, 0, sizeof(int32_t) * 2 * (kMaxOrder + 1)) != hipSuccess))
         rc = 1;
     if (!rc && hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess) rc = 1;
     if (rc) {
@@ -484,6 +494,21 @@ int64_t cdet_batch_out_size(const cdet_batch_plan* plan) { return plan ? plan->o
 int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan) { return plan ? plan->vert_total : -1; }
 int64_t cdet_batch_query_count(const cdet_batch_plan* plan) { return plan ? plan->nq : -1; }
 int cdet_batch_plan_streams(const cdet_batch_plan* plan) { return plan ? plan->lanes : -1; }
+
+int cdet_batch_contracted(cdet_batch_plan* p, int64_t* lookups, int64_t* tiles) {
+    if (!p || !lookups || !tiles) return fail(CDET_ERR_ARGUMENT, "cdet_batch_contracted: bad argument");
+    if (!p->ran) return fail(CDET_ERR_ARGUMENT, "cdet_batch_contracted: the plan has not run");
+    HIP_TRY(hipEventSynchronize(p->done));
+    int32_t nt = 0;
+    HIP_TRY(hipMemcpy(&nt, p->d_ntiles, sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> len(static_cast<size_t>(std::max(nt, 0)));
+    if (nt > 0) HIP_TRY(hipMemcpy(len.data(), p->d_tlen, sizeof(int32_t) * len.size(), hipMemcpyDeviceToHost));
+    int64_t total = 0;
+    for (int32_t l : len) total += l;
+    *lookups = total;
+    *tiles = nt;
+    return 0;
+}
 
 int cdet_batch_run(cdet_batch_plan* p, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
                    int use_graph, void* stream) {

@@ -24,6 +24,7 @@ struct KScalars {
     const float* split_hi[4][2];       // FP32 hi/lo split of coeff (split-FP32 mode)
     const float* split_lo[4][2];
     const double* abssum[4][2];        // per coefficient block, sum of |c_ab| (error-bound kind)
+    const double* rowsum[4];           // lower branch, per block row a: sum over b of c_ab (FP64 matrix-core mode)
 
     // Coefficient block of a query key (class, branch, sitepair).
     __device__ inline void key_tables(int32_t key, const double*& cu, const double*& cd, bool& same) const {

@@ -266,7 +266,12 @@ int cdet_batch_plan_create(cdet_ctx* ctx, const int32_t* h_orders, int64_t count
 int cdet_batch_plan_destroy(cdet_batch_plan* plan);  /* before its context's cdet_ctx_destroy */
 int64_t cdet_batch_out_size(const cdet_batch_plan* plan);     /* doubles written by one run */
 int64_t cdet_batch_vertex_count(const cdet_batch_plan* plan); /* vertices read by one run */
-int64_t cdet_batch_query_count(const cdet_batch_plan* plan);  /* Green-pair lookups per run */
+int64_t cdet_batch_query_count(const cdet_batch_plan* plan);  /* Green-pair slots per run (2 sum n^2) */
+/* Lookups the plan's last run sent to the table contraction and the tiles it
+ * cut them into (waits for that run).  Corner pairs, reused pairs and, in the
+ * FP64 matrix-core mode, the diagonal pairs (evaluated from row sums) are not
+ * among them.  No counterpart in the reference (measurement). */
+int cdet_batch_contracted(cdet_batch_plan* plan, int64_t* lookups, int64_t* tiles);
 /* Streams a run spreads the plan's order buckets over (1 = the run's stream
  * only; up to 4 for batches of several orders, forked from and joined back to
  * the run's stream).  No counterpart in the reference. */
@@ -342,6 +347,9 @@ typedef struct cdet_tuning {
                                                       on 16 lanes per mask (64) */
     int32_t group64_min_terms[CDET_MAX_ORDER + 1]; /* per order: a single chain of at least this many
                                                       terms runs on a whole wavefront (128) */
+    int32_t fused_minors_max_order; /* paired order buckets up to this order (at most 10) run principal
+                                       minors and subset recursion in one workgroup per configuration,
+                                       the minors kept in LDS; 1 = none (1) */
 } cdet_tuning;
 int cdet_ctx_set_tuning(cdet_ctx* ctx, const cdet_tuning* tuning);
 int cdet_ctx_get_tuning(cdet_ctx* ctx, cdet_tuning* out); /* the values in effect */

@@ -237,7 +237,8 @@ def test_paired_classes_bit_identical(gpu_available):
     # classes of a configuration in one workgroup and combines them in place;
     # values and magnitude scales equal those of the same configurations in
     # small batches (one workgroup per class, separate combine pass) bit for
-    # bit, and the oracle's on a sample.
+    # bit, and the oracle's on a sample; with the principal minors in the same
+    # workgroup too (fused_minors_max_order) as well.
     import torch
     ctx = gpu_ctx("c1")
     orc = helpers.oracle_of("c1")
@@ -259,10 +260,79 @@ def test_paired_classes_bit_identical(gpu_available):
         big = run(configs, kind)
         small = np.concatenate([run(configs[i:i + 90], kind) for i in range(0, count, 90)])
         assert bits_equal(big, small)
+        try:
+            ctx.set_tuning({"fused_minors_max_order": 10})
+            say(f"paired classes kind {kind}, fused minors")
+            fused = run(configs, kind)
+        finally:
+            ctx.set_tuning({})
+        assert bits_equal(fused, small)
         if kind == 0:
             per = 1 << 6
             for k in range(0, count, 53):
                 assert bits_equal(big[k * per:(k + 1) * per], orc.all_subsets(configs[k])), k
+
+
+def test_fused_minors_deferred_fallback_bit_identical(gpu_available):
+    # Paired order buckets up to order 10 run the principal minors and the
+    # recursion in one workgroup per configuration; a configuration whose
+    # matrix trips TOL_FPM is deferred to the pivoted-determinant pass and a
+    # second recursion pass.  At TOL_FPM = 1e-3 about a fifth of the order-8
+    # and two thirds of the order-10 configurations of this set fall back, so
+    # both routes run in each bucket.  Graph replays must repeat the values
+    # (the deferred-queue words are reset by the pass that drains them), and
+    # every value equals the unpaired kernels' and the oracle's bit for bit.
+    ctx = gpu_ctx("c1", tol_fpm=1e-3)
+    orc = oracle_for("c1", tol_fpm=1e-3)
+    configs = helpers.seeded_configs(9750, [8] * 260 + [10] * 260)
+    try:
+        # paired from 256 configurations per bucket, principal minors and
+        # recursion fused up to order 10
+        ctx.set_tuning({"pair_min_per_cu": 1, "fused_minors_max_order": 10})
+        say("fused minors, deferred fallbacks: graph replays")
+        outs, ob, status = run_batch(ctx, configs, "exact", graph=True, repeats=3)
+        for o in outs[1:]:
+            assert bits_equal(o, outs[0])
+        deferred = np.flatnonzero(status & 2)
+        say(f"{len(deferred)} of {len(configs)} configurations took the fallback")
+        assert 0 < len(deferred) < len(configs)
+        assert not np.any(status & 1)
+        ctx.set_tuning({"pair_min_per_cu": 1 << 20})  # unpaired kernels
+        say("unpaired kernels, same batch")
+        (ref,), _, ref_status = run_batch(ctx, configs, "exact")
+        assert bits_equal(outs[0], ref)
+        assert np.array_equal(status, ref_status)
+    finally:
+        ctx.set_tuning({})
+    out = outs[0]
+    for k in sorted(set(deferred[::3].tolist()) | set(range(0, len(configs), 29))):
+        assert bits_equal(out[ob[k]:ob[k + 1]], orc.all_subsets(configs[k])), k
+
+
+def test_contracted_lookups_exclude_diagonal_pairs(gpu_available):
+    # The FP64 matrix-core mode evaluates the diagonal pairs (i, i) -- lower
+    # branch at v = 1 exactly -- from the coefficient blocks' row sums while
+    # planning, so its contraction takes 2 n fewer lookups per configuration
+    # than the bit-identical form's, which takes every non-corner pair (no
+    # corners at these times: tau >= 1e-13 for every vertex here).
+    ctx = gpu_ctx("c1")
+    orders = [2, 5, 6, 7, 9, 10] * 5
+    configs = helpers.seeded_configs(9810, orders)
+    assert min(float(np.min(v[:, 0])) for v in configs) * helpers.SETS["c1"]["beta"] > 1e-13
+    counts = {}
+    for mode in ("exact", "mfma64"):
+        import torch
+        plan = ctx.plan(orders, mode=mode)
+        verts = torch.tensor(np.concatenate(configs), dtype=torch.float64, device="cuda")
+        out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+        plan.run(verts, out, None, use_graph=False, stream=torch.cuda.current_stream().cuda_stream)
+        counts[mode] = plan.contracted()
+        plan.close()
+    say(f"contracted lookups {counts}")
+    assert counts["exact"][0] == sum(2 * n * n for n in orders)
+    assert counts["exact"][0] - counts["mfma64"][0] == sum(2 * n for n in orders)
+    for lookups, tiles in counts.values():  # tiles of at most 64 lookups of one block
+        assert 0 < tiles and lookups <= 64 * tiles
 
 
 def rel_err(got, want):
@@ -708,6 +778,7 @@ def test_tuning_choices_bit_identical(gpu_available):
         {"contract_wg_per_cu": 4, "contract_tile": 64},
         {"contract_wg_per_cu": 5, "contract_tile": 16, "max_lanes": 1},
         {"pair_min_per_cu": 1, "max_lanes": 2},
+        {"pair_min_per_cu": 1, "fused_minors_max_order": 10},
         {"pair_min_per_cu": 1 << 20},
         {"minors_threads": {11: 1024, 12: 1024}, "group16_min_terms": {n: 32 for n in range(15)},
          "group64_min_terms": {n: 64 for n in range(15)}},
@@ -725,7 +796,7 @@ def test_tuning_choices_bit_identical(gpu_available):
                 else:
                     assert bits_equal(out, ref[mode]), (mode, ch)
         for bad in ({"contract_wg_per_cu": 9}, {"contract_tile": 32}, {"max_lanes": 5},
-                    {"minors_threads": {12: 512}}):
+                    {"minors_threads": {12: 512}}, {"fused_minors_max_order": 11}):
             with pytest.raises(RuntimeError):
                 ctx.set_tuning(bad)
     finally:

@@ -273,40 +273,45 @@ def test_paired_classes_bit_identical(gpu_available):
                 assert bits_equal(big[k * per:(k + 1) * per], orc.all_subsets(configs[k])), k
 
 
-def test_fused_minors_deferred_fallback_bit_identical(gpu_available):
-    # Paired order buckets up to order 10 run the principal minors and the
-    # recursion in one workgroup per configuration; a configuration whose
-    # matrix trips TOL_FPM is deferred to the pivoted-determinant pass and a
-    # second recursion pass.  At TOL_FPM = 1e-3 about a fifth of the order-8
-    # and two thirds of the order-10 configurations of this set fall back, so
-    # both routes run in each bucket.  Graph replays must repeat the values
-    # (the deferred-queue words are reset by the pass that drains them), and
-    # every value equals the unpaired kernels' and the oracle's bit for bit.
+def test_fallback_configurations_in_paired_buckets_bit_identical(gpu_available):
+    # Paired order buckets up to order 10 with many pivoted-determinant
+    # fallbacks: the unfused kernels queue the flagged matrices between the
+    # principal minors and the recursion; the fused minors-and-recursion
+    # kernel defers such a configuration to the fallback pass and a second
+    # recursion pass.  At TOL_FPM = 1e-3 about a fifth of the order-8 and two
+    # thirds of the order-10 configurations of this set fall back, so both
+    # routes run in each bucket.  Graph replays must repeat the values (the
+    # deferral words are reset by the runs that drain them), and every value
+    # and status word equals the unpaired kernels' and the values the
+    # oracle's, bit for bit.
     ctx = gpu_ctx("c1", tol_fpm=1e-3)
     orc = oracle_for("c1", tol_fpm=1e-3)
     configs = helpers.seeded_configs(9750, [8] * 260 + [10] * 260)
+    runs = {}
     try:
-        # paired from 256 configurations per bucket, principal minors and
-        # recursion fused up to order 10
-        ctx.set_tuning({"pair_min_per_cu": 1, "fused_minors_max_order": 10})
-        say("fused minors, deferred fallbacks: graph replays")
-        outs, ob, status = run_batch(ctx, configs, "exact", graph=True, repeats=3)
-        for o in outs[1:]:
-            assert bits_equal(o, outs[0])
-        deferred = np.flatnonzero(status & 2)
-        say(f"{len(deferred)} of {len(configs)} configurations took the fallback")
-        assert 0 < len(deferred) < len(configs)
-        assert not np.any(status & 1)
+        # paired from 256 configurations per bucket; unfused, then fused
+        for name, tuning in (("paired", {"pair_min_per_cu": 1}),
+                             ("fused", {"pair_min_per_cu": 1, "fused_minors_max_order": 10})):
+            ctx.set_tuning(tuning)
+            say(f"fallbacks in paired buckets, {name}: graph replays")
+            outs, ob, status = run_batch(ctx, configs, "exact", graph=True, repeats=3)
+            for o in outs[1:]:
+                assert bits_equal(o, outs[0]), name
+            runs[name] = (outs[0], status)
         ctx.set_tuning({"pair_min_per_cu": 1 << 20})  # unpaired kernels
         say("unpaired kernels, same batch")
         (ref,), _, ref_status = run_batch(ctx, configs, "exact")
-        assert bits_equal(outs[0], ref)
-        assert np.array_equal(status, ref_status)
     finally:
         ctx.set_tuning({})
-    out = outs[0]
+    deferred = np.flatnonzero(ref_status & 2)
+    say(f"{len(deferred)} of {len(configs)} configurations took the fallback")
+    assert 0 < len(deferred) < len(configs)
+    assert not np.any(ref_status & 1)
+    for name, (out, status) in runs.items():
+        assert bits_equal(out, ref), name
+        assert np.array_equal(status, ref_status), name
     for k in sorted(set(deferred[::3].tolist()) | set(range(0, len(configs), 29))):
-        assert bits_equal(out[ob[k]:ob[k + 1]], orc.all_subsets(configs[k])), k
+        assert bits_equal(ref[ob[k]:ob[k + 1]], orc.all_subsets(configs[k])), k
 
 
 def test_contracted_lookups_exclude_diagonal_pairs(gpu_available):

@@ -139,11 +139,12 @@ constexpr int kKeySub = 16;
 // looked up: they are lower-branch lookups at v = (2 tau) / tau - 1 = 1
 // exactly, where T_b(1) = 1 for every b, so G = sum_a T_a(u) (sum_b c_ab), a
 // Clenshaw over u of the block's row sums (ks.rowsum).  The rows of the
-// configuration's diagonal pairs are staged in LDS by coalesced workgroup
-// loads, kPlanRowDoubles doubles per pass, and one lane per pair runs the
-// Clenshaw from there.
+// configuration's diagonal pairs are staged in dynamic LDS (row_cap doubles,
+// sized by the plan for all rows of its highest order, split into passes
+// beyond kPlanRowCapMax) by coalesced workgroup loads, and one lane per pair
+// runs the Clenshaw from there.
 constexpr int kPlanPairsPerThread = (2 * kMaxOrder * kMaxOrder + 255) / 256;
-constexpr int kPlanRowDoubles = 1024;
+constexpr int kPlanRowCapMax = 6144;  // doubles (48 KB)
 
 __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restrict__ verts,
                                               const int32_t* __restrict__ cfg_order,
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
                                               int32_t* __restrict__ status, double* __restrict__ e_up,
                                               double* __restrict__ e_dn, double err_scale, int reuse,
                                               double* __restrict__ prev_verts, int32_t* __restrict__ prev_ok,
-                                              unsigned long long* __restrict__ cache, int rows_at_v1) {
+                                              unsigned long long* __restrict__ cache, int rows_at_v1, int row_cap) {
     __shared__ Vtx vs[2][kMaxOrder];
     __shared__ int pidx[kMaxOrder];
     __shared__ int s_hits, s_misses;
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
     __shared__ int64_t s_dsp[2 * kMaxOrder];   // its sitepair
     __shared__ int s_dku[2 * kMaxOrder];       // its up-spin table kind
     __shared__ int s_dok[2 * kMaxOrder];       // evaluated from row sums in this run
-    __shared__ double rows[kPlanRowDoubles];
+    extern __shared__ double rows[];             // row_cap doubles
     const int c = blockIdx.x;
     const int n = cfg_order[c];
     if (threadIdx.x < 2 * kMaxOrder) s_dok[threadIdx.x] = 0;
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(256) void k_plan(KScalars ks, const double* __restr
         __syncthreads();
         const int nu = ks.nu;
         const bool dn_rows = ks.rowsum[0] != ks.rowsum[1] || ks.rowsum[2] != ks.rowsum[3];
-        const int per_pass = kPlanRowDoubles / (nu * (dn_rows ? 2 : 1));
+        const int per_pass = row_cap / (nu * (dn_rows ? 2 : 1));
         for (int k0 = 0; k0 < 2 * n; k0 += per_pass) {
             const int kc = min(per_pass, 2 * n - k0);
             for (int idx = threadIdx.x; idx < kc * nu; idx += blockDim.x) {

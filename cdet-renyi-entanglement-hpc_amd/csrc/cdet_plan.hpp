@@ -241,11 +241,21 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
 // cleared at plan creation).
 static void plan_group(cdet_batch_plan* p, const cdet_vertex* d_vertices, int32_t* d_status, hipStream_t s) {
     cdet_ctx* ctx = p->ctx;
-    hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256), 0, s, ctx->ks,
+    // LDS for the row sums of the diagonal pairs (FP64 matrix-core mode): all
+    // 2 n rows of the plan's highest order in one pass, up to kPlanRowCapMax
+    const int rows_at_v1 = p->contract_mode == CDET_MODE_MFMA64 ? 1 : 0;
+    int nmax = 1;
+    for (int n = 1; n <= kMaxOrder; ++n)
+        if (p->bucket_cnt[static_cast<size_t>(n)]) nmax = n;
+    const bool dn_rows = ctx->ks.rowsum[0] != ctx->ks.rowsum[1] || ctx->ks.rowsum[2] != ctx->ks.rowsum[3];
+    const int spins = dn_rows ? 2 : 1;
+    const int row_cap = rows_at_v1 ? std::min(kPlanRowCapMax, std::max(2 * nmax, 1) * ctx->ks.nu * spins) : 0;
+    hipLaunchKernelGGL(k_plan, dim3(static_cast<unsigned>(p->count)), dim3(256),
+                       sizeof(double) * static_cast<size_t>(row_cap), s, ctx->ks,
                        reinterpret_cast<const double*>(d_vertices), p->d_order, p->d_vbase, p->d_pbase, p->d_gup,
                        p->d_gdn, p->d_qkey, p->d_qrank, p->d_qu, p->d_qv, p->d_hist, d_status, p->d_eup, p->d_edn,
                        static_cast<double>(ctx->ks.nu + ctx->ks.nv) * kUnitRoundoff, p->reuse, p->d_prev_verts,
-                       p->d_prev_ok, ctx->d_pair_cache, p->contract_mode == CDET_MODE_MFMA64 ? 1 : 0);
+                       p->d_prev_ok, ctx->d_pair_cache, rows_at_v1, row_cap);
     const unsigned kwg = static_cast<unsigned>((p->nkeys + kScanKeys - 1) / kScanKeys);
     hipLaunchKernelGGL(k_key_totals, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot);
     hipLaunchKernelGGL(k_scan_keys, dim3(kwg), dim3(kScanKeys), 0, s, p->d_hist, p->nkeys, p->tile, p->d_wgtot,

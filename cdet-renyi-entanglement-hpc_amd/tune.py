@@ -15,6 +15,9 @@ that bench.py (and any caller, through Context.set_tuning) applies:
   group16/64_min_terms[n]  per order: chain lengths from which the subset
                        recursion runs a popcount layer's chains on 16 lanes per
                        mask / a whole wavefront
+  fused_minors_max_order  paired buckets up to this order run principal minors
+                       and recursion in one workgroup per configuration (1:
+                       none, 10)
   inflight             batches in flight (plans on their own streams); a
                        property of the caller, recorded with the table
 
@@ -127,7 +130,8 @@ def candidates(orders):
     out = [("contract_wg_per_cu", [8, 7, 6, 5, 4]),
            ("contract_tile", [16, 64]),
            ("pair_min_per_cu", [1, 2, 4, 1 << 20]),
-           ("max_lanes", [1, 2, 3, 4])]
+           ("max_lanes", [1, 2, 3, 4]),
+           ("fused_minors_max_order", [1, 10])]
     for n in present:
         if n >= 11:
             out.append((("minors_threads", n), [256, 1024]))

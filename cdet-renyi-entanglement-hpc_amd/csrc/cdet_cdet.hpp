@@ -219,7 +219,7 @@ template <int G>
 __device__ __forceinline__ void group_chains(const double* a, double* dp, int beg, int end, int R, double thr,
                                              bool filter, const uint16_t* masks_by_pop, double* rhs, bool single,
                                              double* tb, int wave0 = 0, int nwaves = kFoldWaves) {
-    static_assert(G == 16 || G == 32 || G == 64, "groups fold eight terms at a time");
+    static_assert(G == 8 || G == 16 || G == 32 || G == 64, "groups fold eight terms at a time");
     constexpr int kPerWave = 64 / G;
     const int tid = threadIdx.x;
     const int w = tid >> 6;
@@ -280,10 +280,11 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
 // a, dp (and rhs, if non-null) hold class 0 at [0, 2^n) and class 1 at
 // [2^n, 2^(n+1)).  Each popcount layer runs the masks of both classes before
 // one barrier: one lane per (class, mask), or grouped chains with the two
-// classes' masks on different waves when a layer has a single long chain per
-// class, or the classes one after the other with all fold waves.  Every value
-// is formed by the same chain as in ztransform_layers, so the results are
-// identical to two separate runs.
+// classes' masks on different waves -- a single long chain per class on a
+// wave each, up to 16 masks per class on 8-lane groups with two waves per
+// class -- or the classes one after the other with all fold waves.  Every
+// value is formed by the same chain as in ztransform_layers, so the results
+// are identical to two separate runs.
 __device__ __forceinline__ void ztransform_layers_pair(const double* a, double* dp, int n, double thr, bool filter,
                                                        const uint16_t* masks_by_pop, const int* pop_off, double* rhs,
                                                        double* tb, int g16 = 64, int g64 = 128) {
@@ -311,6 +312,12 @@ __device__ __forceinline__ void ztransform_layers_pair(const double* a, double* 
             group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb);
             group_chains<64>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,
                              tb);
+        } else if (G == 16 && waves >= 4 && M <= 16) {
+            // both classes in one round: 8 lanes per mask, class c on waves
+            // 2c and 2c + 1 (16 groups per class)
+            group_chains<8>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb, 0, 2);
+            group_chains<8>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,
+                            tb, 2, 2);
         } else if (G == 16) {
             group_chains<16>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb);
             group_chains<16>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,

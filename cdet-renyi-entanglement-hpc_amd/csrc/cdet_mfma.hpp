@@ -417,13 +417,18 @@ __device__ inline void contract_f64_tile_slabs(const KScalars& ks, int g, int j,
     BasisK Au;
     Au.init(s_u[p], g);
     double G = 0.0, Gd = 0.0;
-    const int lane_off = 16 * g + j;
+    // byte offset of the lane's element within a step: the loads take a
+    // wave-uniform 64-bit base, this unsigned 32-bit offset and an immediate
+    // 512 r, with no per-step vector address arithmetic
+    const uint32_t lane_byte = static_cast<uint32_t>(16 * g + j) * 8u;
     double a[PF][4], b[PF][4];
     auto load = [&](double* x, double* y, int off) {
+        const char* bu = reinterpret_cast<const char*>(cu + __builtin_amdgcn_readfirstlane(off));
+        const char* bd = reinterpret_cast<const char*>(cd + __builtin_amdgcn_readfirstlane(off));
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            x[r] = cu[off + lane_off + 64 * r];
-            if (!SAME) y[r] = cd[off + lane_off + 64 * r];
+            x[r] = *reinterpret_cast<const double*>(bu + lane_byte + 512 * r);
+            if (!SAME) y[r] = *reinterpret_cast<const double*>(bd + lane_byte + 512 * r);
         }
     };
     const int slab_stride = groups * 256;  // doubles between the steps of one K group in two slabs

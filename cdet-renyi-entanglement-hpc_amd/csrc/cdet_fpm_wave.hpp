@@ -277,16 +277,43 @@ __device__ __forceinline__ int fpm_wave_minors(const KScalars& ks, int n, const 
     return kFpmWaveOk;
 }
 
+// Minors of one spin matrix (order n, n - fpm_wave_L(n) = M) by the calling
+// wave into dst[mask] (the mask's low L bits across lanes), or the matrix
+// queued as fb_entry for the pivoted-determinant fallback.
+template <int M>
+__device__ __forceinline__ void fpm_wave_matrix(const KScalars& ks, int n, const double* g, double shift, double* lds,
+                                                int lane, double* dst, int32_t* fb_list, int32_t* fb_count,
+                                                int32_t fb_entry) {
+    double b[1 << M];
+    if (fpm_wave_minors<M>(ks, n, g, shift, lds, lds + fpm_wave_e0(n), lane, b) == kFpmWaveFallback) {
+        if (lane == 0) {
+            const int slot = atomicAdd(fb_count, 1);
+            fb_list[slot] = fb_entry;
+        }
+        return;
+    }
+    const int T = 1 << fpm_wave_L(n);
+    if (lane < T) {
+#pragma unroll
+        for (int h = 0; h < (1 << M); ++h) dst[static_cast<size_t>(h) * T + lane] = b[h];
+    }
+}
+
 // One spin matrix per 64-thread workgroup.  Work item w as k_fpm: 2 (li,
 // class) pairs x 2 spins, or x 1 when both classes have identical spin tables.
-// Writes pm[(4 li + 2 class + spin) 2^n + mask], or queues the matrix for the
-// pivoted-determinant fallback exactly as k_fpm does.
+// Writes pm[(4 li + 2 class + spin) 2^pm_log + mask], or queues the matrix for
+// the pivoted-determinant fallback exactly as k_fpm does.  With cfg_order
+// non-null the configurations are of several orders: configuration c has
+// order cfg_order[c] <= n, and n only sized the LDS.  M = 0 takes
+// n - fpm_wave_L(n) per configuration (orders 2 .. kWaveFpmMaxOrder in one
+// launch); otherwise every order of the launch has n - fpm_wave_L(n) = M.
 template <int M>
 __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, const int32_t* __restrict__ cfg_list,
                                                  const int64_t* __restrict__ cfg_pbase,
                                                  const double* __restrict__ g_up, const double* __restrict__ g_dn,
                                                  double* __restrict__ pm, int32_t* __restrict__ fb_list,
-                                                 int32_t* __restrict__ fb_count) {
+                                                 int32_t* __restrict__ fb_count, const int32_t* __restrict__ cfg_order,
+                                                 int pm_log) {
     extern __shared__ double lds_dyn[];
     const int lane = threadIdx.x;
     const int w = blockIdx.x;
@@ -297,22 +324,29 @@ __global__ __launch_bounds__(64) void k_fpm_wave(KScalars ks, int n, int items, 
     const int spin = both_same ? 0 : (w & 1);
     const bool same = ks.same_spin[cls] != 0;
     if (spin == 1 && same) return;  // the up minors serve both spins
-    const int T = 1 << fpm_wave_L(n);
     const int c = cfg_list[li];
+    if (cfg_order) n = cfg_order[c];
     const double* g = (spin ? g_dn : g_up) + 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n * n;
-    double b[1 << M];
-    if (fpm_wave_minors<M>(ks, n, g, spin ? ks.ddn : ks.dup, lds_dyn, lds_dyn + fpm_wave_e0(n), lane, b) ==
-        kFpmWaveFallback) {
-        if (lane == 0) {
-            const int slot = atomicAdd(fb_count, 1);
-            fb_list[slot] = (li << 3) | (cls << 2) | (same ? 3 : (1 << spin));
+    const double shift = spin ? ks.ddn : ks.dup;
+    double* dst = pm + (static_cast<size_t>(4 * li + 2 * cls + spin) << pm_log);
+    const int32_t entry = (li << 3) | (cls << 2) | (same ? 3 : (1 << spin));
+    if constexpr (M == 0) {
+        switch (n - fpm_wave_L(n)) {
+            case 1:
+                fpm_wave_matrix<1>(ks, n, g, shift, lds_dyn, lane, dst, fb_list, fb_count, entry);
+                break;
+            case 2:
+                fpm_wave_matrix<2>(ks, n, g, shift, lds_dyn, lane, dst, fb_list, fb_count, entry);
+                break;
+            case 3:
+                fpm_wave_matrix<3>(ks, n, g, shift, lds_dyn, lane, dst, fb_list, fb_count, entry);
+                break;
+            default:
+                fpm_wave_matrix<4>(ks, n, g, shift, lds_dyn, lane, dst, fb_list, fb_count, entry);
+                break;
         }
-        return;
-    }
-    double* dst = pm + static_cast<size_t>(4 * li + 2 * cls + spin) * (size_t(1) << n);
-    if (lane < T) {
-#pragma unroll
-        for (int h = 0; h < (1 << M); ++h) dst[static_cast<size_t>(h) * T + lane] = b[h];
+    } else {
+        fpm_wave_matrix<M>(ks, n, g, shift, lds_dyn, lane, dst, fb_list, fb_count, entry);
     }
 }
 

@@ -773,23 +773,24 @@ __global__ __launch_bounds__(NT) void k_fpm(KScalars ks, int n, int items, const
 // up to order 10 (det_minor_regs) and interleaved in LDS above
 // (det_minor_brute).  Counts the fallbacks
 // (include/cdet/runtime_counters.h:27) and flags the configuration's status.
+// With cfg_order non-null configuration c has order cfg_order[c] <= n (n sizes
+// the chunks and the LDS); minors go to pm[(4 li + 2 class + spin) 2^pm_log +
+// mask].
 // (Working matrices in LDS at every order, in 16-thread workgroups, measured
 // slower: the order-10 minors stage 94 -> 120 us.)
 constexpr int kDetThreads = 64;
 constexpr int kDetRegMax = 10;  // orders whose fallback determinants run in registers
 template <bool REG>
 __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
-    KScalars ks, int n, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
+    KScalars ks, int n_max, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
     const double* __restrict__ g_up, const double* __restrict__ g_dn, const int32_t* __restrict__ fb_list,
     const int32_t* __restrict__ fb_count, double* __restrict__ pm, int32_t* __restrict__ status,
-    uint64_t* __restrict__ fallback_total) {
+    uint64_t* __restrict__ fallback_total, const int32_t* __restrict__ cfg_order, int pm_log) {
     extern __shared__ double lds_dyn[];
     const int tid = threadIdx.x;
-    const int n2 = n * n;
-    const uint32_t sc = 1u << n;
-    const int chunks = static_cast<int>((sc + kDetThreads - 1) / kDetThreads);
+    const int chunks = static_cast<int>(((1u << n_max) + kDetThreads - 1) / kDetThreads);
     double* A = lds_dyn;
-    double* ws = lds_dyn + n2;
+    double* ws = lds_dyn + n_max * n_max;
     const int total = *fb_count * 2 * chunks;
     for (int w = blockIdx.x; w < total; w += gridDim.x) {
         const int e = w / (2 * chunks);
@@ -802,6 +803,9 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
         // spin 1 of identical spin tables is never read (the up minors serve both)
         if (!((spins >> spin) & 1) || (spin == 1 && ks.same_spin[cls])) continue;
         const int c = cfg_list[li];
+        const int n = cfg_order ? cfg_order[c] : n_max;
+        const int n2 = n * n;
+        const uint32_t sc = 1u << n;
         if (chunk == 0 && tid == 0 && spin == __builtin_ctz(spins)) {
             atomicAdd(reinterpret_cast<unsigned long long*>(fallback_total),
                       static_cast<unsigned long long>(__builtin_popcount(spins)));
@@ -826,22 +830,27 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
                 else
                     det = det_minor_brute(A, n, m, ws + tid, kDetThreads);
             }
-            pm[static_cast<size_t>(4 * li + 2 * cls + spin) * sc + m] = det;
+            pm[(static_cast<size_t>(4 * li + 2 * cls + spin) << pm_log) + m] = det;
         }
     }
 }
 
+// masks_by_pop / pop_off: the tables of all orders (order k at k 2^kMaxOrder
+// and k (kMaxOrder + 2)).  With cfg_order non-null configuration c has order
+// cfg_order[c] <= n (n sized the LDS or gscratch_stride); the minors of list
+// entry li are at pm + (4 li + 2 class + spin) 2^pm_log.
 template <bool GSCR, int NT>
-__global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int items,
+__global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, int items,
                                               const int32_t* __restrict__ cfg_list,
                                               const int64_t* __restrict__ cfg_pbase,
                                               const int64_t* __restrict__ cfg_obase, const double* __restrict__ g_up,
                                               const double* __restrict__ g_dn, const double* __restrict__ pm,
                                               double* __restrict__ out, double* __restrict__ out_ord,
                                               double* __restrict__ single_tmp, int32_t* __restrict__ status,
-                                              const uint16_t* __restrict__ masks_by_pop,
-                                              const int* __restrict__ pop_off, double* __restrict__ gscratch,
-                                              size_t gscratch_stride, int32_t* __restrict__ fb_count, int paired) {
+                                              const uint16_t* __restrict__ masks_all,
+                                              const int* __restrict__ pop_off_all, double* __restrict__ gscratch,
+                                              size_t gscratch_stride, int32_t* __restrict__ fb_count, int paired,
+                                              const int32_t* __restrict__ cfg_order, int pm_log) {
     extern __shared__ double lds_dyn[];
     __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
     __shared__ int s_popoff[kMaxOrder + 2];  // popcount-layer offsets, read every layer
@@ -849,16 +858,8 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
     // before); clear its length for the next run.
     if (fb_count && blockIdx.x == 0 && threadIdx.x == 0) *fb_count = 0;  // wave-wide chain terms of the subset recursion
     const int tid = threadIdx.x;
-    for (int k = tid; k < kMaxOrder + 2; k += blockDim.x) s_popoff[k] = pop_off[k];
     const int nt = blockDim.x;
-    const uint32_t sc = 1u << n;
-    const int n2 = n * n;
     double* a = GSCR ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
-    double* dp = a + sc;
-    double* rhs = dp + sc;  // mode 2 only
-    double* dU = rhs + (mode == 2 ? sc : 0);
-    double* dD = dU + n;
-    uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + n);
     // paired (modes 0 and 2): one work item per configuration, its two
     // classes in turn, the ordinary class combined into the twisted values in
     // place; otherwise one work item per (configuration, class)
@@ -868,6 +869,16 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
         const int li = paired ? w : (w >> 1);
         const int cls = paired ? ci : (w & 1);
         const int c = cfg_list[li];
+        const int n = cfg_order ? cfg_order[c] : n_max;
+        const uint32_t sc = 1u << n;
+        const int n2 = n * n;
+        double* dp = a + sc;
+        double* rhs = dp + sc;  // mode 2 only
+        double* dU = rhs + (mode == 2 ? sc : 0);
+        double* dD = dU + n;
+        uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + n);
+        const uint16_t* masks_by_pop = masks_all + (static_cast<size_t>(n) << kMaxOrder);
+        for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off_all[n * (kMaxOrder + 2) + k];
         const bool same = ks.same_spin[cls] != 0;
         const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
         // unscaled shifted diagonal entries for the singletons (:751-758)
@@ -884,8 +895,8 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n, int mode, int i
             __syncthreads();
             continue;
         }
-        const double* pu = pm + static_cast<size_t>(4 * li + 2 * cls) * sc;
-        const double* pd = same ? pu : pu + sc;
+        const double* pu = pm + (static_cast<size_t>(4 * li + 2 * cls) << pm_log);
+        const double* pd = same ? pu : pu + (size_t(1) << pm_log);
         // a[mask] = pm_up * pm_down with a[0] = 0; the global loads of up to
         // four masks per thread are issued before any product is stored
         {
@@ -1033,36 +1044,48 @@ __device__ __forceinline__ void pair_diagonals(const KScalars& ks, int n, int64_
 // configurations of defer_list (the fused kernel's deferred ones, below)
 // instead of cfg_list[0 .. items); the grid's last workgroup to finish
 // resets both queue words (defer_count[1] counts finished workgroups).
-__global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode, int items,
+// masks_all / pop_off_all hold the tables of every order; with cfg_order
+// non-null configuration c has order cfg_order[c] <= n (n sized the LDS) and
+// the minors of list entry li are at pm + (4 li + 2 class) 2^pm_log.
+__global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n_max, int mode, int items,
                                                    const int32_t* __restrict__ cfg_list,
                                                    const int64_t* __restrict__ cfg_pbase,
                                                    const int64_t* __restrict__ cfg_obase,
                                                    const double* __restrict__ g_up, const double* __restrict__ g_dn,
                                                    const double* __restrict__ pm, double* __restrict__ out,
                                                    int32_t* __restrict__ status,
-                                                   const uint16_t* __restrict__ masks_by_pop,
-                                                   const int* __restrict__ pop_off, int32_t* __restrict__ fb_count,
+                                                   const uint16_t* __restrict__ masks_all,
+                                                   const int* __restrict__ pop_off_all, int32_t* __restrict__ fb_count,
                                                    const int32_t* __restrict__ defer_list,
-                                                   int32_t* __restrict__ defer_count) {
+                                                   int32_t* __restrict__ defer_count,
+                                                   const int32_t* __restrict__ cfg_order, int pm_log) {
     extern __shared__ double lds_dyn[];
     __shared__ __attribute__((aligned(16))) double tb[kFoldWaves * kFoldSteps * 64];
     __shared__ int s_popoff[kMaxOrder + 2];
     if (fb_count && blockIdx.x == 0 && threadIdx.x == 0) *fb_count = 0;  // the bucket's fallback queue is consumed
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
-    for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off[k];
-    const uint32_t sc = 1u << n;
-    double* a = lds_dyn;
-    double* dp = a + 2 * sc;
-    double* rhs = dp + 2 * sc;  // mode 2 only
-    double* dU = rhs + (mode == 2 ? 2 * sc : 0);
-    double* dD = dU + 2 * n;
-    uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + 2 * n);
-    for (uint32_t m = tid; m < sc; m += nt) lmask[m] = masks_by_pop[m];
+    int n = -1;  // order of the tables in LDS
     if (defer_count) items = __atomic_load_n(defer_count, __ATOMIC_RELAXED);
     for (int w = blockIdx.x; w < items; w += gridDim.x) {
         const int li = defer_count ? defer_list[w] : w;
         const int c = cfg_list[li];
+        const int nc = cfg_order ? cfg_order[c] : n_max;
+        const uint32_t sc = 1u << nc;
+        double* a = lds_dyn;
+        double* dp = a + 2 * sc;
+        double* rhs = dp + 2 * sc;  // mode 2 only
+        double* dU = rhs + (mode == 2 ? 2 * sc : 0);
+        double* dD = dU + 2 * nc;
+        uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + 2 * nc);
+        if (nc != n) {
+            // a change of order moves lmask: every read of the previous
+            // configuration's tables ended at pair_finish's closing barrier
+            n = nc;
+            for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off_all[n * (kMaxOrder + 2) + k];
+            const uint16_t* masks_by_pop = masks_all + (static_cast<size_t>(n) << kMaxOrder);
+            for (uint32_t m = tid; m < sc; m += nt) lmask[m] = masks_by_pop[m];
+        }
         pair_diagonals(ks, n, 2 * cfg_pbase[c], g_up, g_dn, dU, dD, tid, nt);
         // a[class][mask] = pm_up * pm_down, a[class][0] = 0; four masks per
         // thread in flight
@@ -1076,8 +1099,8 @@ __global__ __launch_bounds__(256) void k_cdet_pair(KScalars ks, int n, int mode,
                     if (m < 2 * sc) {
                         const int cls = m >= sc ? 1 : 0;
                         const uint32_t mm = m - cls * sc;
-                        const double* pu = pm + static_cast<size_t>(4 * li + 2 * cls) * sc;
-                        const double* pd = ks.same_spin[cls] ? pu : pu + sc;
+                        const double* pu = pm + (static_cast<size_t>(4 * li + 2 * cls) << pm_log);
+                        const double* pd = ks.same_spin[cls] ? pu : pu + (size_t(1) << pm_log);
                         vu[q] = pu[mm];
                         vd[q] = pd[mm];
                     }
@@ -1777,11 +1800,25 @@ static size_t minors_scratch_bytes(int n, int count, int mode) {
     return b;
 }
 
-// Launch the minors and connected-value kernels of one order bucket.
+// Runs of several orders: orders 2 .. kMixedOrderMax, whose principal minors
+// all take the one-wave kernel (k_fpm_wave), and whose fallback determinants
+// all run in registers.  Up to order kMixedSmallMax, n - fpm_wave_L(n) = 1, so
+// the run's minors need no per-configuration subtree depth.
+constexpr int kMixedOrderMax = 10;
+constexpr int kMixedSmallMax = 7;
+static_assert(kMixedOrderMax <= kWaveFpmMaxOrder && kMixedOrderMax <= kDetRegMax && kMixedOrderMax <= kPairOrderMax,
+              "a mixed run takes the one-wave minors, register determinants and paired recursion");
+
+// Launch the minors and connected-value kernels of one order bucket, or, with
+// cfg_order non-null, of a run of configurations of orders 2 .. n (n <=
+// kMixedOrderMax, modes 0 - 2): configuration c has order cfg_order[c], and
+// the kernels lay out every configuration's minors at the stride of order n.
 static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const int32_t* cfg_list, int count,
                          const int64_t* pbase, const int64_t* obase, const double* gup, const double* gdn,
-                         const MinorsOut& o, hipStream_t stream) {
+                         const MinorsOut& o, hipStream_t stream, const int32_t* cfg_order = nullptr) {
     if (count <= 0) return 0;
+    if (cfg_order && (n < 2 || n > kMixedOrderMax || mode == 3))
+        return fail(CDET_ERR_ARGUMENT, "launch_minors: a run of mixed orders needs orders 2..7 and modes 0..2");
     int rc = grow(&scratch.ptr, scratch.cap, minors_scratch_bytes(n, count, mode));
     if (rc) return rc;
     const size_t sc = size_t(1) << n;
@@ -1800,7 +1837,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     double* next = pm + 4 * static_cast<size_t>(count) * sc;
     const uint16_t* masks = ctx->d_masks + static_cast<size_t>(n) * (1u << kMaxOrder);
     const int* popoff = ctx->d_popoff + n * (kMaxOrder + 2);
-    const bool fused = o.paired && o.defer_count && (mode == 0 || mode == 2) && n >= 2 &&
+    const bool fused = !cfg_order && o.paired && o.defer_count && (mode == 0 || mode == 2) && n >= 2 &&
                        n <= std::min(kPairOrderMax, ctx->tuning.fused_minors_max_order) && ctx->ks.same_spin[0] &&
                        ctx->ks.same_spin[1];
     if (fused) {
@@ -1830,34 +1867,40 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
 #undef CDET_MINORS_PAIR
         const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
         hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
-                           cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+                           cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks, nullptr,
+                           n);
         hipLaunchKernelGGL(k_cdet_pair, dim3(kDeferGrid), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n,
-                           mode, 0, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_count,
-                           o.defer_list, o.defer_count);
+                           mode, 0, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, ctx->d_masks, ctx->d_popoff,
+                           o.fb_count, o.defer_list, o.defer_count, nullptr, n);
         HIP_TRY(hipGetLastError());
         return 0;
     }
     if (n >= 2) {
         const int items = (ctx->ks.same_spin[0] && ctx->ks.same_spin[1] ? 2 : 4) * count;
-        if (n <= kWaveFpmMaxOrder) {
+        if (cfg_order && n > kMixedSmallMax) {
+            // subtree depth per configuration
+            const size_t lds = sizeof(double) * static_cast<size_t>(fpm_wave_lds_doubles(n));
+            hipLaunchKernelGGL(k_fpm_wave<0>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list, pbase,
+                               gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
+        } else if (n <= kWaveFpmMaxOrder) {
             // one wavefront per spin matrix, no workgroup barriers
             const size_t lds = sizeof(double) * static_cast<size_t>(fpm_wave_lds_doubles(n));
             switch (n - fpm_wave_L(n)) {
                 case 1:
                     hipLaunchKernelGGL(k_fpm_wave<1>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
-                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
                 case 2:
                     hipLaunchKernelGGL(k_fpm_wave<2>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
-                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
                 case 3:
                     hipLaunchKernelGGL(k_fpm_wave<3>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
-                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
                 default:
                     hipLaunchKernelGGL(k_fpm_wave<4>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
-                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count);
+                                       pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
             }
         } else if (fpm_gscr(n)) {
@@ -1880,18 +1923,20 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         if (n <= kDetRegMax) {
             const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
             hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
-                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
+                               cfg_order, n);
         } else {
             const size_t det_lds = sizeof(double) * (kDetThreads + 1) * static_cast<size_t>(n) * n;
             hipLaunchKernelGGL(k_fallback_dets<false>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
-                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks);
+                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
+                               cfg_order, n);
         }
     }
     const int paired = (mode != 1 && o.paired) ? 1 : 0;
     if (paired && n <= kPairOrderMax) {
         hipLaunchKernelGGL(k_cdet_pair, dim3(count), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n, mode,
-                           count, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_count,
-                           nullptr, nullptr);
+                           count, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, ctx->d_masks, ctx->d_popoff,
+                           o.fb_count, nullptr, nullptr, cfg_order, n);
         HIP_TRY(hipGetLastError());
         return 0;
     }
@@ -1900,21 +1945,23 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         const int grid = std::min(items, kGscrGrid);
         if (wide_minors(ctx, n))
             hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, mode, items,
-                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
-                               popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired);
+                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
+                               ctx->d_masks, ctx->d_popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired,
+                               cfg_order, n);
         else
             hipLaunchKernelGGL((k_cdet<true, 256>), dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items,
-                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, masks,
-                               popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired);
+                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
+                               ctx->d_masks, ctx->d_popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired,
+                               cfg_order, n);
     } else {
         if (wide_minors(ctx, n))
             hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ctx->ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
-                               o.status, masks, popoff, nullptr, 0, o.fb_count, paired);
+                               o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired, cfg_order, n);
         else
             hipLaunchKernelGGL((k_cdet<false, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
-                               o.status, masks, popoff, nullptr, 0, o.fb_count, paired);
+                               o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired, cfg_order, n);
     }
     HIP_TRY(hipGetLastError());
     return 0;

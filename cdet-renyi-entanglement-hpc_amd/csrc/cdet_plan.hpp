@@ -55,10 +55,21 @@ struct cdet_batch_plan {
     uint64_t reuse_tables = 0;       // table generation the previous values belong to
     Scratch scratch;                // minors global scratch of lane 0, sized at creation
     int contract_grid = 0;
-    // Order buckets spread over streams: bucket n runs on lane lane_of[n]
+    // Minors runs, highest order first: one per order bucket, except that
+    // the buckets of orders 2 .. 7 and the small ones of orders 8 .. 10 form a
+    // single run of mixed orders when more than one of them is non-empty
+    // (modes 0 - 2), which saves the launch chains of the small buckets.
+    struct MinorRun {
+        int n;    // the run's order (mixed: the highest)
+        int off;  // first list entry
+        int cnt;  // configurations
+        bool mixed;
+    };
+    std::vector<MinorRun> runs;
+    // Minors runs spread over streams: run r runs on lane run_lane[r]
     // (lane 0 = the run's stream, lane k > 0 = aux[k - 1], forked from and
     // joined back to the run's stream by events); each lane has its own
-    // minors scratch, since its buckets run one after another.  The aux
+    // minors scratch, since its runs run one after another.  The aux
     // streams belong to the context (or to the submit slot the plan serves)
     // and are shared by its plans.
     // Both classes of a configuration in one k_cdet workgroup, combined in
@@ -68,7 +79,7 @@ struct cdet_batch_plan {
     bool paired = false;
     int lanes = 1;
     int max_lanes = kMinorLanes;
-    std::vector<int> lane_of;
+    std::vector<int> run_lane;
     Scratch lane_scratch[kMinorLanes];  // [0] unused (lane 0 uses scratch)
     hipStream_t aux[kMinorLanes - 1] = {};
     hipEvent_t ev_fork = nullptr;
@@ -151,20 +162,51 @@ static double bucket_cost_us(int n, int count, int mode, int ncu) {
     return 10.0 + per_wg * rounds;
 }
 
-// Spread the order buckets over up to max_lanes streams, longest first onto
-// the least-loaded lane, when the buckets other than the longest carry enough
+// The plan's minors runs (cdet_batch_plan::runs), highest order first.  The
+// mixed run takes the buckets of orders 2 .. kMixedSmallMax and, above, each
+// next order's bucket while it holds fewer configurations than the device
+// has CUs: such a bucket alone is a chain of short launches that leave the
+// device mostly idle, and in the mixed run its workgroups only hold LDS sized
+// for the run's highest order.
+static void plan_runs(cdet_batch_plan* p) {
+    p->runs.clear();
+    int lo = 0, hi = 0, nonempty = 0;
+    for (int n = 2; n <= kMixedOrderMax; ++n) {
+        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
+        if (!cnt) continue;
+        if (n > kMixedSmallMax && cnt >= p->ctx->ncu) break;
+        if (!lo) lo = n;
+        hi = n;
+        ++nonempty;
+    }
+    const bool mix = p->eval_kind != 3 && nonempty >= 2;
+    for (int n = kMaxOrder; n >= 1; --n) {
+        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
+        if (!cnt) continue;
+        if (mix && n >= lo && n <= hi) {
+            if (n == hi) {
+                const int off = p->bucket_off[static_cast<size_t>(lo)];
+                p->runs.push_back({hi, off, p->bucket_off[static_cast<size_t>(hi + 1)] - off, true});
+            }
+            continue;
+        }
+        p->runs.push_back({n, p->bucket_off[static_cast<size_t>(n)], cnt, false});
+    }
+}
+
+// Spread the minors runs over up to max_lanes streams, longest first onto
+// the least-loaded lane, when the runs other than the longest carry enough
 // work to pay for the fork (about 10-15 us per forked stream); size each
 // lane's scratch and create the lanes' streams and events.
 static int plan_lanes(cdet_batch_plan* p, hipStream_t* lane_streams) {
     cdet_ctx* ctx = p->ctx;
-    p->lane_of.assign(kMaxOrder + 1, 0);
-    std::vector<std::pair<double, int>> order;  // (cost, n)
+    p->run_lane.assign(p->runs.size(), 0);
+    std::vector<std::pair<double, int>> order;  // (cost, run)
     double total = 0.0, longest = 0.0;
-    for (int n = 1; n <= kMaxOrder; ++n) {
-        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
-        if (!cnt) continue;
-        const double c = bucket_cost_us(n, cnt, p->eval_kind, ctx->ncu);
-        order.emplace_back(c, n);
+    for (size_t r = 0; r < p->runs.size(); ++r) {
+        const auto& run = p->runs[r];
+        const double c = bucket_cost_us(run.n, run.cnt, p->eval_kind, ctx->ncu);
+        order.emplace_back(c, -static_cast<int>(r));  // ties: the higher order (earlier run) first
         total += c;
         longest = std::max(longest, c);
     }
@@ -180,14 +222,14 @@ static int plan_lanes(cdet_batch_plan* p, hipStream_t* lane_streams) {
     for (const auto& b : order) {
         const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
         load[static_cast<size_t>(k)] += b.first;
-        p->lane_of[static_cast<size_t>(b.second)] = k;
+        p->run_lane[static_cast<size_t>(-b.second)] = k;
     }
     p->lanes = lanes;
     for (int k = 0; k < lanes; ++k) {
         size_t need = 0;
-        for (int n = 1; n <= kMaxOrder; ++n)
-            if (p->bucket_cnt[static_cast<size_t>(n)] && p->lane_of[static_cast<size_t>(n)] == k)
-                need = std::max(need, minors_scratch_bytes(n, p->bucket_cnt[static_cast<size_t>(n)], p->eval_kind));
+        for (size_t r = 0; r < p->runs.size(); ++r)
+            if (p->run_lane[r] == k)
+                need = std::max(need, minors_scratch_bytes(p->runs[r].n, p->runs[r].cnt, p->eval_kind));
         Scratch& sc = k ? p->lane_scratch[k] : p->scratch;
         int rc = grow(&sc.ptr, sc.cap, need);
         if (rc) return rc;
@@ -211,18 +253,19 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
         HIP_TRY(hipEventRecord(p->ev_fork, s));
         for (int k = 1; k < p->lanes; ++k) HIP_TRY(hipStreamWaitEvent(p->aux[k - 1], p->ev_fork, 0));
     }
-    for (int n = kMaxOrder; n >= 1; --n) {
-        const int cnt = p->bucket_cnt[static_cast<size_t>(n)];
-        if (!cnt) continue;
-        const int off = p->bucket_off[static_cast<size_t>(n)];
-        const int lane = p->lane_of[static_cast<size_t>(n)];
+    for (size_t r = 0; r < p->runs.size(); ++r) {
+        const auto& run = p->runs[r];
+        const int n = run.n;
+        const int off = run.off;
+        const int lane = p->run_lane[r];
         MinorsOut o{d_out, p->d_out_ord, p->d_single, d_status, p->d_fb_list + 4 * off, p->d_fb_count + n, p->paired};
         o.err_up = p->d_eup;
         o.err_dn = p->d_edn;
         o.defer_list = p->d_defer_list + off;
         o.defer_count = p->d_defer_count + 2 * n;
-        int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off, cnt,
-                               p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s);
+        int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off,
+                               run.cnt, p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s,
+                               run.mixed ? p->d_order : nullptr);
         if (rc) return rc;
     }
     if (p->lanes > 1) {
@@ -446,10 +489,10 @@ static int plan_create(cdet_ctx* ctx, const int32_t* orders, int64_t count, int 
     p->contract_grid = contract_grid(ctx, p->max_tiles, mode, p->tile);
     // Global scratch (minors of every spin matrix, workspaces of large
     // orders), sized now so that runs never allocate and can be graph-captured.
+    plan_runs(p);
     {
         int min_cnt = std::numeric_limits<int>::max();
-        for (int n = 1; n <= kMaxOrder; ++n)
-            if (p->bucket_cnt[static_cast<size_t>(n)]) min_cnt = std::min(min_cnt, p->bucket_cnt[static_cast<size_t>(n)]);
+        for (const auto& run : p->runs) min_cnt = std::min(min_cnt, run.cnt);
         p->paired = (p->eval_kind == 0 || p->eval_kind == 2) &&
                     static_cast<int64_t>(min_cnt) >= static_cast<int64_t>(ctx->tuning.pair_min_per_cu) * ctx->ncu;
     }

@@ -314,6 +314,66 @@ def test_fallback_configurations_in_paired_buckets_bit_identical(gpu_available):
         assert bits_equal(ref[ob[k]:ob[k + 1]], orc.all_subsets(configs[k])), k
 
 
+def test_mixed_order_runs_bit_identical(gpu_available):
+    # The order buckets 2..7 of a batch, and those of orders 8..10 with fewer
+    # configurations than the device has CUs, run as one run of mixed orders:
+    # one launch each of the principal minors, the fallback determinants and
+    # the recursion, every configuration at its own order.  At TOL_FPM = 0.9
+    # many matrices take the pivoted-determinant fallback.  Values and magnitude
+    # scales (kinds 0 and 2; unpaired and paired recursion kernels; eager and
+    # over graph replays) and single-order values (kind 1, order 1 beside the
+    # run) equal those of the same configurations in single-order batches bit
+    # for bit, and the oracle's on a sample.
+    ctx = gpu_ctx("c1", tol_fpm=0.9)
+    orc = oracle_for("c1", tol_fpm=0.9)
+    rng = np.random.default_rng(11)
+    orders = [int(x) for x in rng.permutation([int(x) for x in rng.integers(2, 8, size=300)] + [8] * 12 + [9] * 10
+                                             + [10] * 8)]
+    configs = helpers.seeded_configs(9900, orders)
+
+    def per_order(cfgs, kind, single=False):
+        # the same configurations in single-order batches, in batch order
+        res = [None] * len(cfgs)
+        for n in sorted(set(len(v) for v in cfgs)):
+            sel = [k for k, v in enumerate(cfgs) if len(v) == n]
+            (o,), ob1, st = run_batch(ctx, [cfgs[k] for k in sel], "exact", single=single, kind=kind)
+            for j, k in enumerate(sel):
+                res[k] = (o[ob1[j]:ob1[j + 1]], st[j])
+        return res
+
+    try:
+        for tname, tuning in (("unpaired", {"pair_min_per_cu": 1 << 20}), ("paired", {"pair_min_per_cu": 1})):
+            ctx.set_tuning(tuning)
+            for kind in (0, 2):
+                say(f"mixed-order run, {tname}, kind {kind}: graph replays")
+                outs, ob, status = run_batch(ctx, configs, "exact", graph=True, repeats=3, kind=kind)
+                for o in outs[1:]:
+                    assert bits_equal(o, outs[0]), (tname, kind)
+                (eager,), _, st_eager = run_batch(ctx, configs, "exact", kind=kind)
+                assert bits_equal(eager, outs[0]), (tname, kind)
+                assert np.array_equal(st_eager, status)
+                assert np.count_nonzero(status & 2) > 0
+                assert not np.any(status & 1)
+                say(f"mixed-order run, {tname}, kind {kind}: single-order batches")
+                ref = per_order(configs, kind)
+                for k in range(len(configs)):
+                    assert bits_equal(eager[ob[k]:ob[k + 1]], ref[k][0]), (tname, kind, k)
+                    assert status[k] == ref[k][1], (tname, kind, k)
+                if kind == 0:
+                    for k in range(0, len(configs), 7):
+                        assert bits_equal(eager[ob[k]:ob[k + 1]], orc.all_subsets(configs[k])), (tname, k)
+    finally:
+        ctx.set_tuning({})
+    orders1 = [int(x) for x in rng.integers(1, 11, size=96)]
+    configs1 = helpers.seeded_configs(9950, orders1)
+    say("mixed-order run, single-order values")
+    (out1,), _, _ = run_batch(ctx, configs1, "exact", single=True)
+    ref1 = per_order(configs1, None, single=True)
+    for k, v in enumerate(configs1):
+        assert bits_equal(out1[k], orc.single(v)), k
+        assert bits_equal(out1[k:k + 1], ref1[k][0]), k
+
+
 def test_contracted_lookups_exclude_diagonal_pairs(gpu_available):
     # The FP64 matrix-core mode evaluates the diagonal pairs (i, i) -- lower
     # branch at v = 1 exactly -- from the coefficient blocks' row sums while

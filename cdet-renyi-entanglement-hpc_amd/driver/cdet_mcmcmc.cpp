@@ -103,6 +103,8 @@ struct MaskPlan {  // :403-431
     std::vector<double> inv_binom;
 };
 
+constexpr int kMaxEnvelope = 48;  // highest envelope order (validate)
+
 MaskPlan make_plan(const cdet_mcmcmc_config& c) {
     MaskPlan p;
     p.subset_count = uint64_t{1} << c.envelope_order;
@@ -124,7 +126,7 @@ void validate(const cdet_mcmcmc_config& c) {  // :465-521
     if (c.order_max < c.order_min) throw std::runtime_error("MCMCMC_ORDER_MAX must be >= MCMCMC_ORDER_MIN");
     if (c.envelope_order <= c.order_max)
         throw std::runtime_error("MCMCMC_ENVELOPE_ORDER must be larger than MCMCMC_ORDER_MAX");
-    if (c.envelope_order > 48)
+    if (c.envelope_order > kMaxEnvelope)
         throw std::runtime_error("MCMCMC_ENVELOPE_ORDER must be <= 48 for the current principal-minor backend");
     if (c.normalization_order < c.order_min || c.normalization_order > c.order_max)
         throw std::runtime_error("MCMCMC_NORMALIZATION_ORDER must lie inside the measured order range");
@@ -798,7 +800,13 @@ StepOut heat_bath_step(const cdet_mcmcmc_config& c, const MaskPlan& plan, const 
     uint64_t last_positive = 0;
     uint64_t chosen = 0;
     bool found = false;
-    for (size_t k = 0; k < nc; ++k) {
+    // The order signals (:563-571) accumulate in the same pass as the
+    // cumulative search, each sum in mask order as in two separate loops: the
+    // search's serial chain of additions then overlaps the signal work.
+    size_t k = 0;
+    for (; k < nc && !found; ++k) {
+        const int order = plan.orders[k];
+        order_signal[static_cast<size_t>(order)] += (fluxes[k] / norm) * plan.inv_binom[static_cast<size_t>(order)];
         const double w = weights[k];
         if (!(w > 0.0)) continue;
         if (std::isfinite(w)) last_positive = plan.masks[k];
@@ -806,16 +814,14 @@ StepOut heat_bath_step(const cdet_mcmcmc_config& c, const MaskPlan& plan, const 
         if (cumulative > threshold) {
             chosen = plan.masks[k];
             found = true;
-            break;
         }
+    }
+    for (; k < nc; ++k) {
+        const int order = plan.orders[k];
+        order_signal[static_cast<size_t>(order)] += (fluxes[k] / norm) * plan.inv_binom[static_cast<size_t>(order)];
     }
     if (!found) chosen = last_positive;
     const int chosen_order = __builtin_popcountll(chosen);
-    for (size_t k = 0; k < nc; ++k) {
-        const int order = plan.orders[k];
-        const double t = fluxes[k] / norm;
-        order_signal[static_cast<size_t>(order)] += t * plan.inv_binom[static_cast<size_t>(order)];
-    }
     // create_next_vertices_from_seed_into (:380-399): kept vertices in index
     // order, then each regenerated vertex is a child of a uniformly drawn
     // vertex of the list so far.
@@ -972,6 +978,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
         std::vector<int32_t> ho;
         std::vector<double> hout, hcond;
         std::vector<size_t> batch, hoff;
+        std::vector<size_t> by_size;  // positions in batch, largest envelope first
         size_t lo = 0, hi = 0;  // chain index range
         bool in_flight = false;
     } groups[2];
@@ -998,7 +1005,17 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
             gr.ho.push_back(ch.cfg.envelope_order);
             gr.hoff.push_back(gr.hoff.back() + (size_t{1} << ch.cfg.envelope_order));
         }
-        gr.hout.assign(gr.hoff.back(), 0.0);
+        // every value is written by the evaluation callback
+        gr.hout.resize(gr.hoff.back());
+        // the host threads take the chains largest first, so that the last
+        // ones to start are short (a stable counting sort by envelope order)
+        gr.by_size.resize(gr.batch.size());
+        {
+            size_t pos[kMaxEnvelope + 2] = {};
+            for (int e : gr.ho) ++pos[kMaxEnvelope - e + 1];
+            for (int e = 1; e <= kMaxEnvelope + 1; ++e) pos[e] += pos[e - 1];
+            for (size_t bi = 0; bi < gr.ho.size(); ++bi) gr.by_size[pos[kMaxEnvelope - gr.ho[bi]]++] = bi;
+        }
         return true;
     };
     auto submit = [&](int g) {
@@ -1021,14 +1038,15 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
         }
         if (rc != 0) throw std::runtime_error("integrand evaluation failed (status " + std::to_string(rc) + ")");
         if (gpu_cond) {
-            gr.hcond.assign(gr.hoff.back(), 0.0);
+            gr.hcond.resize(gr.hoff.back());  // every value is written by the callback
             const int rcc = be.cond(be.cond_user, hseed, gr.hv.data(), gr.ho.data(),
                                     static_cast<int64_t>(gr.batch.size()), gr.hcond.data());
             if (rcc != 0)
                 throw std::runtime_error("conditional generation probabilities failed (status " + std::to_string(rcc) +
                                          ")");
         }
-        pool.run(gr.batch.size(), [&](size_t bi) {
+        pool.run(gr.batch.size(), [&](size_t i) {
+            const size_t bi = gr.by_size[i];
             Chain& ch = chains[gr.batch[bi]];
             try {
                 advance(ch, gr.hout.data() + gr.hoff[bi], gpu_cond ? gr.hcond.data() + gr.hoff[bi] : nullptr);

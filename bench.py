@@ -507,8 +507,12 @@ def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):
     fn = ctypes.cast(L.cdet_eval_batch_host, ctypes.c_void_p)
     sub = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p) if groups == 2 else None
     wt = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p) if groups == 2 else None
+    # the value buffers in pinned memory: the values arrive by DMA
+    ha = ctypes.cast(L.cdet_host_alloc, ctypes.c_void_p)
+    hf = ctypes.cast(L.cdet_host_free, ctypes.c_void_p)
     t = time.perf_counter()
-    results, ratios, _ = mcmcmc.run_backend(c, fn, ctx.handle, envelopes=envelopes, submit_fn=sub, wait_fn=wt)
+    results, ratios, _ = mcmcmc.run_backend(c, fn, ctx.handle, envelopes=envelopes, submit_fn=sub, wait_fn=wt,
+                                            host_alloc_fn=ha, host_free_fn=hf)
     dt = time.perf_counter() - t
     run_sweeps.last_stats = mcmcmc.cross_chain_stats(results, 2, order_max)
     cnt = ctx.counters()

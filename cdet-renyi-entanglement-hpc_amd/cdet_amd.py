@@ -179,6 +179,8 @@ def _signatures():
         "cdet_eval_batch_host": (i, [vp, vp, P(ctypes.c_int32), i64, dp]),
         "cdet_eval_batch_submit": (i, [vp, i, vp, P(ctypes.c_int32), i64]),
         "cdet_eval_batch_wait": (i, [vp, i, dp]),
+        "cdet_host_alloc": (vp, [vp, ctypes.c_size_t]),
+        "cdet_host_free": (None, [vp, vp]),
         "cdet_ctx_set_batch_mode": (i, [vp, i]),
         "cdet_seed_cond_batch_host": (i, [vp, vp, vp, P(ctypes.c_int32), i64, dp]),
         "cdet_get_counters": (i, [vp, P(CdetCounters)]),

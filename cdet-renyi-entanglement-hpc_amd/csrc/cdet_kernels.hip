@@ -26,6 +26,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -1466,6 +1467,7 @@ struct cdet_ctx {
     size_t h_hv_cap = 0;
     void* h_hout = nullptr;
     size_t h_hout_cap = 0;
+    std::map<const char*, size_t> host_bufs;  // cdet_host_alloc: pinned buffers by start, bytes (host_mu)
     void* h_hstatus = nullptr;
     size_t h_hstatus_cap = 0;
     void* d_hv = nullptr;
@@ -2351,6 +2353,7 @@ int cdet_ctx_destroy(cdet_ctx* ctx) {
         if (l) (void)hipStreamDestroy(l);
     (void)hipHostFree(ctx->h_hv);
     (void)hipHostFree(ctx->h_hout);
+    for (auto& hb : ctx->host_bufs) (void)hipHostFree(const_cast<char*>(hb.first));
     (void)hipHostFree(ctx->h_hstatus);
     (void)hipFree(ctx->d_hv);
     (void)hipFree(ctx->d_hout);

@@ -758,6 +758,41 @@ static int grow_pinned(void** p, size_t& cap, size_t need) {
     return 0;
 }
 
+extern "C" void* cdet_host_alloc(void* user, size_t bytes) {
+    cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
+    if (!ctx || bytes == 0) {
+        fail(CDET_ERR_ARGUMENT, "cdet_host_alloc: bad argument");
+        return nullptr;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        fail(CDET_ERR_DEVICE, "cdet_host_alloc: hipHostMalloc failed");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
+    ctx->host_bufs[static_cast<const char*>(p)] = bytes;
+    return p;
+}
+
+extern "C" void cdet_host_free(void* user, void* p) {
+    cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
+    if (!ctx || !p) return;
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
+    auto it = ctx->host_bufs.find(static_cast<const char*>(p));
+    if (it == ctx->host_bufs.end()) return;
+    ctx->host_bufs.erase(it);
+    (void)hipHostFree(p);
+}
+
+// [p, p + bytes) inside one of the context's cdet_host_alloc buffers (host_mu held).
+static bool in_host_buf(const cdet_ctx* ctx, const void* p, size_t bytes) {
+    const char* c = static_cast<const char*>(p);
+    auto it = ctx->host_bufs.upper_bound(c);
+    if (it == ctx->host_bufs.begin()) return false;
+    --it;
+    return c >= it->first && c + bytes <= it->first + it->second;
+}
+
 extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, const int32_t* h_orders, int64_t count,
                                     double* h_out) {
     cdet_ctx* ctx = static_cast<cdet_ctx*>(user);
@@ -796,13 +831,16 @@ extern "C" int cdet_eval_batch_host(void* user, const cdet_vertex* h_vertices, c
     rc = cdet_batch_run(p, static_cast<const cdet_vertex*>(ctx->d_hv), static_cast<double*>(ctx->d_hout),
                         static_cast<int32_t*>(ctx->d_hstatus), 1, s);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(ctx->h_hout, ctx->d_hout, obytes, hipMemcpyDeviceToHost, s));
+    // values straight into a pinned caller buffer, else through the staging
+    const bool direct = in_host_buf(ctx, h_out, obytes);
+    HIP_TRY(hipMemcpyAsync(direct ? static_cast<void*>(h_out) : ctx->h_hout, ctx->d_hout, obytes,
+                           hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(ctx->h_hstatus, ctx->d_hstatus, sbytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     const int32_t* st = static_cast<const int32_t*>(ctx->h_hstatus);
     for (int64_t i = 0; i < count; ++i)
         if (st[i] & 1) return fail(CDET_ERR_NONFINITE, "Error (all-subset integrand): Value is not finite");
-    std::memcpy(h_out, ctx->h_hout, obytes);
+    if (!direct) std::memcpy(h_out, ctx->h_hout, obytes);
     return 0;
 }
 

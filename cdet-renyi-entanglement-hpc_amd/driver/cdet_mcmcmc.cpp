@@ -973,16 +973,49 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
     const bool async = be.submit != nullptr && be.wait != nullptr && c.chain_count >= 2;
     const int ngroups = async ? 2 : 1;
     const size_t half = static_cast<size_t>((c.chain_count + 1) / 2);
+    // Value buffer of a group: from the backend's allocator when it has one
+    // (pinned memory for a GPU backend), else a vector.
+    struct Values {
+        const cdet_mcmcmc_backend* be = nullptr;
+        std::vector<double> vec;
+        double* buf = nullptr;
+        size_t cap = 0;
+        Values() = default;
+        Values(const Values&) = delete;
+        Values& operator=(const Values&) = delete;
+        ~Values() {
+            if (buf) be->host_free(be->eval_user, buf);
+        }
+        // room for n values; every value is written by the evaluation callback
+        double* ensure(size_t n) {
+            if (!be || !be->host_alloc || !be->host_free) {
+                vec.resize(n);
+                return vec.data();
+            }
+            if (n > cap) {
+                if (buf) be->host_free(be->eval_user, buf);
+                buf = nullptr;
+                cap = 0;
+                buf = static_cast<double*>(be->host_alloc(be->eval_user, sizeof(double) * std::max<size_t>(n, 1)));
+                if (!buf) throw std::runtime_error("integrand value buffer allocation failed");
+                cap = n;
+            }
+            return buf;
+        }
+    };
     struct Group {
         std::vector<cdet_vertex> hv;
         std::vector<int32_t> ho;
-        std::vector<double> hout, hcond;
+        Values hout;
+        double* out = nullptr;  // hout's values of the current batch
+        std::vector<double> hcond;
         std::vector<size_t> batch, hoff;
         std::vector<size_t> by_size;  // positions in batch, largest envelope first
         size_t lo = 0, hi = 0;  // chain index range
         bool in_flight = false;
     } groups[2];
     for (int g = 0; g < ngroups; ++g) {
+        groups[g].hout.be = &be;
         groups[g].lo = ngroups == 1 ? 0 : (g == 0 ? 0 : half);
         groups[g].hi = ngroups == 1 ? chains.size() : (g == 0 ? half : chains.size());
     }
@@ -1005,8 +1038,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
             gr.ho.push_back(ch.cfg.envelope_order);
             gr.hoff.push_back(gr.hoff.back() + (size_t{1} << ch.cfg.envelope_order));
         }
-        // every value is written by the evaluation callback
-        gr.hout.resize(gr.hoff.back());
+        gr.out = gr.hout.ensure(gr.hoff.back());
         // the host threads take the chains largest first, so that the last
         // ones to start are short (a stable counting sort by envelope order)
         gr.by_size.resize(gr.batch.size());
@@ -1031,10 +1063,10 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
         int rc = 0;
         if (async) {
             gr.in_flight = false;
-            rc = be.wait(be.eval_user, g, gr.hout.data());
+            rc = be.wait(be.eval_user, g, gr.out);
         } else {
             rc = be.eval(be.eval_user, gr.hv.data(), gr.ho.data(), static_cast<int64_t>(gr.batch.size()),
-                         gr.hout.data());
+                         gr.out);
         }
         if (rc != 0) throw std::runtime_error("integrand evaluation failed (status " + std::to_string(rc) + ")");
         if (gpu_cond) {
@@ -1049,7 +1081,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
             const size_t bi = gr.by_size[i];
             Chain& ch = chains[gr.batch[bi]];
             try {
-                advance(ch, gr.hout.data() + gr.hoff[bi], gpu_cond ? gr.hcond.data() + gr.hoff[bi] : nullptr);
+                advance(ch, gr.out + gr.hoff[bi], gpu_cond ? gr.hcond.data() + gr.hoff[bi] : nullptr);
             } catch (const std::exception& e) {
                 ch.error = e.what();
                 ch.stopped = true;
@@ -1063,7 +1095,7 @@ void run_chains(const cdet_mcmcmc_config& c, const SeedKernel& seed, const cdet_
         for (int g = 0; g < ngroups; ++g)
             if (groups[g].in_flight) {
                 groups[g].in_flight = false;
-                (void)be.wait(be.eval_user, g, groups[g].hout.data());
+                (void)be.wait(be.eval_user, g, groups[g].out);
             }
     };
     try {

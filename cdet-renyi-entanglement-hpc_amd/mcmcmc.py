@@ -102,7 +102,8 @@ class Backend(ctypes.Structure):
 
     _fields_ = [("eval", ctypes.c_void_p), ("eval_user", ctypes.c_void_p),
                 ("cond", ctypes.c_void_p), ("cond_user", ctypes.c_void_p),
-                ("submit", ctypes.c_void_p), ("wait", ctypes.c_void_p)]
+                ("submit", ctypes.c_void_p), ("wait", ctypes.c_void_p),
+                ("host_alloc", ctypes.c_void_p), ("host_free", ctypes.c_void_p)]
 
 
 # cdet_batch_submit_fn / cdet_batch_wait_fn, for callbacks written in Python
@@ -406,15 +407,17 @@ def run_envelopes(config, envelopes, eval_fn, eval_user, seed=None, trace_steps=
 
 
 def run_backend(config, eval_fn, eval_user, cond_fn=None, cond_user=None, seed=None, envelopes=None,
-                trace_steps=0, submit_fn=None, wait_fn=None):
+                trace_steps=0, submit_fn=None, wait_fn=None, host_alloc_fn=None, host_free_fn=None):
     """Chains through a backend (cdet_mcmcmc_run_backend): integrand callback
     and, optionally, the conditional-generation callback of the histogram
-    seed and the asynchronous submit/wait pair (two groups of chains in
-    flight).  envelopes: per-chain envelope orders or None."""
+    seed, the asynchronous submit/wait pair (two groups of chains in flight)
+    and the allocator of the value buffers (cdet_host_alloc / cdet_host_free
+    for the GPU backend).  envelopes: per-chain envelope orders or None."""
     n = config.chain_count
     env = None if envelopes is None else np.ascontiguousarray(envelopes, dtype=np.int32)
     cast = lambda f: ctypes.cast(f, ctypes.c_void_p) if f is not None else None
-    be = Backend(cast(eval_fn), eval_user, cast(cond_fn), cond_user, cast(submit_fn), cast(wait_fn))
+    be = Backend(cast(eval_fn), eval_user, cast(cond_fn), cond_user, cast(submit_fn), cast(wait_fn),
+                 cast(host_alloc_fn), cast(host_free_fn))
     results = (ChainResult * n)()
     ratios = (ctypes.c_double * (MAXE + 1))()
     trace = np.zeros((n, max(trace_steps, 1)), dtype=np.uint64)

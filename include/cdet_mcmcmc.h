@@ -115,6 +115,15 @@ typedef int (*cdet_batch_submit_fn)(void* user, int slot, const cdet_vertex* h_v
                                     int64_t count);
 typedef int (*cdet_batch_wait_fn)(void* user, int slot, double* h_out);
 
+/*
+ * Optional allocator of the host value buffers handed to the evaluation
+ * callbacks (user = the eval user).  A GPU backend gives pinned memory, which
+ * its copies reach by DMA with no staging copy.  No counterpart in the
+ * reference, whose evaluator fills a std::vector.
+ */
+typedef void* (*cdet_host_alloc_fn)(void* user, size_t bytes);
+typedef void (*cdet_host_free_fn)(void* user, void* p);
+
 /* Per-chain results (MCMCMCChainResult, include/cdet/mcmcmc_sampler.h:120-134). */
 typedef struct cdet_mcmcmc_chain_result {
     uint64_t seed;
@@ -226,6 +235,8 @@ typedef struct cdet_mcmcmc_backend {
     void* cond_user;
     cdet_batch_submit_fn submit;  /* optional, with wait: two groups of chains in flight (user = eval_user) */
     cdet_batch_wait_fn wait;
+    cdet_host_alloc_fn host_alloc; /* optional, with host_free: allocator of the value buffers (user = eval_user) */
+    cdet_host_free_fn host_free;
 } cdet_mcmcmc_backend;
 
 /* cdet_mcmcmc_run_envelopes with a backend; envelope_orders may be NULL (every
@@ -249,6 +260,12 @@ int cdet_eval_batch_host(void* ctx, const cdet_vertex* h_vertices, const int32_t
 int cdet_eval_batch_submit(void* ctx, int slot, const cdet_vertex* h_vertices, const int32_t* h_orders,
                            int64_t count);
 int cdet_eval_batch_wait(void* ctx, int slot, double* h_out);
+/* cdet_host_alloc_fn / cdet_host_free_fn over a context: pinned host memory
+ * that the context remembers.  An h_out of cdet_eval_batch_host inside such
+ * a buffer receives the values by DMA, without the copy out of the context's
+ * staging.  Freed by cdet_host_free or with the context. */
+void* cdet_host_alloc(void* ctx, size_t bytes);
+void cdet_host_free(void* ctx, void* p);
 int cdet_ctx_set_batch_mode(cdet_ctx* ctx, int mode);
 /* Green-pair reuse of those plans (cdet_batch_plan_reuse_pairs), on by
  * default: consecutive calls are steps of the same chains, and a chain's

@@ -12,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 HIP_FROM_DRIVER_HEADER = ("cdet_eval_batch_host", "cdet_ctx_set_batch_mode", "cdet_seed_cond_batch_host",
-                          "cdet_eval_batch_submit", "cdet_eval_batch_wait", "cdet_ctx_set_pair_reuse")
+                          "cdet_eval_batch_submit", "cdet_eval_batch_wait", "cdet_ctx_set_pair_reuse",
+                          "cdet_host_alloc", "cdet_host_free")
 
 
 def functions_in(header):

@@ -216,6 +216,46 @@ def test_driver_two_groups_in_flight_gpu(gpu_available, tmp_path):
     ctx.close()
 
 
+def test_driver_pinned_value_buffers_gpu(gpu_available, tmp_path):
+    # Value buffers from cdet_host_alloc (pinned, filled by DMA with no
+    # staging copy) give the steps and statistics of the driver's own
+    # buffers, synchronously and with two groups in flight; a buffer inside
+    # an allocation, or a pageable one, still gets the values.
+    ctx = gpu_ctx()
+    L = cdet_amd.lib()
+    hfn = ctypes.cast(L.cdet_eval_batch_host, ctypes.c_void_p)
+    sfn = ctypes.cast(L.cdet_eval_batch_submit, ctypes.c_void_p)
+    wfn = ctypes.cast(L.cdet_eval_batch_wait, ctypes.c_void_p)
+    ha = ctypes.cast(L.cdet_host_alloc, ctypes.c_void_p)
+    hf = ctypes.cast(L.cdet_host_free, ctypes.c_void_p)
+    env = [7, 5, 6, 3, 7]
+    assert L.cdet_ctx_set_batch_mode(ctx.handle, cdet_amd.MODES["mfma64"]) == 0
+    c = c1_config(tmp_path, chain_count=5, checkpoint_enabled=0, lambda_warmup_steps=200, production_steps=300)
+    print("driver pinned buffers: vector buffers", flush=True)
+    r0, rat0, t0 = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=500)
+    for name, kw in (("sync", {}), ("async", {"submit_fn": sfn, "wait_fn": wfn})):
+        print(f"driver pinned buffers: {name}", flush=True)
+        r1, rat1, t1 = mcmcmc.run_backend(c, hfn, ctx.handle, envelopes=env, trace_steps=500, host_alloc_fn=ha,
+                                          host_free_fn=hf, **kw)
+        assert np.array_equal(t0, t1), name
+        assert np.array_equal(rat0, rat1, equal_nan=True), name
+        for a, b in zip(r0, r1):
+            assert list(a.order_signal_mean) == list(b.order_signal_mean), name
+    v = np.array(helpers.seeded_configs(4321, [6])[0])
+    o = np.array([6], dtype=np.int32)
+    vp = v.ctypes.data_as(ctypes.c_void_p)
+    op = o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    ref = np.zeros(64)
+    assert L.cdet_eval_batch_host(ctx.handle, vp, op, 1, ref.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    buf = L.cdet_host_alloc(ctx.handle, 8 * 80)
+    assert buf
+    inner = (ctypes.c_double * 64).from_address(buf + 8 * 16)
+    assert L.cdet_eval_batch_host(ctx.handle, vp, op, 1, inner) == 0
+    assert np.array_equal(np.ctypeslib.as_array(inner), ref)
+    L.cdet_host_free(ctx.handle, buf)
+    ctx.close()
+
+
 def test_driver_two_groups_forked_slot_plans(gpu_available, tmp_path):
     # Envelopes up to 12: each slot's plan spreads its order buckets over two
     # streams and launches from the slot's own thread; the chains still take

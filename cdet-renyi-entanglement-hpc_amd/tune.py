@@ -26,7 +26,8 @@ configurations per second of graph-replayed batches, with `inflight` batches
 in flight for C2 (searched over 1..6) and one at a time for C4 (the batch of
 the 256 chain envelopes) and C5.  The search is coordinate descent from the built-in defaults,
 two passes, each candidate timed as the median of several runs and kept only
-for a gain above 2%.  Every choice
+for a gain above 2% (C2, batches in flight) or 1% (C4, C5: one batch at a
+time, whose runs spread less).  Every choice
 gives bit-identical values; the tuner checks that the tuned batch reproduces
 the default batch's output bit for bit before writing the table.
 
@@ -166,7 +167,9 @@ def tune(workload, mode="mfma64", steps=None, passes=2, inflight_choices=(1, 2, 
     # above `gain`, well outside the run-to-run spread
     steps = steps or {"c2": 20, "c4": 40, "c5": 6}.get(workload, 20)
     reps = {"c2": 15, "c4": 5, "c5": 3}.get(workload, 3)
-    gain = 1.02
+    # batches in flight (C2) spread wider from run to run than one batch at a
+    # time (C4, C5)
+    gain = 1.02 if workload == "c2" else 1.01
     base = ctx.tuning()
     reference = batch.run_once()
     inflight = 3 if workload == "c2" else 1  # as bench.py runs each workload

@@ -425,6 +425,7 @@ def main_c3(args):
     torch.cuda.synchronize()
     build_s = time.time() - t0
     print(f"c3: tables built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    tuning_rec, _ = apply_tuning(ctx, "c3", args)
     qi, qt = c3_queries()
     n = len(qi)
     d_qi = torch.tensor(qi, device=dev)
@@ -466,6 +467,7 @@ def main_c3(args):
         "config": {"workload": "8x8 Hubbard, beta=8, 4x4 corner subsystem, 128x128 Chebyshev; 2^20 seeded "
                                "twisted-table lookups per step (planning, grouping, contraction)",
                    "table_build_s": round(build_s, 2)},
+        "tuning": tuning_rec,
         "fp64_clenshaw_lookups_per_s": round(rates["exact"][0], 1),
         "speedup_vs_fp64_clenshaw": round(rates["mfma64"][0] / rates["exact"][0], 2),
         "whole_call_fp64_tflops": round(flops / rates["mfma64"][1] / 1e12, 3),

@@ -26,6 +26,7 @@ MODE_EXACT = 0
 MODE_MFMA64 = 1
 MODE_SPLIT32 = 2
 MODES = {"exact": MODE_EXACT, "mfma64": MODE_MFMA64, "split32": MODE_SPLIT32}
+SUM_ORDERS = {"tree": 0, "reference": 1}  # include/cdet_hip.h CDET_SUM_*
 
 TWISTED_UP, TWISTED_DOWN, ORDINARY_UP, ORDINARY_DOWN = 0, 1, 2, 3
 MASK_KINDS = {"empty": 0, "checkerboard": 1, "first_row": 2, "first_column": 3,
@@ -186,6 +187,7 @@ def _signatures():
         "cdet_get_counters": (i, [vp, P(CdetCounters)]),
         "cdet_ctx_set_tuning": (i, [vp, P(CdetTuning)]),
         "cdet_ctx_get_tuning": (i, [vp, P(CdetTuning)]),
+        "cdet_ctx_set_sum_order": (i, [vp, i]),
         "cdet_reset_counters": (i, [vp]),
         # include/cdet_chunks.h
         "cdet_chunks_write_plan": (i, [ctypes.c_char_p, P(ChunkSpec)]),
@@ -380,6 +382,13 @@ class Context:
         CdetTuning); plans created afterwards use it."""
         t = tuning if isinstance(tuning, CdetTuning) else CdetTuning.from_dict(tuning)
         _check(lib().cdet_ctx_set_tuning(self._h, ctypes.byref(t)))
+
+    def set_sum_order(self, order):
+        """Order of the subset recursion's sums in the non-exact modes:
+        "tree" (default) or "reference" (include/cdet_hip.h
+        cdet_ctx_set_sum_order); the exact mode always sums in the
+        reference's order."""
+        _check(lib().cdet_ctx_set_sum_order(self._h, SUM_ORDERS[order]))
 
     def tuning(self):
         t = CdetTuning()

@@ -215,6 +215,133 @@ __device__ __forceinline__ void group_chain_round(const double* a, double* dp, i
     }
 }
 
+// One popcount layer in tree order (the FP64 matrix-core and split-FP32
+// modes, whose values are not bit-identical to the reference anyway): G = 2^lg
+// lanes per mask, lane j of a group summing the terms x = G X + j of the
+// chain enumeration of group_chain_round (s_lo the j-th subset of the lowest
+// lg bits of the rest, s_hi descending over the subsets of the others), then
+// a butterfly over the group's lanes.  acc = a[mask] - (that sum), rhs =
+// |a[mask]| + (the sum of |terms|, alike), and the round-off filter of the
+// reference (include/cdet/integrand_replica_randomly_sampled.h:330-341) on
+// them.  The butterfly leaves the same bits in every lane of a group (a + b =
+// b + a), so a value depends on (n, k, lg) only, never on the workgroup shape.
+// Items q < classes M: class q / M at offset cls_stride, mask masks_by_pop[beg
+// + q mod M]; the loop is uniform per wave (a group is all valid or all
+// discarded), steps = R / G is a multiple of 8.  Per step a lane issues two
+// LDS reads and seven VALU instructions; the serial chain of the reference
+// order costs about as much per term on a single lane.
+__device__ __forceinline__ void tree_layer(const double* a, double* dp, double* rhs, uint32_t cls_stride,
+                                           int classes, int beg, int M, int R, int lg, double thr, bool filter,
+                                           const uint16_t* masks_by_pop, bool single) {
+    const int G = 1 << lg;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int waves = blockDim.x >> 6;
+    const int per_wave = 64 >> lg;
+    const int gi = lane >> lg;
+    const int j = lane & (G - 1);
+    const int items = classes * M;
+    const int steps = R >> lg;
+    for (int q0 = wave * per_wave; q0 < items; q0 += waves * per_wave) {
+        const int q = q0 + gi;
+        const bool have = q < items;
+        const int cls = (have && q >= M) ? 1 : 0;
+        const uint32_t off = cls ? cls_stride : 0u;
+        // a discarded group reads the layer's first mask: its loads stay in bounds
+        const uint32_t mask = masks_by_pop[beg + (have ? q - cls * M : 0)];
+        const bool rec = have && (!single || (mask & 1u));
+        const uint32_t anchor = mask & (~mask + 1u);
+        const uint32_t rest = mask ^ anchor;
+        uint32_t rl = 0, rr = rest;
+        for (int b = 0; b < lg; ++b) {
+            const uint32_t low = rr & (~rr + 1u);
+            rl |= low;
+            rr ^= low;
+        }
+        const uint32_t rh = rr;
+        uint32_t slo = 0;
+        {
+            uint32_t bl = rl;
+            int v = j;
+            for (int b = 0; b < lg; ++b) {
+                const uint32_t low = bl & (~bl + 1u);
+                if (v & 1) slo |= low;
+                v >>= 1;
+                bl ^= low;
+            }
+        }
+        const double* ac = a + off;
+        double* dc = dp + off;
+        // dp[anchor | s_lo | s_hi] and a[mask ^ sub] = a[(rest ^ s_lo) - s_hi]:
+        // one add from a per-lane base, s_hi kept as a byte offset
+        const char* dpb = reinterpret_cast<const char*>(dc + (anchor | slo));
+        const char* ab = reinterpret_cast<const char*>(ac + (rest ^ slo));
+        const uint32_t rh8 = rh << 3;
+        uint32_t s8 = rh8;
+        double part = 0.0, prs = 0.0;
+        for (int st = 0; st < steps; st += 8) {
+            double dv[8], av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                dv[u] = *reinterpret_cast<const double*>(dpb + s8);
+                av[u] = *reinterpret_cast<const double*>(ab - s8);
+                s8 = (s8 - 8u) & rh8;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                double prod = dv[u] * av[u];
+                // the first term of the last lane is sub = mask itself, no term
+                if (u == 0 && st == 0 && j == G - 1) prod = 0.0;
+                part += prod;
+                prs += fabs(prod);
+            }
+        }
+        for (int o = 1; o < G; o <<= 1) {
+            part += __shfl_xor(part, o);
+            prs += __shfl_xor(prs, o);
+        }
+        if (have && j == 0) {
+            if (rec) {
+                const double am = ac[mask];
+                double acc = am - part;
+                const double rs = fabs(am) + prs;
+                if (filter && fabs(acc) < thr * rs) acc = 0.0;
+                dc[mask] = acc;
+                if (rhs) rhs[off + mask] = rs;
+            } else {
+                dc[mask] = ac[mask];
+            }
+        }
+    }
+}
+
+// Lanes per mask (log2) of the tree-order form of popcount layer k at order n,
+// for the workgroups that usually run it: both classes of orders up to 10 on
+// 256 lanes, one class of orders 11, 12 on 256 and of 13, 14 on 1024.  Each
+// choice G = 2^lg (at least 8 steps per lane) is priced as rounds x (steps x 9
+// + lg x 8 + 30) instructions per lane, lg = 0 (the serial chain) as rounds x
+// (9 (R - 1) + 20); the cheapest wins.  A fixed function of (n, k), so the
+// values do not change with the launch shapes the tuning picks.
+__host__ inline int tree_lanes_log2(int n, int k) {
+    if (k < 2 || k > n) return 0;
+    double masks = 1.0;
+    for (int i = 0; i < k; ++i) masks = masks * (n - i) / (i + 1);
+    const long long M = static_cast<long long>(masks + 0.5) * (n <= 10 ? 2 : 1);
+    const long long L = n >= 13 ? 1024 : 256;
+    const long long R = 1LL << (k - 1);
+    long long best = ((M + L - 1) / L) * (9 * (R - 1) + 20);
+    int best_lg = 0;
+    for (int lg = 1; lg <= 6 && (R >> lg) >= 8; ++lg) {
+        const long long G = 1LL << lg;
+        const long long cost = ((M * G + L - 1) / L) * ((R >> lg) * 9 + lg * 8 + 30);
+        if (cost < best) {
+            best = cost;
+            best_lg = lg;
+        }
+    }
+    return best_lg;
+}
+
 template <int G>
 __device__ __forceinline__ void group_chains(const double* a, double* dp, int beg, int end, int R, double thr,
                                              bool filter, const uint16_t* masks_by_pop, double* rhs, bool single,
@@ -244,7 +371,7 @@ __device__ __forceinline__ void group_chains(const double* a, double* dp, int be
 // layer_group).  tb holds kFoldWaves x kFoldSteps x 64 doubles.
 __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, int n, double thr, bool filter,
                                          const uint16_t* masks_by_pop, const int* pop_off, double* rhs, bool single,
-                                         double* tb, int g16 = 64, int g64 = 128) {
+                                         double* tb, int g16 = 64, int g64 = 128, uint64_t tree_bits = 0) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
@@ -254,8 +381,11 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
         const int beg = pop_off[bits];
         const int end = pop_off[bits + 1];
         const int R = 1 << (bits - 1);
-        const int G = layer_group(end - beg, R, nt >> 6, g16, g64);
-        if (G == 64) {
+        const int lg = static_cast<int>((tree_bits >> (3 * bits)) & 7u);
+        const int G = lg ? -1 : layer_group(end - beg, R, nt >> 6, g16, g64);
+        if (lg) {
+            tree_layer(a, dp, rhs, 0u, 1, beg, end - beg, R, lg, thr, filter, masks_by_pop, single);
+        } else if (G == 64) {
             group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, single, tb);
         } else if (G == 16) {
             group_chains<16>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, single, tb);
@@ -287,7 +417,8 @@ __device__ __forceinline__ void ztransform_layers(const double* a, double* dp, i
 // are identical to two separate runs.
 __device__ __forceinline__ void ztransform_layers_pair(const double* a, double* dp, int n, double thr, bool filter,
                                                        const uint16_t* masks_by_pop, const int* pop_off, double* rhs,
-                                                       double* tb, int g16 = 64, int g64 = 128) {
+                                                       double* tb, int g16 = 64, int g64 = 128,
+                                                       uint64_t tree_bits = 0) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const int waves = nt >> 6;
@@ -302,8 +433,11 @@ __device__ __forceinline__ void ztransform_layers_pair(const double* a, double* 
         const int end = pop_off[bits + 1];
         const int M = end - beg;
         const int R = 1 << (bits - 1);
-        const int G = layer_group(M, R, waves, g16, g64);
-        if (G == 64 && waves >= 2) {
+        const int lg = static_cast<int>((tree_bits >> (3 * bits)) & 7u);
+        const int G = lg ? -1 : layer_group(M, R, waves, g16, g64);
+        if (lg) {
+            tree_layer(a, dp, rhs, sc, 2, beg, M, R, lg, thr, filter, masks_by_pop, false);
+        } else if (G == 64 && waves >= 2) {
             // one long chain per class: class c on wave c
             group_chains<64>(a, dp, beg, end, R, thr, filter, masks_by_pop, rhs, false, tb, 0, 1);
             group_chains<64>(a + sc, dp + sc, beg, end, R, thr, filter, masks_by_pop, rhs ? rhs + sc : nullptr, false,

@@ -929,7 +929,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
             // mode 2 writes the magnitude scale of each value instead of the value:
             // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
             ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, false, tb,
-                              ks.group16_R[n], ks.group64_R[n]);
+                              ks.group16_R[n], ks.group64_R[n], ks.tree_row(n));
             // Assembly (:833-877).
             // paired: twisted - 2 * ordinary (include/cdet/cdet_integrand_evaluator.h:348-350),
             // or for magnitude scales twisted + 2 * ordinary, formed in place;
@@ -955,7 +955,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
             if (bad && status) atomicOr(&status[c], 1);
         } else {
             ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, nullptr, true, tb, ks.group16_R[n],
-                              ks.group64_R[n]);
+                              ks.group64_R[n], ks.tree_row(n));
             const double cres = dp[sc - 1];
             if (tid == 0) {
                 const DD v = dd_mul(DD{ks.pref_hi[cls][n], ks.pref_lo[cls][n]}, DD{cres, 0.0});
@@ -1001,7 +1001,7 @@ __device__ __forceinline__ void pair_finish(const KScalars& ks, int n, int mode,
     const int nt = blockDim.x;
     const uint32_t sc = 1u << n;
     ztransform_layers_pair(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, tb,
-                           ks.group16_R[n], ks.group64_R[n]);
+                           ks.group16_R[n], ks.group64_R[n], ks.tree_row(n));
     bool bad = false;
     for (uint32_t m = tid; m < sc; m += nt) {
         double v[2];
@@ -1425,6 +1425,7 @@ struct cdet_ctx {
     unsigned char* noalloc_scratch = nullptr;
     // host-buffer batch path (driver callback)
     int batch_mode = CDET_MODE_EXACT;
+    int sum_order = CDET_SUM_TREE;  // cdet_ctx_set_sum_order
     cdet_batch_plan* host_plan = nullptr;
     std::vector<int32_t> host_plan_orders;
     // cdet_eval_batch_submit / wait: two slots, each with its own plan cache,
@@ -1601,7 +1602,11 @@ void refresh_scalars(cdet_ctx* ctx) {
     for (int order = 0; order <= kMaxOrder; ++order) {
         k.group16_R[order] = ctx->tuning.group16_min_terms[order];
         k.group64_R[order] = ctx->tuning.group64_min_terms[order];
+        k.tree_bits[order] = 0;
+        for (int layer = 0; layer <= kMaxOrder; ++layer)
+            k.tree_bits[order] |= static_cast<uint64_t>(tree_lanes_log2(order, layer)) << (3 * layer);
     }
+    k.tree = 0;  // the reference's order; a launch in a non-exact mode sets it on its copy
     for (int cls = 0; cls < 2; ++cls) {
         const int ku = cls == 0 ? 0 : 2;
         k.same_spin[cls] = (k.coeff[ku][0] == k.coeff[ku + 1][0] && k.coeff[ku][1] == k.coeff[ku + 1][1] &&
@@ -1817,8 +1822,13 @@ static_assert(kMixedOrderMax <= kWaveFpmMaxOrder && kMixedOrderMax <= kDetRegMax
 // the kernels lay out every configuration's minors at the stride of order n.
 static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const int32_t* cfg_list, int count,
                          const int64_t* pbase, const int64_t* obase, const double* gup, const double* gdn,
-                         const MinorsOut& o, hipStream_t stream, const int32_t* cfg_order = nullptr) {
+                         const MinorsOut& o, hipStream_t stream, const int32_t* cfg_order = nullptr,
+                         bool tree = false) {
     if (count <= 0) return 0;
+    // tree: the recursion's sums in tree order (tree_layer), for the modes
+    // whose values are not bit-identical to the reference
+    KScalars ks = ctx->ks;
+    ks.tree = tree ? 1 : 0;
     if (cfg_order && (n < 2 || n > kMixedOrderMax || mode == 3))
         return fail(CDET_ERR_ARGUMENT, "launch_minors: a run of mixed orders needs orders 2..7 and modes 0..2");
     int rc = grow(&scratch.ptr, scratch.cap, minors_scratch_bytes(n, count, mode));
@@ -1829,7 +1839,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         const int* popoff3 = ctx->d_popoff + n * (kMaxOrder + 2);
         const int items = 2 * count;
         const int grid = std::min(items, kBoundGrid);
-        hipLaunchKernelGGL(k_bound, dim3(grid), dim3(kBoundThreads), 0, stream, ctx->ks, n, items, cfg_list, pbase,
+        hipLaunchKernelGGL(k_bound, dim3(grid), dim3(kBoundThreads), 0, stream, ks, n, items, cfg_list, pbase,
                            obase, gup, gdn, o.err_up, o.err_dn, o.out, o.out_ord, masks3, popoff3,
                            static_cast<double*>(scratch.ptr), bound_doubles(n));
         HIP_TRY(hipGetLastError());
@@ -1849,7 +1859,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         const size_t lds = MinorsPairLds::bytes(n, mode);
 #define CDET_MINORS_PAIR(NN)                                                                                          \
     case NN:                                                                                                          \
-        hipLaunchKernelGGL(k_minors_pair<NN>, dim3(count), dim3(256), lds, stream, ctx->ks, NN, mode, count,         \
+        hipLaunchKernelGGL(k_minors_pair<NN>, dim3(count), dim3(256), lds, stream, ks, NN, mode, count,         \
                            cfg_list,                                                                                  \
                            pbase, obase, gup, gdn, pm, o.out, o.status, masks, popoff, o.fb_list, o.fb_count,         \
                            o.defer_list, o.defer_count);                                                              \
@@ -1868,10 +1878,10 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         }
 #undef CDET_MINORS_PAIR
         const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
-        hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
+        hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ks, n,
                            cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks, nullptr,
                            n);
-        hipLaunchKernelGGL(k_cdet_pair, dim3(kDeferGrid), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n,
+        hipLaunchKernelGGL(k_cdet_pair, dim3(kDeferGrid), dim3(256), CdetPairLds::bytes(n, mode), stream, ks, n,
                            mode, 0, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, ctx->d_masks, ctx->d_popoff,
                            o.fb_count, o.defer_list, o.defer_count, nullptr, n);
         HIP_TRY(hipGetLastError());
@@ -1882,61 +1892,61 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         if (cfg_order && n > kMixedSmallMax) {
             // subtree depth per configuration
             const size_t lds = sizeof(double) * static_cast<size_t>(fpm_wave_lds_doubles(n));
-            hipLaunchKernelGGL(k_fpm_wave<0>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list, pbase,
+            hipLaunchKernelGGL(k_fpm_wave<0>, dim3(items), dim3(64), lds, stream, ks, n, items, cfg_list, pbase,
                                gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
         } else if (n <= kWaveFpmMaxOrder) {
             // one wavefront per spin matrix, no workgroup barriers
             const size_t lds = sizeof(double) * static_cast<size_t>(fpm_wave_lds_doubles(n));
             switch (n - fpm_wave_L(n)) {
                 case 1:
-                    hipLaunchKernelGGL(k_fpm_wave<1>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                    hipLaunchKernelGGL(k_fpm_wave<1>, dim3(items), dim3(64), lds, stream, ks, n, items, cfg_list,
                                        pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
                 case 2:
-                    hipLaunchKernelGGL(k_fpm_wave<2>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                    hipLaunchKernelGGL(k_fpm_wave<2>, dim3(items), dim3(64), lds, stream, ks, n, items, cfg_list,
                                        pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
                 case 3:
-                    hipLaunchKernelGGL(k_fpm_wave<3>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                    hipLaunchKernelGGL(k_fpm_wave<3>, dim3(items), dim3(64), lds, stream, ks, n, items, cfg_list,
                                        pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
                 default:
-                    hipLaunchKernelGGL(k_fpm_wave<4>, dim3(items), dim3(64), lds, stream, ctx->ks, n, items, cfg_list,
+                    hipLaunchKernelGGL(k_fpm_wave<4>, dim3(items), dim3(64), lds, stream, ks, n, items, cfg_list,
                                        pbase, gup, gdn, pm, o.fb_list, o.fb_count, cfg_order, n);
                     break;
             }
         } else if (fpm_gscr(n)) {
             const int grid = std::min(items, kGscrGrid);
             if (wide_minors(ctx, n))
-                hipLaunchKernelGGL((k_fpm<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, items, cfg_list,
+                hipLaunchKernelGGL((k_fpm<true, 1024>), dim3(grid), dim3(1024), 0, stream, ks, n, items, cfg_list,
                                    pbase, gup, gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
             else
-                hipLaunchKernelGGL((k_fpm<true, 256>), dim3(grid), dim3(256), 0, stream, ctx->ks, n, items, cfg_list,
+                hipLaunchKernelGGL((k_fpm<true, 256>), dim3(grid), dim3(256), 0, stream, ks, n, items, cfg_list,
                                    pbase, gup, gdn, pm, o.fb_list, o.fb_count, next, FpmLds::doubles(n));
             next += static_cast<size_t>(grid) * FpmLds::doubles(n);
         } else {
             if (wide_minors(ctx, n))
-                hipLaunchKernelGGL((k_fpm<false, 1024>), dim3(items), dim3(1024), FpmLds::bytes(n), stream, ctx->ks, n,
+                hipLaunchKernelGGL((k_fpm<false, 1024>), dim3(items), dim3(1024), FpmLds::bytes(n), stream, ks, n,
                                    items, cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
             else
-                hipLaunchKernelGGL((k_fpm<false, 256>), dim3(items), dim3(256), FpmLds::bytes(n), stream, ctx->ks, n,
+                hipLaunchKernelGGL((k_fpm<false, 256>), dim3(items), dim3(256), FpmLds::bytes(n), stream, ks, n,
                                    items, cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
         }
         if (n <= kDetRegMax) {
             const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
-            hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
+            hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ks, n,
                                cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
                                cfg_order, n);
         } else {
             const size_t det_lds = sizeof(double) * (kDetThreads + 1) * static_cast<size_t>(n) * n;
-            hipLaunchKernelGGL(k_fallback_dets<false>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ctx->ks, n,
+            hipLaunchKernelGGL(k_fallback_dets<false>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ks, n,
                                cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
                                cfg_order, n);
         }
     }
     const int paired = (mode != 1 && o.paired) ? 1 : 0;
     if (paired && n <= kPairOrderMax) {
-        hipLaunchKernelGGL(k_cdet_pair, dim3(count), dim3(256), CdetPairLds::bytes(n, mode), stream, ctx->ks, n, mode,
+        hipLaunchKernelGGL(k_cdet_pair, dim3(count), dim3(256), CdetPairLds::bytes(n, mode), stream, ks, n, mode,
                            count, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.status, ctx->d_masks, ctx->d_popoff,
                            o.fb_count, nullptr, nullptr, cfg_order, n);
         HIP_TRY(hipGetLastError());
@@ -1946,22 +1956,22 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     if (cdet_gscr(n, mode)) {
         const int grid = std::min(items, kGscrGrid);
         if (wide_minors(ctx, n))
-            hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ctx->ks, n, mode, items,
+            hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ks, n, mode, items,
                                cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
                                ctx->d_masks, ctx->d_popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired,
                                cfg_order, n);
         else
-            hipLaunchKernelGGL((k_cdet<true, 256>), dim3(grid), dim3(256), 0, stream, ctx->ks, n, mode, items,
+            hipLaunchKernelGGL((k_cdet<true, 256>), dim3(grid), dim3(256), 0, stream, ks, n, mode, items,
                                cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
                                ctx->d_masks, ctx->d_popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired,
                                cfg_order, n);
     } else {
         if (wide_minors(ctx, n))
-            hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ctx->ks,
+            hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
                                o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired, cfg_order, n);
         else
-            hipLaunchKernelGGL((k_cdet<false, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ctx->ks,
+            hipLaunchKernelGGL((k_cdet<false, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ks,
                                n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
                                o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired, cfg_order, n);
     }
@@ -2658,6 +2668,15 @@ int cdet_ctx_set_tuning(cdet_ctx* ctx, const cdet_tuning* in) {
     }
     ctx->tuning = t;
     refresh_scalars(ctx);  // plans made afterwards capture the new thresholds
+    return 0;
+}
+
+int cdet_ctx_set_sum_order(cdet_ctx* ctx, int order) {
+    if (!ctx) return fail(CDET_ERR_ARGUMENT, "null argument");
+    if (order != CDET_SUM_TREE && order != CDET_SUM_REFERENCE)
+        return fail(CDET_ERR_ARGUMENT, "sum order must be CDET_SUM_TREE or CDET_SUM_REFERENCE");
+    ctx->sum_order = order;
+    refresh_scalars(ctx);  // a new generation: captured plan graphs are captured again
     return 0;
 }
 

@@ -265,7 +265,8 @@ static int plan_minors(cdet_batch_plan* p, double* d_out, int32_t* d_status, hip
         o.defer_count = p->d_defer_count + 2 * n;
         int rc = launch_minors(ctx, lane ? p->lane_scratch[lane] : p->scratch, n, p->eval_kind, p->d_lists + off,
                                run.cnt, p->d_pbase, p->d_obase, p->d_gup, p->d_gdn, o, lane ? p->aux[lane - 1] : s,
-                               run.mixed ? p->d_order : nullptr);
+                               run.mixed ? p->d_order : nullptr,
+                               p->contract_mode != CDET_MODE_EXACT && ctx->sum_order == CDET_SUM_TREE);
         if (rc) return rc;
     }
     if (p->lanes > 1) {

@@ -18,6 +18,12 @@ struct KScalars {
     int same_spin[2];                  // class up/down tables and shifts identical
     int group16_R[kMaxOrder + 1];      // per order: chain length from which <= 16-mask layers group 16 lanes
     int group64_R[kMaxOrder + 1];      // per order: chain length from which a single chain takes a wave
+    int tree;                          // recursion sums in tree order (tree_layer), not the reference's
+    uint64_t tree_bits[kMaxOrder + 1];  // per order n: bits 3k..3k+2 = log2 lanes per mask of layer k
+    // Order n's layer shapes when the sums run in tree order, else 0 (a value,
+    // not a pointer into the argument block, which would put the block in
+    // scratch memory).
+    __device__ inline uint64_t tree_row(int n) const { return tree ? tree_bits[n] : 0; }
     const double* coeff[4][2];         // [kind][branch 0 = upper, 1 = lower]
     const double* corner[4][2];
     const double* steps[4][2];         // coeff in matrix-core step order (FP64 MFMA mode)

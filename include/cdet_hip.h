@@ -354,6 +354,23 @@ typedef struct cdet_tuning {
 int cdet_ctx_set_tuning(cdet_ctx* ctx, const cdet_tuning* tuning);
 int cdet_ctx_get_tuning(cdet_ctx* ctx, cdet_tuning* out); /* the values in effect */
 
+/*
+ * Order of the subset recursion's sums in the non-exact modes.  No counterpart
+ * in the reference, whose sums run serially
+ * (include/cdet/integrand_replica_randomly_sampled.h:326-343).
+ * CDET_SUM_TREE (the default): plans in CDET_MODE_MFMA64 and CDET_MODE_SPLIT32
+ * sum the terms of each connected value on several lanes and add the lanes'
+ * sums pairwise, the lanes per mask fixed per order and popcount layer, so the
+ * values do not depend on the launch shapes of cdet_ctx_set_tuning.
+ * CDET_SUM_REFERENCE: the reference's serial order in every mode.
+ * CDET_MODE_EXACT and the plain entry points always sum in the reference's
+ * order.  Plans read the setting when they run; do not change it while runs
+ * on the context are in flight.
+ */
+#define CDET_SUM_TREE 0
+#define CDET_SUM_REFERENCE 1
+int cdet_ctx_set_sum_order(cdet_ctx* ctx, int order);
+
 /* Runtime counters (include/cdet/runtime_counters.h:7-51). */
 int cdet_get_counters(cdet_ctx* ctx, cdet_counters* out);
 int cdet_reset_counters(cdet_ctx* ctx);

@@ -392,6 +392,42 @@ def test_c2_mfma64_within_error_bound(gpu_available, c2):
     assert rel_mf <= 1e-9 and frac_mf > 0.0
 
 
+def test_c2_tree_order_sums(gpu_available, c2):
+    # The non-exact modes sum each connected value's terms in tree order
+    # (cdet_ctx_set_sum_order, default CDET_SUM_TREE).  At order 10 popcount
+    # layers 8, 9 and 10 run 8, 8 and 64 lanes per mask and the layers below
+    # keep the serial chain, so masks of at most seven vertices equal the
+    # reference-order values bit for bit and the others move at round-off.
+    # Both orders stay within twice the first-order bound of the FP64 values;
+    # the exact mode ignores the setting.
+    ctx = c2["ctx"]
+    orders, verts = c2["orders"], c2["verts"]
+    idx = list(np.nonzero(orders == 10)[0])[:256]
+    configs = [verts[i] for i in idx]
+    say(f"c2: {len(configs)} order-10 configurations, tree-order and reference-order sums")
+    ex, _, _ = run_plan(ctx, configs, "exact")
+    bd, _, _ = run_plan(ctx, configs, "exact", kind=3)
+    tree, _, _ = run_plan(ctx, configs, "mfma64")
+    ctx.set_sum_order("reference")
+    try:
+        serial, _, _ = run_plan(ctx, configs, "mfma64")
+        ex_ref, _, _ = run_plan(ctx, configs, "exact")
+    finally:
+        ctx.set_sum_order("tree")
+    assert bits_equal(ex, ex_ref)
+    pop = np.array([bin(m).count("1") for m in range(1 << 10)])
+    t2, s2 = tree.reshape(len(configs), -1), serial.reshape(len(configs), -1)
+    assert bits_equal(t2[:, pop <= 7], s2[:, pop <= 7])
+    moved = np.count_nonzero(t2[:, pop >= 8] != s2[:, pop >= 8])
+    r_tree, rel_tree, _ = bound_report(tree, ex, bd)
+    r_ser, rel_ser, _ = bound_report(serial, ex, bd)
+    say(f"c2 order 10: {moved} of {t2[:, pop >= 8].size} values of popcount >= 8 moved; max err/bound "
+        f"tree {r_tree:.3e} serial {r_ser:.3e}; max rel on resolved values tree {rel_tree:.3e} serial {rel_ser:.3e}")
+    assert moved > 0
+    assert r_tree <= 2.0 and r_ser <= 2.0
+    assert rel_tree <= 1e-9 and rel_ser <= 1e-9
+
+
 def test_c5_mfma64_within_error_bound(gpu_available, c5):
     ctx, orders, verts = c5["ctx"], c5["orders"], c5["verts"]
     idx = pick_per_order(orders, 1, extra={12: 2, 13: 3, 14: 3})

@@ -304,9 +304,14 @@ int cdet_batch_run(cdet_batch_plan* plan, const cdet_vertex* d_vertices, double*
 /*
  * Eager run with HIP events between the three stages, on `stream`: stage_ms[0]
  * = lookup planning and grouping, [1] = coefficient x basis contraction,
- * [2] = minors, subset recursion and assembly.  Synchronises.  For
- * measurement; results are identical to cdet_batch_run.
+ * [2] = minors, subset recursion and assembly.  The contraction (which reads
+ * the grouped lookups and writes the pair values, the same values each time)
+ * is launched CDET_PROFILE_CONTRACT_REPS times back to back between its two
+ * events and [1] is the mean, so that the events' own overhead does not enter
+ * the kernel's duration.  Synchronises.  For measurement; results are
+ * identical to cdet_batch_run.
  */
+#define CDET_PROFILE_CONTRACT_REPS 10
 int cdet_batch_profile(cdet_batch_plan* plan, const cdet_vertex* d_vertices, double* d_out, int32_t* d_status,
                        void* stream, float* stage_ms);
 

@@ -250,79 +250,97 @@ __device__ inline double det_minor_brute(const double* A, int n, uint32_t mask, 
     return sign < 0 ? -det : det;
 }
 
-// det_minor_brute with the k x k working matrix in registers, for n <= NM:
-// every loop runs to the compile-time bound with the rows and columns past k
-// predicated off, and the pivot row swap is a select over all rows, so no
-// register array is indexed at run time.  The arithmetic, pivot choice and
-// early zero are those of det_minor_brute, value for value.
+// det_minor_brute with the k x k working matrix spread over a group of 16
+// lanes (lanes gbase .. gbase + 15 of the wave): lane r holds row r in NM
+// registers, so a wave runs four masks at once at a few dozen registers per
+// lane instead of NM^2 doubles per mask on one lane.  Every lane of the wave
+// must call it (the exchanges are wave-wide); groups whose mask is 0 return 1.
+//   pivot: the reference scans rows col+1.. and takes a row only if its |a|
+//     is strictly greater than the best so far, i.e. the first row of the
+//     largest |a| among those greater than the pivot candidate's (NaN rows
+//     never qualify; a NaN candidate keeps its row).  Each lane offers |a| if
+//     it qualifies, else -1, and a butterfly keeps the larger value, the
+//     lower row on ties;
+//   swap: rows col and prow exchange their entries from column col on;
+//   elimination: each lower row forms f = a_rc / pivot and, if f != 0,
+//     a_rc' = fma(-f, pivot row c', a_rc') for the columns to its right,
+//     with the pivot row broadcast from its lane.
+// The arithmetic and the early zero are those of det_minor_brute, value for
+// value; the determinant (det *= pivot, sign flips) is formed in every lane.
 template <int NM>
-__device__ inline double det_minor_regs(const double* A, int n, uint32_t mask) {
+__device__ inline double det_minor_rows(const double* A, int n, uint32_t mask, int r, int gbase) {
+    static_assert(NM <= 16, "one row per lane of a 16-lane group");
     const int k = __builtin_popcount(mask);
-    if (k == 0) return 1.0;
-    int idx[NM];
+    // row index of this lane and the column indices (the set bits of mask)
+    int cidx[NM];
     {
         uint32_t mm = mask;
 #pragma unroll
-        for (int r = 0; r < NM; ++r) {
-            idx[r] = mm ? __builtin_ctz(mm) : 0;
+        for (int c = 0; c < NM; ++c) {
+            cidx[c] = mm ? __builtin_ctz(mm) : 0;
             mm &= mm - 1u;
         }
     }
-    double W[NM][NM];
+    int ridx = 0;
 #pragma unroll
-    for (int r = 0; r < NM; ++r)
+    for (int c = 0; c < NM; ++c)
+        if (c == r) ridx = cidx[c];
+    const bool row_live = r < k;
+    double W[NM];
 #pragma unroll
-        for (int c = 0; c < NM; ++c) W[r][c] = (r < k && c < k) ? A[idx[r] * n + idx[c]] : 0.0;
+    for (int c = 0; c < NM; ++c) W[c] = (row_live && c < k) ? A[ridx * n + cidx[c]] : 0.0;
     double det = 1.0;
     int sign = 1;
     bool zero = false;
 #pragma unroll
     for (int col = 0; col < NM; ++col) {
-        if (col < k && !zero) {
-            int prow = col;
-            double pabs = fabs(W[col][col]);
+        const bool act = col < k && !zero;  // uniform within the group
+        // pivot candidate |a_col,col| from lane col, then the qualifying rows
+        const double vcol = __shfl(fabs(W[col]), gbase + col);
+        const double v = fabs(W[col]);
+        double best = (act && r > col && r < k && v > vcol) ? v : -1.0;
+        int brow = r;
 #pragma unroll
-            for (int rr = col + 1; rr < NM; ++rr) {
-                const double cand = fabs(W[rr][col]);
-                if (rr < k && cand > pabs) {
-                    pabs = cand;
-                    prow = rr;
-                }
+        for (int o = 1; o < 16; o <<= 1) {
+            const double ob = __shfl_xor(best, o);
+            const int orow = __shfl_xor(brow, o);
+            if (ob > best || (ob == best && orow < brow)) {
+                best = ob;
+                brow = orow;
             }
-            if (pabs == 0.0) {
-                zero = true;
-            } else {
-                if (prow != col) {
+        }
+        const int prow = best >= 0.0 ? brow : col;
+        const double pabs = best >= 0.0 ? best : vcol;
+        if (act && pabs == 0.0) zero = true;
+        const bool go = act && !zero;
+        // rows col and prow exchange their entries from column col on
+        const bool swap = go && prow != col;
+        const int partner = (r == col) ? prow : (r == prow ? col : r);
 #pragma unroll
-                    for (int rr = col + 1; rr < NM; ++rr) {
-                        if (rr == prow) {
+        for (int c = col; c < NM; ++c) {
+            const double o = __shfl(W[c], gbase + partner);
+            if (swap) W[c] = o;
+        }
+        if (swap) sign = -sign;
+        // the pivot row, broadcast from lane col
+        double P[NM];
 #pragma unroll
-                            for (int c = col; c < NM; ++c) {
-                                const double t = W[col][c];
-                                W[col][c] = W[rr][c];
-                                W[rr][c] = t;
-                            }
-                        }
-                    }
-                    sign = -sign;
-                }
-                const double piv = W[col][col];
-                det *= piv;
+        for (int c = col; c < NM; ++c) P[c] = __shfl(W[c], gbase + col);
+        if (go) {
+            const double piv = P[col];
+            det *= piv;
+            if (r > col && r < k) {
+                const double f = W[col] / piv;
+                if (f != 0.0) {
+                    W[col] = 0.0;
 #pragma unroll
-                for (int rr = col + 1; rr < NM; ++rr) {
-                    if (rr < k) {
-                        const double f = W[rr][col] / piv;
-                        if (f != 0.0) {
-                            W[rr][col] = 0.0;
-#pragma unroll
-                            for (int c = col + 1; c < NM; ++c)
-                                if (c < k) W[rr][c] = __builtin_fma(-f, W[col][c], W[rr][c]);
-                        }
-                    }
+                    for (int c = col + 1; c < NM; ++c)
+                        if (c < k) W[c] = __builtin_fma(-f, P[c], W[c]);
                 }
             }
         }
     }
+    if (k == 0) return 1.0;
     if (zero) return 0.0;
     return sign < 0 ? -det : det;
 }

@@ -769,18 +769,22 @@ __global__ __launch_bounds__(NT) void k_fpm(KScalars ks, int n, int items, const
 
 // Pivoted determinants of every principal submatrix of the queued fallback
 // spin matrices (include/cdet/fast_principle_minors.h:39-115, :180-186),
-// spread over a fixed grid: work item = (queue entry, spin, chunk of 64
-// masks), one mask per thread, the thread's k x k working matrix in registers
-// up to order 10 (det_minor_regs) and interleaved in LDS above
-// (det_minor_brute).  Counts the fallbacks
-// (include/cdet/runtime_counters.h:27) and flags the configuration's status.
-// With cfg_order non-null configuration c has order cfg_order[c] <= n (n sizes
-// the chunks and the LDS); minors go to pm[(4 li + 2 class + spin) 2^pm_log +
-// mask].
-// (Working matrices in LDS at every order, in 16-thread workgroups, measured
-// slower: the order-10 minors stage 94 -> 120 us.)
+// spread over a fixed grid.  Up to order 10 (REG): work item = (queue entry,
+// spin, chunk of kDetRowsItem masks), one mask per 16-lane group with one row
+// of the working matrix per lane (det_minor_rows), four masks per wave and
+// pass.  Above: work item = (queue entry, spin, chunk of 64 masks), one mask
+// per thread, the working matrices interleaved in LDS (det_minor_brute).
+// Counts the fallbacks (include/cdet/runtime_counters.h:27) and flags the
+// configuration's status.  With cfg_order non-null configuration c has order
+// cfg_order[c] <= n (n sizes the chunks and the LDS); minors go to pm[(4 li +
+// 2 class + spin) 2^pm_log + mask].
+// (Also measured: the whole k x k matrix in one thread's registers, 256
+// VGPRs: the same C2 rates alone and with four batches in flight; working
+// matrices in LDS at every order, in 16-thread workgroups: the order-10
+// minors stage 94 -> 120 us.)
 constexpr int kDetThreads = 64;
-constexpr int kDetRegMax = 10;  // orders whose fallback determinants run in registers
+constexpr int kDetRegMax = 10;   // orders whose fallback determinants run in row groups
+constexpr int kDetRowsItem = 4;  // masks per work item of the row-group form (one pass)
 template <bool REG>
 __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
     KScalars ks, int n_max, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
@@ -789,7 +793,8 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
     uint64_t* __restrict__ fallback_total, const int32_t* __restrict__ cfg_order, int pm_log) {
     extern __shared__ double lds_dyn[];
     const int tid = threadIdx.x;
-    const int chunks = static_cast<int>(((1u << n_max) + kDetThreads - 1) / kDetThreads);
+    constexpr int kItem = REG ? kDetRowsItem : kDetThreads;  // masks per work item
+    const int chunks = static_cast<int>(((1u << n_max) + kItem - 1) / kItem);
     double* A = lds_dyn;
     double* ws = lds_dyn + n_max * n_max;
     const int total = *fb_count * 2 * chunks;
@@ -822,16 +827,20 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
             A[idx] = v * ks.scale;
         }
         __syncthreads();
-        const uint32_t m = static_cast<uint32_t>(chunk) * kDetThreads + tid;
-        if (m < sc) {
-            double det = 1.0;
-            if (m != 0) {
-                if constexpr (REG)
-                    det = det_minor_regs<kDetRegMax>(A, n, m);
-                else
-                    det = det_minor_brute(A, n, m, ws + tid, kDetThreads);
+        double* dst = pm + (static_cast<size_t>(4 * li + 2 * cls + spin) << pm_log);
+        if constexpr (REG) {
+            // one mask per 16-lane group, every lane of the wave in the call
+            const int r = tid & 15;
+            const int gbase = (tid & 63) - r;
+#pragma unroll
+            for (int p = 0; p < kItem; p += kDetThreads / 16) {
+                const uint32_t m = static_cast<uint32_t>(chunk * kItem + p + (tid >> 4));
+                const double det = det_minor_rows<kDetRegMax>(A, n, m < sc ? m : 0u, r, gbase);
+                if (m < sc && r == 0) dst[m] = det;  // mask 0: 1
             }
-            pm[(static_cast<size_t>(4 * li + 2 * cls + spin) << pm_log) + m] = det;
+        } else {
+            const uint32_t m = static_cast<uint32_t>(chunk) * kItem + tid;
+            if (m < sc) dst[m] = m != 0 ? det_minor_brute(A, n, m, ws + tid, kDetThreads) : 1.0;
         }
     }
 }
@@ -1748,6 +1757,7 @@ static constexpr size_t kMaxLds = 160 * 1024 - 2048;
 // among other batches' kernels (32 against 512: +1.8% configurations/s with
 // four batches in flight).
 static constexpr int kDetGrid = 32;
+static constexpr int kDetGridRows = 256;  // workgroups of the row-group form (90 registers a lane)
 static constexpr int kDeferGrid = 16;  // workgroups of the deferred k_cdet_pair pass
 static constexpr int kGscrGrid = 1024;  // workgroups of the global-scratch forms
 static bool fpm_gscr(int n) { return FpmLds::bytes(n) > kMaxLds; }
@@ -1878,7 +1888,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         }
 #undef CDET_MINORS_PAIR
         const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
-        hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ks, n,
+        hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks, n,
                            cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks, nullptr,
                            n);
         hipLaunchKernelGGL(k_cdet_pair, dim3(kDeferGrid), dim3(256), CdetPairLds::bytes(n, mode), stream, ks, n,
@@ -1934,7 +1944,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         }
         if (n <= kDetRegMax) {
             const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
-            hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ks, n,
+            hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks, n,
                                cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
                                cfg_order, n);
         } else {

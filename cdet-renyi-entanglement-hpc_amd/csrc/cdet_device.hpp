@@ -188,70 +188,16 @@ struct UClenshaw {
 };
 
 // ---------------------------------------------------------------------------
-// Workgroup-cooperative fast principal minors
-// (include/cdet/fast_principle_minors.h:120-246).  On return sm[mask] holds
-// det(A[mask,mask]) for every mask (sm[0] = 1).  All intermediates are formed
-// with the reference's operations; only the iteration order across
+// Principal minors: the pivoted-determinant fallback and the
+// workgroup-cooperative fast principal minors
+// (include/cdet/fast_principle_minors.h:39-246).  All intermediates are
+// formed with the reference's operations; only the iteration order across
 // independent elements differs, which cannot change any value.
-// Returns 1 when a pivot triggered the brute-force fallback (then sm holds the
-// pivoted-determinant minors, include/cdet/fast_principle_minors.h:39-115).
-// Scratch: bufA/bufB hold max_l 2^(l+1)(n-l-1)^2 doubles (and n*n), inv holds
-// 2^(n-2) doubles, red holds blockDim doubles.
 // ---------------------------------------------------------------------------
 // Pivoted determinant of A[mask, mask] with the row elimination of
-// include/cdet/fast_principle_minors.h:39-115.  The k x k working matrix of
-// the calling thread lives at w with element (r, c) at w[(r k + c) ws], so a
-// workgroup's threads interleave their matrices (ws = blockDim) in LDS.
-__device__ inline double det_minor_brute(const double* A, int n, uint32_t mask, double* w, int ws) {
-    const int k = __builtin_popcount(mask);
-    if (k == 0) return 1.0;
-    int r = 0;
-    for (int i = 0; i < n; ++i) {
-        if (!(mask & (1u << i))) continue;
-        int cc = 0;
-        for (int j = 0; j < n; ++j) {
-            if (!(mask & (1u << j))) continue;
-            w[(r * k + cc) * ws] = A[i * n + j];
-            ++cc;
-        }
-        ++r;
-    }
-    auto W = [&](int rr, int c) -> double& { return w[(rr * k + c) * ws]; };
-    double det = 1.0;
-    int sign = 1;
-    for (int col = 0; col < k; ++col) {
-        int prow = col;
-        double pabs = fabs(W(col, col));
-        for (int rr = col + 1; rr < k; ++rr) {
-            const double cand = fabs(W(rr, col));
-            if (cand > pabs) {
-                pabs = cand;
-                prow = rr;
-            }
-        }
-        if (pabs == 0.0) return 0.0;
-        if (prow != col) {
-            for (int c = col; c < k; ++c) {
-                const double t = W(col, c);
-                W(col, c) = W(prow, c);
-                W(prow, c) = t;
-            }
-            sign = -sign;
-        }
-        const double piv = W(col, col);
-        det *= piv;
-        for (int rr = col + 1; rr < k; ++rr) {
-            const double f = W(rr, col) / piv;
-            if (f == 0.0) continue;
-            W(rr, col) = 0.0;
-            for (int c = col + 1; c < k; ++c) W(rr, c) = __builtin_fma(-f, W(col, c), W(rr, c));
-        }
-    }
-    return sign < 0 ? -det : det;
-}
-
-// det_minor_brute with the k x k working matrix spread over a group of 16
-// lanes (lanes gbase .. gbase + 15 of the wave): lane r holds row r in NM
+// include/cdet/fast_principle_minors.h:39-115, the k x k working matrix
+// spread over a group of 16 lanes (lanes gbase .. gbase + 15 of the wave):
+// lane r holds row r in NM
 // registers, so a wave runs four masks at once at a few dozen registers per
 // lane instead of NM^2 doubles per mask on one lane.  Every lane of the wave
 // must call it (the exchanges are wave-wide); groups whose mask is 0 return 1.
@@ -265,8 +211,8 @@ __device__ inline double det_minor_brute(const double* A, int n, uint32_t mask, 
 //   elimination: each lower row forms f = a_rc / pivot and, if f != 0,
 //     a_rc' = fma(-f, pivot row c', a_rc') for the columns to its right,
 //     with the pivot row broadcast from its lane.
-// The arithmetic and the early zero are those of det_minor_brute, value for
-// value; the determinant (det *= pivot, sign flips) is formed in every lane.
+// The arithmetic and the early zero are the reference's, value for value;
+// the determinant (det *= pivot, sign flips) is formed in every lane.
 template <int NM>
 __device__ inline double det_minor_rows(const double* A, int n, uint32_t mask, int r, int gbase) {
     static_assert(NM <= 16, "one row per lane of a 16-lane group");

@@ -769,23 +769,27 @@ __global__ __launch_bounds__(NT) void k_fpm(KScalars ks, int n, int items, const
 
 // Pivoted determinants of every principal submatrix of the queued fallback
 // spin matrices (include/cdet/fast_principle_minors.h:39-115, :180-186),
-// spread over a fixed grid.  Up to order 10 (REG): work item = (queue entry,
-// spin, chunk of kDetRowsItem masks), one mask per 16-lane group with one row
-// of the working matrix per lane (det_minor_rows), four masks per wave and
-// pass.  Above: work item = (queue entry, spin, chunk of 64 masks), one mask
-// per thread, the working matrices interleaved in LDS (det_minor_brute).
+// spread over a fixed grid: work item = (queue entry, spin, chunk of
+// det_rows_item<NM>() masks), one mask per 16-lane group with one row of the
+// working matrix per lane (det_minor_rows), four masks per wave and pass.
+// NM = 10 serves orders up to 10 (90 VGPRs), NM = 14 the orders above.
 // Counts the fallbacks (include/cdet/runtime_counters.h:27) and flags the
 // configuration's status.  With cfg_order non-null configuration c has order
 // cfg_order[c] <= n (n sizes the chunks and the LDS); minors go to pm[(4 li +
 // 2 class + spin) 2^pm_log + mask].
 // (Also measured: the whole k x k matrix in one thread's registers, 256
 // VGPRs: the same C2 rates alone and with four batches in flight; working
-// matrices in LDS at every order, in 16-thread workgroups: the order-10
-// minors stage 94 -> 120 us.)
+// matrices interleaved in LDS, one mask per thread: the order-10 minors stage
+// 94 -> 120 us, and 432 us for an order-14 fallback matrix of the C5 batch.)
 constexpr int kDetThreads = 64;
-constexpr int kDetRegMax = 10;   // orders whose fallback determinants run in row groups
-constexpr int kDetRowsItem = 4;  // masks per work item of the row-group form (one pass)
-template <bool REG>
+constexpr int kDetRegMax = 10;  // orders served by the NM = 10 form
+// masks per work item: one pass of four masks up to order 10, four passes
+// above (their matrices take longer to stage than to use once)
+template <int NM>
+constexpr int det_rows_item() {
+    return NM <= kDetRegMax ? 4 : 16;
+}
+template <int NM>
 __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
     KScalars ks, int n_max, const int32_t* __restrict__ cfg_list, const int64_t* __restrict__ cfg_pbase,
     const double* __restrict__ g_up, const double* __restrict__ g_dn, const int32_t* __restrict__ fb_list,
@@ -793,10 +797,9 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
     uint64_t* __restrict__ fallback_total, const int32_t* __restrict__ cfg_order, int pm_log) {
     extern __shared__ double lds_dyn[];
     const int tid = threadIdx.x;
-    constexpr int kItem = REG ? kDetRowsItem : kDetThreads;  // masks per work item
+    constexpr int kItem = det_rows_item<NM>();
     const int chunks = static_cast<int>(((1u << n_max) + kItem - 1) / kItem);
     double* A = lds_dyn;
-    double* ws = lds_dyn + n_max * n_max;
     const int total = *fb_count * 2 * chunks;
     for (int w = blockIdx.x; w < total; w += gridDim.x) {
         const int e = w / (2 * chunks);
@@ -828,19 +831,14 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
         }
         __syncthreads();
         double* dst = pm + (static_cast<size_t>(4 * li + 2 * cls + spin) << pm_log);
-        if constexpr (REG) {
-            // one mask per 16-lane group, every lane of the wave in the call
-            const int r = tid & 15;
-            const int gbase = (tid & 63) - r;
+        // one mask per 16-lane group, every lane of the wave in the call
+        const int r = tid & 15;
+        const int gbase = (tid & 63) - r;
 #pragma unroll
-            for (int p = 0; p < kItem; p += kDetThreads / 16) {
-                const uint32_t m = static_cast<uint32_t>(chunk * kItem + p + (tid >> 4));
-                const double det = det_minor_rows<kDetRegMax>(A, n, m < sc ? m : 0u, r, gbase);
-                if (m < sc && r == 0) dst[m] = det;  // mask 0: 1
-            }
-        } else {
-            const uint32_t m = static_cast<uint32_t>(chunk) * kItem + tid;
-            if (m < sc) dst[m] = m != 0 ? det_minor_brute(A, n, m, ws + tid, kDetThreads) : 1.0;
+        for (int p = 0; p < kItem; p += kDetThreads / 16) {
+            const uint32_t m = static_cast<uint32_t>(chunk * kItem + p + (tid >> 4));
+            const double det = det_minor_rows<NM>(A, n, m < sc ? m : 0u, r, gbase);
+            if (m < sc && r == 0) dst[m] = det;  // mask 0: 1
         }
     }
 }
@@ -849,7 +847,24 @@ __global__ __launch_bounds__(kDetThreads) void k_fallback_dets(
 // and k (kMaxOrder + 2)).  With cfg_order non-null configuration c has order
 // cfg_order[c] <= n (n sized the LDS or gscratch_stride); the minors of list
 // entry li are at pm + (4 li + 2 class + spin) 2^pm_log.
-template <bool GSCR, int NT>
+// LAY: where the work item's arrays live.
+//   kCdetLayLds    a, dp (rhs), dU, dD and the mask list in LDS;
+//   kCdetLayGlobal all of them in global scratch (gscratch_stride doubles per
+//                  workgroup): orders whose arrays exceed the LDS;
+//   kCdetLaySplit  a (read only once formed) in global scratch, dp, dU and dD
+//                  in LDS, the mask list read from masks_all: order 14 in
+//                  modes 0 and 1, whose 2^14 connected values (128 KB) fit
+//                  the LDS where a and dp together do not.
+constexpr int kCdetLayLds = 0;
+constexpr int kCdetLayGlobal = 1;
+constexpr int kCdetLaySplit = 2;
+struct CdetSplitLds {
+    static __host__ __device__ inline size_t doubles(int n) {
+        return (size_t(1) << n) + 2 * static_cast<size_t>(n);
+    }
+    static __host__ __device__ inline size_t bytes(int n) { return sizeof(double) * doubles(n); }
+};
+template <int LAY, int NT>
 __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, int items,
                                               const int32_t* __restrict__ cfg_list,
                                               const int64_t* __restrict__ cfg_pbase,
@@ -869,6 +884,8 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
     if (fb_count && blockIdx.x == 0 && threadIdx.x == 0) *fb_count = 0;  // wave-wide chain terms of the subset recursion
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    constexpr bool GSCR = LAY != kCdetLayLds;
+    constexpr bool SPLIT = LAY == kCdetLaySplit;
     double* a = GSCR ? gscratch + static_cast<size_t>(blockIdx.x) * gscratch_stride : lds_dyn;
     // paired (modes 0 and 2): one work item per configuration, its two
     // classes in turn, the ordinary class combined into the twisted values in
@@ -882,12 +899,15 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
         const int n = cfg_order ? cfg_order[c] : n_max;
         const uint32_t sc = 1u << n;
         const int n2 = n * n;
-        double* dp = a + sc;
-        double* rhs = dp + sc;  // mode 2 only
+        double* dp = SPLIT ? lds_dyn : a + sc;
+        double* rhs = dp + sc;  // mode 2 only (never split)
         double* dU = rhs + (mode == 2 ? sc : 0);
         double* dD = dU + n;
-        uint16_t* lmask = reinterpret_cast<uint16_t*>(dD + n);
         const uint16_t* masks_by_pop = masks_all + (static_cast<size_t>(n) << kMaxOrder);
+        // the mask list of the recursion: a copy in the work item's arrays, or
+        // the table itself when split
+        uint16_t* lmask = SPLIT ? nullptr : reinterpret_cast<uint16_t*>(dD + n);
+        const uint16_t* lm = SPLIT ? masks_by_pop : lmask;
         for (int k = tid; k < kMaxOrder + 2; k += nt) s_popoff[k] = pop_off_all[n * (kMaxOrder + 2) + k];
         const bool same = ks.same_spin[cls] != 0;
         const int64_t pb = 2 * cfg_pbase[c] + static_cast<int64_t>(cls) * n2;
@@ -920,7 +940,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
                     if (m < sc) {
                         vu[q] = pu[m];
                         vd[q] = pd[m];
-                        mk[q] = masks_by_pop[m];
+                        if (!SPLIT) mk[q] = masks_by_pop[m];
                     }
                 }
 #pragma unroll
@@ -928,7 +948,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
                     const uint32_t m = m0 + q * nt;
                     if (m < sc) {
                         a[m] = (m == 0) ? 0.0 : vu[q] * vd[q];
-                        lmask[m] = mk[q];
+                        if (!SPLIT) lmask[m] = mk[q];
                     }
                 }
             }
@@ -937,7 +957,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
         if (mode == 0 || mode == 2) {
             // mode 2 writes the magnitude scale of each value instead of the value:
             // |prefactor| * (|a[mask]| + sum |terms|), |value| for singletons.
-            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, mode == 2 ? rhs : nullptr, false, tb,
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lm, s_popoff, mode == 2 ? rhs : nullptr, false, tb,
                               ks.group16_R[n], ks.group64_R[n], ks.tree_row(n));
             // Assembly (:833-877).
             // paired: twisted - 2 * ordinary (include/cdet/cdet_integrand_evaluator.h:348-350),
@@ -963,7 +983,7 @@ __global__ __launch_bounds__(NT) void k_cdet(KScalars ks, int n_max, int mode, i
             }
             if (bad && status) atomicOr(&status[c], 1);
         } else {
-            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lmask, s_popoff, nullptr, true, tb, ks.group16_R[n],
+            ztransform_layers(a, dp, n, ks.thr, ks.filter != 0, lm, s_popoff, nullptr, true, tb, ks.group16_R[n],
                               ks.group64_R[n], ks.tree_row(n));
             const double cres = dp[sc - 1];
             if (tid == 0) {
@@ -1751,13 +1771,9 @@ static int noalloc_scratch_of(int device, unsigned char** out) {
 }
 
 static constexpr size_t kMaxLds = 160 * 1024 - 2048;
-// Workgroups of k_fallback_dets: its launches are mostly empty (fallbacks are
-// rare) and each workgroup of the register form needs a quarter of a SIMD's
-// registers, so a small grid that loops over the queue is dispatched sooner
-// among other batches' kernels (32 against 512: +1.8% configurations/s with
-// four batches in flight).
-static constexpr int kDetGrid = 32;
-static constexpr int kDetGridRows = 256;  // workgroups of the row-group form (90 registers a lane)
+// Workgroups of k_fallback_dets (single waves that loop over the queue's
+// work items; its launches are mostly empty, fallbacks being rare).
+static constexpr int kDetGridRows = 256;
 static constexpr int kDeferGrid = 16;  // workgroups of the deferred k_cdet_pair pass
 static constexpr int kGscrGrid = 1024;  // workgroups of the global-scratch forms
 static bool fpm_gscr(int n) { return FpmLds::bytes(n) > kMaxLds; }
@@ -1888,7 +1904,7 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
         }
 #undef CDET_MINORS_PAIR
         const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
-        hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks, n,
+        hipLaunchKernelGGL(k_fallback_dets<kDetRegMax>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks, n,
                            cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks, nullptr,
                            n);
         hipLaunchKernelGGL(k_cdet_pair, dim3(kDeferGrid), dim3(256), CdetPairLds::bytes(n, mode), stream, ks, n,
@@ -1942,17 +1958,15 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
                 hipLaunchKernelGGL((k_fpm<false, 256>), dim3(items), dim3(256), FpmLds::bytes(n), stream, ks, n,
                                    items, cfg_list, pbase, gup, gdn, pm, o.fb_list, o.fb_count, nullptr, 0);
         }
-        if (n <= kDetRegMax) {
-            const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
-            hipLaunchKernelGGL(k_fallback_dets<true>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks, n,
-                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
+        const size_t det_lds = sizeof(double) * static_cast<size_t>(n) * n;
+        if (n <= kDetRegMax)
+            hipLaunchKernelGGL(k_fallback_dets<kDetRegMax>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks,
+                               n, cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
                                cfg_order, n);
-        } else {
-            const size_t det_lds = sizeof(double) * (kDetThreads + 1) * static_cast<size_t>(n) * n;
-            hipLaunchKernelGGL(k_fallback_dets<false>, dim3(kDetGrid), dim3(kDetThreads), det_lds, stream, ks, n,
-                               cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
+        else
+            hipLaunchKernelGGL(k_fallback_dets<kMaxOrder>, dim3(kDetGridRows), dim3(kDetThreads), det_lds, stream, ks,
+                               n, cfg_list, pbase, gup, gdn, o.fb_list, o.fb_count, pm, o.status, ctx->d_fallbacks,
                                cfg_order, n);
-        }
     }
     const int paired = (mode != 1 && o.paired) ? 1 : 0;
     if (paired && n <= kPairOrderMax) {
@@ -1965,26 +1979,38 @@ static int launch_minors(cdet_ctx* ctx, Scratch& scratch, int n, int mode, const
     const int items = paired ? count : 2 * count;
     if (cdet_gscr(n, mode)) {
         const int grid = std::min(items, kGscrGrid);
-        if (wide_minors(ctx, n))
-            hipLaunchKernelGGL((k_cdet<true, 1024>), dim3(grid), dim3(1024), 0, stream, ks, n, mode, items,
-                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
-                               ctx->d_masks, ctx->d_popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired,
-                               cfg_order, n);
-        else
-            hipLaunchKernelGGL((k_cdet<true, 256>), dim3(grid), dim3(256), 0, stream, ks, n, mode, items,
-                               cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status,
-                               ctx->d_masks, ctx->d_popoff, next, CdetLds::doubles(n, mode), o.fb_count, paired,
-                               cfg_order, n);
+        // connected values in LDS, the products in global scratch, where they fit
+        const bool split = mode != 2 && CdetSplitLds::bytes(n) <= kCdetMaxDyn;
+        const size_t lds = split ? CdetSplitLds::bytes(n) : 0;
+#define CDET_GSCR_LAUNCH(LAY, NT)                                                                                    \
+    hipLaunchKernelGGL((k_cdet<LAY, NT>), dim3(grid), dim3(NT), lds, stream, ks, n, mode, items, cfg_list, pbase,     \
+                       obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp, o.status, ctx->d_masks, ctx->d_popoff,   \
+                       next, CdetLds::doubles(n, mode), o.fb_count, paired, cfg_order, n)
+        if (wide_minors(ctx, n)) {
+            if (split)
+                CDET_GSCR_LAUNCH(kCdetLaySplit, 1024);
+            else
+                CDET_GSCR_LAUNCH(kCdetLayGlobal, 1024);
+        } else {
+            if (split)
+                CDET_GSCR_LAUNCH(kCdetLaySplit, 256);
+            else
+                CDET_GSCR_LAUNCH(kCdetLayGlobal, 256);
+        }
+#undef CDET_GSCR_LAUNCH
     } else {
         if (wide_minors(ctx, n))
-            hipLaunchKernelGGL((k_cdet<false, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream, ks,
-                               n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
-                               o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired, cfg_order, n);
+            hipLaunchKernelGGL((k_cdet<kCdetLayLds, 1024>), dim3(items), dim3(1024), CdetLds::bytes(n, mode), stream,
+                               ks, n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord,
+                               o.single_tmp, o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired,
+                               cfg_order, n);
         else
-            hipLaunchKernelGGL((k_cdet<false, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream, ks,
-                               n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord, o.single_tmp,
-                               o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired, cfg_order, n);
+            hipLaunchKernelGGL((k_cdet<kCdetLayLds, 256>), dim3(items), dim3(256), CdetLds::bytes(n, mode), stream,
+                               ks, n, mode, items, cfg_list, pbase, obase, gup, gdn, pm, o.out, o.out_ord,
+                               o.single_tmp, o.status, ctx->d_masks, ctx->d_popoff, nullptr, 0, o.fb_count, paired,
+                               cfg_order, n);
     }
+
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2276,12 +2302,14 @@ static int ctx_init(cdet_ctx* ctx) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fpm<false, 1024>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false, 256>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<kCdetLayLds, 256>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<false, 1024>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<kCdetLayLds, 1024>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_fallback_dets<false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<kCdetLaySplit, 256>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cdet<kCdetLaySplit, 1024>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCdetMaxDyn)));
     rc = noalloc_scratch_of(ctx->device, &ctx->noalloc_scratch);
     if (rc) return rc;
     ctx->tuning = default_tuning();

@@ -428,6 +428,39 @@ def test_c2_tree_order_sums(gpu_available, c2):
     assert rel_tree <= 1e-9 and rel_ser <= 1e-9
 
 
+def test_c2_single_order_mfma64_within_error_bound(gpu_available, c2):
+    # Single-order integrands (kind 1) in the FP64 matrix-core mode, whose
+    # single-anchor recursion sums in tree order: the connected value of the
+    # full mask is formed by the same chains as in the all-subset recursion
+    # (masks holding bit 0 have it as their anchor in both), so each value is
+    # within twice the first-order bound of the all-subset value of the full
+    # mask (kind 3) of the bit-identical FP64 value; the prefactors differ by
+    # their last-bit rounding only.  Orders up to 4 have no tree-order layer:
+    # there the reference order gives the same bits.
+    ctx = c2["ctx"]
+    orders, verts = c2["orders"], c2["verts"]
+    idx = [i for i in pick_per_order(orders, 6, extra={10: 24}) if orders[i] >= 2]
+    configs = [verts[i] for i in idx]
+    say(f"c2: {len(configs)} single-order configurations (orders 2..10), exact / mfma64 / bound")
+    ex, _, _ = run_plan(ctx, configs, "exact", single=True)
+    mf, _, _ = run_plan(ctx, configs, "mfma64", single=True)
+    bd, ob, _ = run_plan(ctx, configs, "exact", kind=3)
+    ctx.set_sum_order("reference")
+    try:
+        serial, _, _ = run_plan(ctx, configs, "mfma64", single=True)
+    finally:
+        ctx.set_sum_order("tree")
+    full = np.array([bd[ob[k] + (1 << len(v)) - 1] for k, v in enumerate(configs)])
+    ratio = np.abs(mf - ex) / np.maximum(2.0 * full + 1e-14 * np.abs(ex), 1e-300)
+    n = np.array([len(v) for v in configs])
+    say(f"c2 single-order: max |mfma64 - exact| / (2 bound) {ratio.max():.3e}; "
+        f"{int(np.count_nonzero(mf[n >= 8] != serial[n >= 8]))} of {int(np.count_nonzero(n >= 8))} "
+        "orders >= 8 moved by the tree-order sums")
+    assert np.all(np.isfinite(mf))
+    assert ratio.max() <= 1.0
+    assert bits_equal(mf[n <= 4], serial[n <= 4])
+
+
 def test_c5_mfma64_within_error_bound(gpu_available, c5):
     ctx, orders, verts = c5["ctx"], c5["orders"], c5["verts"]
     idx = pick_per_order(orders, 1, extra={12: 2, 13: 3, 14: 3})

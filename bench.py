@@ -407,6 +407,61 @@ def cpu_baseline_configs(w, params_gpu, tables, configs, seconds, per_unit=1, un
 
 
 
+def cpu_baseline_lookups(ctx, w, params_gpu, qi, qt, seconds, sample=4096):
+    """The oracle's lookup (descriptor and FP64 Clenshaw, the reference's
+    prepare_green_lookup_descriptor + evaluate_green_pair_descriptor) on all
+    host cores over a seeded sample of the queries, thread k taking queries
+    k, k + cores, ... in turn; lookups/s.  The oracle holds the device
+    coefficient blocks the sample reads (np.zeros maps untouched pages
+    lazily)."""
+    from oracle import oracle as O
+    p = O.make_params(w["N1"], w["N2"], w["nc"], w["beta"], w["U"], w["mu"], w["alpha"], w["alpha"])
+    p.scale, p.diag_up_shift, p.diag_down_shift = (params_gpu.scale, params_gpu.diag_up_shift,
+                                                   params_gpu.diag_down_shift)
+    N1, N2 = w["N1"], w["N2"]
+    pick = np.sort(np.random.default_rng(3).choice(len(qi), sample, replace=False))
+    q, t = qi[pick], qt[pick]
+    sp = ((((q[:, 1].astype(np.int64) * 2 + q[:, 2]) * N1 + q[:, 3]) * N2 + q[:, 4]) * N1 + q[:, 5]) * N2 + q[:, 6]
+    S = N1 * N2
+    SP = 4 * S * S
+    blk = w["nc"] * w["nc"]
+    up, lo, uc, lc = np.zeros(SP * blk), np.zeros(SP * blk), np.zeros(SP), np.zeros(SP)
+    idx = np.unique(sp)
+    bu, bl, cu, cl = ctx.download_blocks(0, idx)
+    for k, s_ in enumerate(idx):
+        up[s_ * blk:(s_ + 1) * blk] = bu[k].ravel()
+        lo[s_ * blk:(s_ + 1) * blk] = bl[k].ravel()
+    uc[idx] = cu
+    lc[idx] = cl
+    tab = O.Table(up, lo, uc, lc)
+    empty = O.Table(np.zeros(SP * blk), np.zeros(SP * blk), np.zeros(SP), np.zeros(SP))
+    tables = [tab, tab, empty, empty]
+    cores, model = cpu_info()
+    counts = [0] * cores
+    stop = time.time() + seconds
+
+    def worker(k):
+        orc = O.Oracle(p, tables)
+        i = k
+        while time.time() < stop:
+            j = i % sample
+            orc.green_pair(0, 1, *(int(x) for x in q[j, 1:7]), float(t[j, 0]), float(t[j, 1]))
+            counts[k] += 1
+            i += cores
+
+    t0 = time.time()
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(cores)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    dt = time.time() - t0
+    m = sum(counts)
+    return {"value": m / dt, "unit": "lookups/s", "cores": cores, "kind": "port", "cpu_model": model,
+            "sample": f"{m} lookups of a seeded sample of {sample} of the 2^20 queries over {dt:.1f} s wall on "
+                      f"{cores} threads"}
+
+
 def main_c3(args):
     """Table evaluation only (configuration C3): lookups/s of the FP64 matrix-
     core contraction with the FP64 Clenshaw form timed beside it, and the
@@ -459,6 +514,53 @@ def main_c3(args):
     rel = np.abs(got - ref) / np.maximum(np.abs(ref), floor)
     flops = n * (2.0 * w["nc"] * w["nc"] + 2.0 * w["nc"])
     value, ms = rates["mfma64"][0], rates["mfma64"][1] * 1e3
+
+    # stage profile of the FP64 matrix-core call (cdet_green_pair_profile: the
+    # contraction as the mean of back-to-back launches between HIP events on
+    # the launch stream).  The contraction takes every lookup but the corner
+    # ones (both times below the branch epsilon) and reads each distinct
+    # (branch, sitepair) block of the twisted table (both spins alias).
+    prof_out = torch.empty(2 * n, dtype=torch.float64, device=dev)
+    ms3 = (ctypes.c_float * 3)()
+    stage = np.zeros(3)
+    reps = 5
+    for _ in range(reps):
+        rc = cdet_amd.lib().cdet_green_pair_profile(ctx.handle, ctypes.c_void_p(d_qi.data_ptr()),
+                                                    ctypes.c_void_p(d_qt.data_ptr()), n,
+                                                    ctypes.c_void_p(prof_out.data_ptr()), cdet_amd.MODE_MFMA64,
+                                                    ctypes.c_void_p(sptr), ms3)
+        if rc != 0:
+            raise RuntimeError(cdet_amd._err())
+        stage += np.array(ms3)
+    stage /= reps
+    upper = qt[:, 0] > qt[:, 1]
+    corner = np.maximum(qt[:, 0], qt[:, 1]) < params.branch_epsilon
+    N1, N2 = w["N1"], w["N2"]
+    sp = ((((qi[:, 1].astype(np.int64) * 2 + qi[:, 2]) * N1 + qi[:, 3]) * N2 + qi[:, 4]) * N1 + qi[:, 5]) * N2 \
+        + qi[:, 6]
+    keys = np.unique((sp * 2 + np.where(upper, 0, 1))[~corner])
+    contracted = int(np.count_nonzero(~corner))
+    cflops = contracted * (2.0 * w["nc"] * w["nc"] + 2.0 * w["nc"])
+    cbytes = float(len(keys)) * w["nc"] * w["nc"] * 8
+    contract_s = stage[1] * 1e-3
+    tflops = cflops / contract_s / 1e12
+    gbs = cbytes / contract_s / 1e9
+    kname = "cdet::k_contract_f64<true, 4>"
+    mfma_frac, hbm_frac = tflops / FP64_PEAK_TFLOPS, gbs / HBM_PEAK_GBS
+    bound = "mfma" if mfma_frac >= hbm_frac else "hbm"
+    roofline = {"bound": bound, "achieved": round(tflops if bound == "mfma" else gbs, 3),
+                "peak": FP64_PEAK_TFLOPS if bound == "mfma" else HBM_PEAK_GBS,
+                "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
+                "frac": round(max(mfma_frac, hbm_frac), 4), "traffic": traffic_from_profiles(kname, "c3"),
+                "kernel": kname, "kernel_ms": round(stage[1], 4), "flops_per_launch": cflops,
+                "algorithmic_bytes_per_launch": cbytes, "distinct_blocks": int(len(keys)),
+                "lookups_per_launch": contracted, "mfma_frac": round(mfma_frac, 4), "hbm_frac": round(hbm_frac, 4),
+                "stage_ms": {"plan_group": round(stage[0], 4), "contract": round(stage[1], 4),
+                             "interleave": round(stage[2], 4)}}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_lookups(ctx, w, params, qi, qt, args.cpu_seconds)
     line = {
         "metric": "C3 table lookups/s (8x8, beta=8, Nc=128, twisted table, FP64 matrix-core vs FP64 Clenshaw)",
         "value": round(value, 1), "unit": "lookups/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -467,16 +569,21 @@ def main_c3(args):
         "config": {"workload": "8x8 Hubbard, beta=8, 4x4 corner subsystem, 128x128 Chebyshev; 2^20 seeded "
                                "twisted-table lookups per step (planning, grouping, contraction)",
                    "table_build_s": round(build_s, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "tuning": tuning_rec,
         "fp64_clenshaw_lookups_per_s": round(rates["exact"][0], 1),
         "speedup_vs_fp64_clenshaw": round(rates["mfma64"][0] / rates["exact"][0], 2),
         "whole_call_fp64_tflops": round(flops / rates["mfma64"][1] / 1e12, 3),
         "rel_err_vs_fp64_clenshaw": {"max": float(np.max(rel)), "mean": float(np.mean(rel))},
     }
+    if args.copy_probe:  # 256 MiB device copy: calibration of the HBM counters
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     ctx.close()
-
-
 
 
 def run_sweeps(ctx, w, envelopes, sweeps, order_max, groups=1):

@@ -177,6 +177,7 @@ def _signatures():
         "cdet_batch_profile": (i, [vp, vp, vp, vp, vp, P(ctypes.c_float)]),
         "cdet_eval_batch": (i, [vp, vp, P(ctypes.c_int32), i64, vp, vp, i, vp]),
         "cdet_green_pair_eval": (i, [vp, vp, vp, i64, vp, i, vp]),
+        "cdet_green_pair_profile": (i, [vp, vp, vp, i64, vp, i, vp, P(ctypes.c_float)]),
         "cdet_eval_batch_host": (i, [vp, vp, P(ctypes.c_int32), i64, dp]),
         "cdet_eval_batch_submit": (i, [vp, i, vp, P(ctypes.c_int32), i64]),
         "cdet_eval_batch_wait": (i, [vp, i, dp]),

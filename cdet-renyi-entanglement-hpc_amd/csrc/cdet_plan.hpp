@@ -634,8 +634,10 @@ int cdet_eval_batch(cdet_ctx* ctx, const cdet_vertex* d_vertices, const int32_t*
 // Table evaluation only (cdet_green_pair_eval): lookups grouped by coefficient
 // block and contracted by the same kernels as the integrand batches.
 // ---------------------------------------------------------------------------
-extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus, int64_t count,
-                                    double* out, int mode, void* stream) {
+// ev non-null: events ev[0..3] around planning and grouping, the contraction
+// (CDET_PROFILE_CONTRACT_REPS launches back to back) and the interleave.
+static int green_pair_run(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus, int64_t count,
+                          double* out, int mode, void* stream, hipEvent_t* ev) {
     if (!ctx || !query_ints || !query_taus || !out || count <= 0)
         return fail(CDET_ERR_ARGUMENT, "cdet_green_pair_eval: bad argument");
     if (!tables_ready(ctx)) return fail(CDET_ERR_TABLES, "Resident Green-function tables have not been uploaded");
@@ -682,6 +684,7 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     }
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     HIP_TRY(hipStreamWaitEvent(s, q.done, 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     const unsigned gq = static_cast<unsigned>((count + 255) / 256);
     hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>((nkeys * kKeySub + 255) / 256)), dim3(256), 0, s,
                        q.hist, static_cast<int64_t>(nkeys) * kKeySub, 0);
@@ -694,18 +697,46 @@ extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, co
     hipLaunchKernelGGL(k_scatter, dim3(gq), dim3(256), 0, s, count, q.qkey, q.qrank, q.qu, q.qv, q.keyoff, q.su, q.sv,
                        q.sdest);
     const int grid = contract_grid(ctx, max_tiles, mode, tile);
-    if (mode == CDET_MODE_EXACT) {
-        hipLaunchKernelGGL(k_contract_exact, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, ctx->ks, q.ntiles,
-                           q.tkey, q.tstart, q.tlen, q.su, q.sv, q.sdest, q.gup, q.gdn);
-    } else {
-        int rc = contract_launch_mfma(ctx->ks, mode, static_cast<int>(grid), q.ntiles, q.tkey, q.tstart, q.tlen, q.su,
-                                      q.sv, q.sdest, q.gup, q.gdn, s, tile);
-        if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s));
+    for (int r = 0; r < (ev ? CDET_PROFILE_CONTRACT_REPS : 1); ++r) {
+        if (mode == CDET_MODE_EXACT) {
+            hipLaunchKernelGGL(k_contract_exact, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, ctx->ks,
+                               q.ntiles, q.tkey, q.tstart, q.tlen, q.su, q.sv, q.sdest, q.gup, q.gdn);
+        } else {
+            int rc = contract_launch_mfma(ctx->ks, mode, static_cast<int>(grid), q.ntiles, q.tkey, q.tstart, q.tlen,
+                                          q.su, q.sv, q.sdest, q.gup, q.gdn, s, tile);
+            if (rc) return rc;
+        }
     }
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s));
     hipLaunchKernelGGL(k_interleave, dim3(gq), dim3(256), 0, s, count, q.gup, q.gdn, out);
     HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[3], s));
     HIP_TRY(hipEventRecord(q.done, s));
     return 0;
+}
+
+extern "C" int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus, int64_t count,
+                                    double* out, int mode, void* stream) {
+    return green_pair_run(ctx, query_ints, query_taus, count, out, mode, stream, nullptr);
+}
+
+extern "C" int cdet_green_pair_profile(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus,
+                                       int64_t count, double* out, int mode, void* stream, float* stage_ms) {
+    if (!stage_ms) return fail(CDET_ERR_ARGUMENT, "cdet_green_pair_profile: bad argument");
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int rc = 0;
+    for (auto& e : ev)
+        if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(CDET_ERR_DEVICE, "hipEventCreate failed");
+    if (!rc) rc = green_pair_run(ctx, query_ints, query_taus, count, out, mode, stream, ev);
+    if (!rc && hipEventSynchronize(ev[3]) != hipSuccess) rc = fail(CDET_ERR_DEVICE, "hipEventSynchronize failed");
+    for (int k = 0; !rc && k < 3; ++k)
+        if (hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]) != hipSuccess)
+            rc = fail(CDET_ERR_DEVICE, "hipEventElapsedTime failed");
+    if (!rc) stage_ms[1] /= CDET_PROFILE_CONTRACT_REPS;
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 extern "C" int cdet_tables_build(cdet_ctx* ctx, const cdet_lattice* lattice) {

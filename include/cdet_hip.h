@@ -330,6 +330,16 @@ int cdet_green_pair_eval(cdet_ctx* ctx, const int32_t* query_ints, const double*
                          int64_t count, double* out, int mode, void* stream);
 
 /*
+ * cdet_green_pair_eval with HIP events between its stages, on `stream`:
+ * stage_ms[0] = lookup planning and grouping, [1] = the coefficient x basis
+ * contraction (mean of CDET_PROFILE_CONTRACT_REPS launches back to back),
+ * [2] = the interleave into out.  Synchronises.  No counterpart in the
+ * reference; for measurement, with the same results as cdet_green_pair_eval.
+ */
+int cdet_green_pair_profile(cdet_ctx* ctx, const int32_t* query_ints, const double* query_taus,
+                            int64_t count, double* out, int mode, void* stream, float* stage_ms);
+
+/*
  * Launch-shape tuning of a context.  No counterpart in the reference, whose
  * only work-split knob is its integrand thread pool
  * (include/cdet/integrand_parallel.h:34-154).  The package's autotuner

@@ -647,7 +647,18 @@ def test_green_pair_eval_large_batch_tiles(gpu_available):
         torch.cuda.synchronize()
         res[mode] = out.cpu().numpy()
     assert np.max(np.abs(res["mfma64"] - res["exact"])) < 1e-13
-
+    # the stage profile of the same call (cdet_green_pair_profile) gives the
+    # same values bit for bit and positive stage times
+    for mode in ("exact", "mfma64"):
+        out = torch.full((2 * n,), float("nan"), dtype=torch.float64, device="cuda")
+        ms = (ctypes.c_float * 3)()
+        rc = cdet_amd.lib().cdet_green_pair_profile(ctx.handle, ctypes.c_void_p(d_qi.data_ptr()),
+                                                    ctypes.c_void_p(d_qt.data_ptr()), n,
+                                                    ctypes.c_void_p(out.data_ptr()), cdet_amd.MODES[mode], None, ms)
+        assert rc == 0, cdet_amd._err()
+        say(f"green pair profile {mode}: stages {list(ms)} ms")
+        assert np.array_equal(out.cpu().numpy().view(np.int64), res[mode].view(np.int64))
+        assert all(x > 0.0 for x in ms)
 
 
 def test_plain_entry_points_from_concurrent_threads(gpu_available):

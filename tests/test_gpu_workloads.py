@@ -478,3 +478,13 @@ def test_c5_mfma64_within_error_bound(gpu_available, c5):
             f"(resolved fraction {frac:.4f})")
         assert r_mf <= 2.0, (i, orders[i])
         assert rel_mf <= 1e-9, (i, orders[i])
+    # single-order integrands of the same configurations in the matrix-core
+    # mode (order 14: the connected values in LDS and the products in global
+    # scratch, tree-order sums) against the bound of the full mask, as at C2
+    ex1, _, _ = run_plan(ctx, configs, "exact", single=True)
+    mf1, _, _ = run_plan(ctx, configs, "mfma64", single=True)
+    full = np.array([bd[ob[k + 1] - 1] for k in range(len(configs))])
+    ratio = np.abs(mf1 - ex1) / np.maximum(2.0 * full + 1e-14 * np.abs(ex1), 1e-300)
+    say(f"c5 single-order: max |mfma64 - exact| / (2 bound) {ratio.max():.3e}")
+    assert np.all(np.isfinite(mf1))
+    assert ratio.max() <= 1.0

@@ -108,8 +108,8 @@ typedef struct cdet_params {
 
 /* Evaluation modes of cdet_eval_batch. */
 #define CDET_MODE_EXACT 0     /* FP64 Clenshaw in the reference's order: bit-identical */
-#define CDET_MODE_MFMA64 1    /* FP64 MFMA coefficient x basis contraction */
-#define CDET_MODE_SPLIT32 2   /* split-FP32 MFMA (hi/lo, three products), FP64 combine */
+#define CDET_MODE_MFMA64 1    /* FP64 MFMA coefficient x basis contraction, tree-order recursion sums */
+#define CDET_MODE_SPLIT32 2   /* split-FP32 MFMA (hi/lo, three products), FP64 combine, tree-order sums */
 
 typedef struct cdet_counters {
     uint64_t integrand_calls;     /* include/cdet/runtime_counters.h:14 */

@@ -754,6 +754,11 @@ def main_c4(args):
         "pass_criterion_vs_fp64": criterion,
         "green_pair_reuse": reuse_rec,
     }
+    if args.copy_probe:  # 256 MiB device copy: calibration of the HBM counters
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     ctx.close()
 
